@@ -1,0 +1,164 @@
+/*
+ * msim/msim.h — C-ABI of the MI355X (gfx950) per-tick agent pipeline.
+ *
+ * This is the drop-in boundary for movement-sim's compute-shader tick.  The
+ * reference builds a Kompute pipeline from random_move.comp, binds seven
+ * storage buffers and calls it twice per tick; every entry point below names
+ * the reference call it replaces.  All functions are plain C: POD pointers and
+ * sizes in, an int status out (MSIM_OK == 0), no exceptions across the ABI.
+ * On failure msim_last_error(handle) (or msim_get_last_error() when no handle
+ * exists yet) returns a message.
+ *
+ * Ownership: the caller owns every host array it passes; msim_create copies
+ * them into device memory owned by the handle.  Readbacks write into caller
+ * buffers.  A handle is bound to one device and one HIP stream and is not
+ * thread-safe (the reference drives its pipeline from one worker thread,
+ * src/sim/Simulator.cpp:159-211); distinct handles may be used from distinct
+ * threads.
+ */
+#ifndef MSIM_MSIM_H
+#define MSIM_MSIM_H
+
+#include "types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    MSIM_OK = 0,
+    MSIM_ERR_INVALID = 1,  /* bad argument or malformed map / agent data */
+    MSIM_ERR_HIP = 2,      /* a HIP runtime call failed */
+    MSIM_ERR_OOM = 3,      /* device allocation failed */
+    MSIM_ERR_STATE = 4,    /* call not valid in the handle's current state */
+    MSIM_ERR_TIMEOUT = 5   /* host watchdog expired waiting for the stream */
+};
+
+typedef struct msim_sim msim_sim;
+
+/* Tuning knobs; zero-initialise and set struct_size = sizeof(msim_options).
+ * Every zero field selects the default noted beside it. */
+typedef struct msim_options {
+    uint32_t struct_size;
+    float cell_size;           /* 0: smallest power of two >= collisionRadius  */
+    uint32_t table_log2;       /* 0: auto (about one bucket per agent)         */
+    uint32_t block_size;       /* 0: 256 threads (4 wave64s)                   */
+    uint32_t reorder_interval; /* 0: never; k: spatially renumber every k ticks*/
+    uint32_t graph_ticks;      /* 0: 32 ticks per captured hipGraph            */
+    uint32_t history_capacity; /* 0: 4096 entries of per-tick pair counts      */
+    int32_t device;            /* -1 (or 0 with struct_size 0): current device */
+    uint32_t watchdog_ms;      /* 0: 600000 ms host-side wait limit            */
+} msim_options;
+
+typedef struct msim_info {
+    uint32_t n_agents;
+    uint32_t n_roads;
+    uint32_t n_connections;
+    uint32_t tick;             /* PushConsts.tick after the last dispatch     */
+    uint64_t ticks_simulated;  /* completed move dispatches                   */
+    float cell_size;
+    float collision_radius;
+    uint32_t table_width;      /* toroidal bucket table, buckets per row      */
+    uint32_t table_height;
+    uint32_t uniform_speed;    /* 1 when no per-agent speed array was given   */
+    uint32_t n_uninitialized;  /* agents whose Entity.initialized is still 0  */
+    uint64_t device_bytes;     /* device memory held by the handle            */
+} msim_info;
+
+/* Replaces Simulator::init's buffer/pipeline creation (src/sim/Simulator.cpp:
+ * 44-106: mgr->tensor x7, mgr->algorithm) and the one-time upload
+ * OpTensorSyncDevice (:189-193).
+ *   roads/n_roads, connections/n_connections: the Map buffers bound at
+ *     random_move.comp:41-42.  Every endpoint group must satisfy
+ *     connectedIndex + connectedCount < n_connections (one entry after the
+ *     group is readable, as the shader reads connections[idx + count] when the
+ *     draw returns the top offset), and every entry must be < n_roads.
+ *   entities/n_entities: the Entity buffer (binding 0).
+ *   speed: per-agent speed in m/tick, or NULL for the shader's constant 1.4f
+ *     (random_move.comp:750).
+ *   pc: PushConsts (PushConsts.hpp:9-20); tick is the value before the first
+ *     dispatch (the reference starts at 1, Simulator.cpp:101).
+ *   opt: NULL for defaults. */
+int msim_create(const msim_road* roads, uint32_t n_roads,
+                const uint32_t* connections, uint32_t n_connections,
+                const msim_entity* entities, uint32_t n_entities,
+                const float* speed, const msim_push_consts* pc,
+                const msim_options* opt, msim_sim** out);
+
+/* Replaces one calcSeq->eval<kp::OpAlgoDispatch>(algo, pushConsts)
+ * (Simulator.cpp:224 and :235).  pc->tick selects the phase as the shader's
+ * main() does (random_move.comp:860-879): even = move (advance, route choice,
+ * spatial-index rebuild), odd = collision (neighbour pass).  Agents whose
+ * Entity.initialized is 0 only enter the spatial index and become initialised
+ * (random_move.comp:863-867).  Asynchronous on the handle's stream. */
+int msim_dispatch(msim_sim* sim, const msim_push_consts* pc);
+
+/* n_ticks iterations of Simulator::sim_tick (Simulator.cpp:213-279) without
+ * the per-tick readbacks: each is tick+=1 -> dispatch, tick+=1 -> dispatch.
+ * Runs as replays of a captured hipGraph of fused ticks.  Asynchronous. */
+int msim_run_ticks(msim_sim* sim, uint32_t n_ticks);
+
+/* Blocks until the handle's stream drains (bounded by the watchdog). */
+int msim_sync(msim_sim* sim);
+
+/* Replaces OpTensorSyncLocal({tensorEntities}) + tensor->vector<Entity>()
+ * (Simulator.cpp:197,248-263): writes entities [first, first+count) in the
+ * 64-byte layout.  color is (0,0,1,1) for an agent that was in at least one
+ * pair in the last collision dispatch and (0,1,0,1) otherwise
+ * (random_move.comp:545-548,876), or the uploaded colour before any
+ * collision dispatch.  Synchronous. */
+int msim_read_entities(msim_sim* sim, msim_entity* out, uint32_t first, uint32_t count);
+
+/* Replaces the debugData readback (Simulator.cpp:199,273).  out[0] counts
+ * spatial-index insertions (initial inserts plus cell changes; the
+ * reference's quadtree count at random_move.comp:311 is implementation
+ * specific), out[1] the cumulative neighbour-pair count modulo 2^32
+ * (random_move.comp:548), out[2..9] are 0.  Synchronous. */
+int msim_read_debug(msim_sim* sim, uint32_t out[10]);
+
+/* Structure-of-arrays readback in agent order; any pointer may be NULL.
+ * pos/target/direction: 2 floats per agent; rand_state: 4 words per agent;
+ * neighbour_count: neighbours (both ends of every pair) found for each agent
+ * in the last collision dispatch.  Synchronous. */
+int msim_read_state(msim_sim* sim, float* pos, float* target, float* direction,
+                    uint32_t* road_index, uint32_t* rand_state, uint32_t* neighbour_count);
+
+/* Cumulative pair total (64-bit) and per-tick pair counts of the most recent
+ * min(n, history) collision dispatches, oldest first; *n_out receives the
+ * number written.  Synchronous. */
+int msim_read_pairs(msim_sim* sim, uint64_t* total, uint32_t* per_tick, uint32_t n, uint32_t* n_out);
+
+/* Spatial-index export, the uniform-grid counterpart of get_quad_tree_nodes
+ * (Simulator.cpp:149-153; node layout GpuQuadTree.hpp:14-41).  cell_xy gets
+ * the integer cell (floor(x/cell), floor(y/cell)) of every agent as binned by
+ * the last move dispatch (2 ints per agent).  Synchronous. */
+int msim_read_cells(msim_sim* sim, int32_t* cell_xy);
+
+int msim_get_info(msim_sim* sim, msim_info* out);
+
+/* Time n_ticks fused ticks on the handle's stream with HIP events.
+ * ms_total receives the elapsed time; kernel_ms (may be NULL) receives the
+ * summed duration of each kernel of the tick, measured by bracketing every
+ * launch with events in a second, separate pass of n_ticks ticks, in the
+ * order advance, scan-reduce, scan-partials, scan-down, scatter, collide
+ * (n_kernel_ms entries at most). */
+int msim_time_ticks(msim_sim* sim, uint32_t n_ticks, float* ms_total,
+                    float* kernel_ms, uint32_t n_kernel_ms);
+
+/* Stream the handle enqueues on (a hipStream_t), for callers that order
+ * their own work after it. */
+int msim_get_stream(msim_sim* sim, void** stream);
+
+int msim_destroy(msim_sim* sim);
+
+const char* msim_last_error(const msim_sim* sim);
+const char* msim_get_last_error(void);
+
+/* Library build identification: "msim <version> gfx950 fp-contract=off". */
+const char* msim_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MSIM_MSIM_H */

@@ -1,0 +1,764 @@
+// msim_plain.hip — C-ABI of the plain (one scenario per handle) pipeline.
+//
+// Replaces the Kompute pipeline that Simulator drives
+// (src/sim/Simulator.cpp:44-106 creation, :189-199 upload/sequences,
+// :213-279 the two dispatches per tick and the readbacks).  See
+// include/msim/msim.h for the contract of each entry point.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "msim/msim.h"
+#include "msim_kernels.hpp"
+#include "msim_host.hpp"
+
+using namespace msim;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct Buffers {
+    float2* pos = nullptr;
+    float2* tgt = nullptr;
+    float2* dir = nullptr;
+    float* spd = nullptr;
+    uint32_t* road = nullptr;
+    uint4* rng = nullptr;
+    uint32_t* init = nullptr;
+    uint32_t* ncount = nullptr;
+    uint32_t* perm = nullptr;
+};
+
+}  // namespace
+
+struct msim_sim {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t n = 0;
+    uint32_t n_roads = 0;
+    uint32_t n_conns = 0;
+    msim_road* roads = nullptr;
+    uint32_t* conns = nullptr;
+    Buffers buf[2];
+    int cur = 0;
+    bool uniform = true;
+    float spd_const = kShaderSpeed;
+    GridDev grid{};
+    Counters* ctr = nullptr;
+    uint32_t* hist = nullptr;
+    uint32_t hist_cap = 0;
+    msim_push_consts pc{};
+    uint32_t n_uninit = 0;
+    bool any_collide = false;
+    std::vector<msim_rgba> color0;
+    uint64_t ticks_simulated = 0;
+    uint32_t reorder_interval = 0;
+    uint32_t since_reorder = 0;
+    uint32_t graph_ticks = 32;
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    uint32_t watchdog_ms = 600000;
+    uint64_t bytes = 0;
+    std::vector<void*> allocs;
+    std::string err;
+
+    AgentsDev agents(int which) const {
+        const Buffers& b = buf[which];
+        return AgentsDev{n, b.pos, b.tgt, b.dir, uniform ? nullptr : b.spd, spd_const,
+                         b.road, b.rng, b.init, b.ncount, b.perm};
+    }
+};
+
+namespace {
+
+int fail(msim_sim* s, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (s) {
+        s->err = buf;
+    }
+    g_last_error = buf;
+    return code;
+}
+
+#define MSIM_TRY_HIP(s, call)                                                             \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess) {                                                           \
+            return fail((s), MSIM_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
+        }                                                                                 \
+    } while (0)
+
+int dalloc(msim_sim* s, void** p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) {
+        bytes = 16;
+    }
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return fail(s, MSIM_ERR_OOM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
+    s->allocs.push_back(*p);
+    s->bytes += bytes;
+    return MSIM_OK;
+}
+
+template <class T>
+int dalloc_n(msim_sim* s, T** p, size_t count) {
+    return dalloc(s, reinterpret_cast<void**>(p), count * sizeof(T));
+}
+
+uint32_t ceil_div(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+
+uint32_t agent_blocks(uint32_t n) { return std::max(1u, ceil_div(n, kBlock)); }
+
+int check_launch(msim_sim* s, const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        return fail(s, MSIM_ERR_HIP, "launch of %s failed: %s", what, hipGetErrorString(e));
+    }
+    return MSIM_OK;
+}
+
+template <int MODE>
+void launch_advance_mode(msim_sim* s, bool write_dir) {
+    const AgentsDev a = s->agents(s->cur);
+    const dim3 g(agent_blocks(s->n));
+    if (s->uniform) {
+        if (write_dir) {
+            hipLaunchKernelGGL((k_advance_bin<MODE, true, true>), g, dim3(kBlock), 0, s->stream, a,
+                               s->grid, s->roads, s->conns, s->ctr);
+        } else {
+            hipLaunchKernelGGL((k_advance_bin<MODE, true, false>), g, dim3(kBlock), 0, s->stream, a,
+                               s->grid, s->roads, s->conns, s->ctr);
+        }
+    } else {
+        if (write_dir) {
+            hipLaunchKernelGGL((k_advance_bin<MODE, false, true>), g, dim3(kBlock), 0, s->stream, a,
+                               s->grid, s->roads, s->conns, s->ctr);
+        } else {
+            hipLaunchKernelGGL((k_advance_bin<MODE, false, false>), g, dim3(kBlock), 0, s->stream, a,
+                               s->grid, s->roads, s->conns, s->ctr);
+        }
+    }
+}
+
+// Move dispatch: advance + bucket table rebuild.  ev (optional) receives an
+// event record after each of the 5 launches.
+int enqueue_move(msim_sim* s, int mode, bool write_dir, hipEvent_t* ev = nullptr) {
+    switch (mode) {
+        case kMoveAll:
+            launch_advance_mode<kMoveAll>(s, write_dir);
+            break;
+        case kMoveInit:
+            launch_advance_mode<kMoveInit>(s, write_dir);
+            break;
+        default:
+            launch_advance_mode<kBinOnly>(s, false);
+            break;
+    }
+    if (ev) (void)hipEventRecord(ev[0], s->stream);
+    const GridDev& g = s->grid;
+    const uint32_t n_part = g.m / kScanTile;
+    hipLaunchKernelGGL(k_scan_reduce, dim3(n_part), dim3(kBlock), 0, s->stream, g.cnt, g.part);
+    if (ev) (void)hipEventRecord(ev[1], s->stream);
+    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kPartialsThreads), 0, s->stream, g.part, n_part,
+                       g.start + g.m);
+    if (ev) (void)hipEventRecord(ev[2], s->stream);
+    hipLaunchKernelGGL(k_scan_down, dim3(n_part), dim3(kBlock), 0, s->stream, g.cnt, g.part, g.start);
+    if (ev) (void)hipEventRecord(ev[3], s->stream);
+    hipLaunchKernelGGL(k_scatter, dim3(agent_blocks(s->n)), dim3(kBlock), 0, s->stream,
+                       s->agents(s->cur), g);
+    if (ev) (void)hipEventRecord(ev[4], s->stream);
+    return check_launch(s, "move dispatch");
+}
+
+int enqueue_collide(msim_sim* s, bool slow, hipEvent_t* ev = nullptr) {
+    const AgentsDev a = s->agents(s->cur);
+    const dim3 g(agent_blocks(s->n));
+    if (slow) {
+        hipLaunchKernelGGL(k_collide<true>, g, dim3(kBlock), 0, s->stream, a, s->grid, s->ctr, s->hist,
+                           s->hist_cap - 1);
+        hipLaunchKernelGGL(k_set_init, g, dim3(kBlock), 0, s->stream, a.init, s->n);
+    } else {
+        hipLaunchKernelGGL(k_collide<false>, g, dim3(kBlock), 0, s->stream, a, s->grid, s->ctr, s->hist,
+                           s->hist_cap - 1);
+    }
+    if (ev) (void)hipEventRecord(ev[0], s->stream);
+    return check_launch(s, "collision dispatch");
+}
+
+void drop_graphs(msim_sim* s) {
+    for (auto& g : s->gexec) {
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    }
+}
+
+int reorder(msim_sim* s) {
+    const int nxt = 1 - s->cur;
+    hipLaunchKernelGGL(k_reorder, dim3(agent_blocks(s->n)), dim3(kBlock), 0, s->stream, s->agents(s->cur),
+                       s->agents(nxt), s->grid);
+    int rc = check_launch(s, "k_reorder");
+    if (rc != MSIM_OK) {
+        return rc;
+    }
+    s->cur = nxt;
+    s->since_reorder = 0;
+    return MSIM_OK;
+}
+
+int wait_stream(msim_sim* s) {
+    const auto t0 = std::chrono::steady_clock::now();
+    for (;;) {
+        hipError_t e = hipStreamQuery(s->stream);
+        if (e == hipSuccess) {
+            return MSIM_OK;
+        }
+        if (e != hipErrorNotReady) {
+            return fail(s, MSIM_ERR_HIP, "stream failed: %s", hipGetErrorString(e));
+        }
+        const auto ms = std::chrono::duration_cast<std::chrono::milliseconds>(
+                            std::chrono::steady_clock::now() - t0)
+                            .count();
+        if (ms > static_cast<long long>(s->watchdog_ms)) {
+            return fail(s, MSIM_ERR_TIMEOUT, "stream did not drain within %u ms", s->watchdog_ms);
+        }
+        std::this_thread::sleep_for(std::chrono::microseconds(ms < 10 ? 20 : 500));
+    }
+}
+
+int build_graph(msim_sim* s) {
+    hipGraph_t graph = nullptr;
+    MSIM_TRY_HIP(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    int rc = MSIM_OK;
+    for (uint32_t t = 0; t < s->graph_ticks && rc == MSIM_OK; t++) {
+        rc = enqueue_move(s, kMoveAll, false);
+        if (rc == MSIM_OK) {
+            rc = enqueue_collide(s, false);
+        }
+    }
+    hipError_t e = hipStreamEndCapture(s->stream, &graph);
+    if (rc != MSIM_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) {
+        return fail(s, MSIM_ERR_HIP, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
+    }
+    e = hipGraphInstantiate(&s->gexec[s->cur], graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        s->gexec[s->cur] = nullptr;
+        return fail(s, MSIM_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+    }
+    return MSIM_OK;
+}
+
+int after_move(msim_sim* s) {
+    s->ticks_simulated++;
+    s->since_reorder++;
+    if (s->reorder_interval && s->since_reorder >= s->reorder_interval) {
+        return reorder(s);
+    }
+    return MSIM_OK;
+}
+
+int set_radius(msim_sim* s, float r) {
+    if (!(r > 0.0f) || !std::isfinite(r)) {
+        return fail(s, MSIM_ERR_INVALID, "collisionRadius must be positive and finite (got %g)", (double)r);
+    }
+    if (r > 1.0f / s->grid.inv_cell) {
+        return fail(s, MSIM_ERR_INVALID,
+                    "collisionRadius %g exceeds the grid cell %g chosen at creation", (double)r,
+                    (double)(1.0f / s->grid.inv_cell));
+    }
+    if (r != s->grid.radius) {
+        s->grid.radius = r;
+        s->grid.d2_limit = d2_limit_for(r);
+        drop_graphs(s);
+    }
+    return MSIM_OK;
+}
+
+int single_dispatch(msim_sim* s, uint32_t tick) {
+    s->pc.tick = tick;
+    int rc;
+    if ((tick & 1u) == 0u) {
+        rc = enqueue_move(s, s->n_uninit ? kMoveInit : kMoveAll, true);
+        s->n_uninit = 0;
+        if (rc == MSIM_OK) rc = after_move(s);
+    } else {
+        rc = enqueue_collide(s, s->n_uninit != 0);
+        s->n_uninit = 0;
+        s->any_collide = true;
+    }
+    return rc;
+}
+
+template <class T>
+int download(msim_sim* s, std::vector<T>& host, const T* dev, size_t count) {
+    host.resize(count);
+    if (count == 0) return MSIM_OK;
+    MSIM_TRY_HIP(s, hipMemcpyAsync(host.data(), dev, count * sizeof(T), hipMemcpyDeviceToHost, s->stream));
+    return MSIM_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+
+extern "C" int msim_create(const msim_road* roads, uint32_t n_roads, const uint32_t* connections,
+                           uint32_t n_connections, const msim_entity* entities, uint32_t n_entities,
+                           const float* speed, const msim_push_consts* pc, const msim_options* opt,
+                           msim_sim** out) {
+    if (!out) {
+        return fail(nullptr, MSIM_ERR_INVALID, "out handle pointer is NULL");
+    }
+    *out = nullptr;
+    if (!pc) {
+        return fail(nullptr, MSIM_ERR_INVALID, "push constants are NULL");
+    }
+    if ((n_roads && !roads) || (n_connections && !connections) || (n_entities && !entities)) {
+        return fail(nullptr, MSIM_ERR_INVALID, "NULL array with non-zero length");
+    }
+    std::string why;
+    if (!validate_map(roads, n_roads, connections, n_connections, &why)) {
+        return fail(nullptr, MSIM_ERR_INVALID, "%s", why.c_str());
+    }
+    if (!validate_entities(entities, n_entities, n_roads, speed, &why)) {
+        return fail(nullptr, MSIM_ERR_INVALID, "%s", why.c_str());
+    }
+    msim_options o{};
+    if (opt) {
+        if (opt->struct_size != sizeof(msim_options)) {
+            return fail(nullptr, MSIM_ERR_INVALID, "msim_options.struct_size must be %zu",
+                        sizeof(msim_options));
+        }
+        o = *opt;
+    } else {
+        o.device = -1;
+    }
+    const float radius = pc->collisionRadius;
+    if (!(radius > 0.0f) || !std::isfinite(radius)) {
+        return fail(nullptr, MSIM_ERR_INVALID, "collisionRadius must be positive and finite");
+    }
+    float cell = o.cell_size;
+    if (cell == 0.0f) {
+        cell = pow2_cell(radius);
+    } else if (!is_pow2_float(cell) || cell < radius) {
+        return fail(nullptr, MSIM_ERR_INVALID, "cell_size must be a power of two >= collisionRadius");
+    }
+    const uint32_t block = o.block_size ? o.block_size : kBlock;
+    if (block != kBlock) {
+        return fail(nullptr, MSIM_ERR_INVALID, "block_size must be %u in this build", kBlock);
+    }
+    uint32_t tlog = o.table_log2 ? o.table_log2 : auto_table_log2(n_entities);
+    if (tlog < 12 || tlog > 28) {
+        return fail(nullptr, MSIM_ERR_INVALID, "table_log2 must be in [12, 28]");
+    }
+
+    msim_sim* s = new msim_sim();
+    s->n = n_entities;
+    s->n_roads = n_roads;
+    s->n_conns = n_connections;
+    s->pc = *pc;
+    s->reorder_interval = o.reorder_interval;
+    s->graph_ticks = o.graph_ticks ? o.graph_ticks : 32;
+    s->watchdog_ms = o.watchdog_ms ? o.watchdog_ms : 600000;
+    s->hist_cap = next_pow2_u32(o.history_capacity ? o.history_capacity : 4096);
+    s->uniform = (speed == nullptr);
+
+    int rc = MSIM_OK;
+    if (o.device >= 0) {
+        hipError_t e = hipSetDevice(o.device);
+        if (e != hipSuccess) {
+            delete s;
+            return fail(nullptr, MSIM_ERR_HIP, "hipSetDevice(%d): %s", o.device, hipGetErrorString(e));
+        }
+    }
+    if (hipGetDevice(&s->device) != hipSuccess) {
+        delete s;
+        return fail(nullptr, MSIM_ERR_HIP, "no HIP device available");
+    }
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete s;
+        return fail(nullptr, MSIM_ERR_HIP, "hipStreamCreate failed");
+    }
+
+    const uint32_t n = n_entities;
+    GridDev& g = s->grid;
+    g.inv_cell = 1.0f / cell;
+    g.radius = radius;
+    g.d2_limit = d2_limit_for(radius);
+    g.lw = (tlog + 1) / 2;
+    g.wmask = (1u << g.lw) - 1u;
+    g.hmask = (1u << (tlog - g.lw)) - 1u;
+    g.m = 1u << tlog;
+
+#define ALLOC(p, cnt)                                   \
+    if (rc == MSIM_OK) rc = dalloc_n(s, &(p), (cnt));
+    ALLOC(s->roads, n_roads);
+    ALLOC(s->conns, n_connections);
+    for (int k = 0; k < 2; k++) {
+        Buffers& b = s->buf[k];
+        ALLOC(b.pos, n);
+        ALLOC(b.tgt, n);
+        ALLOC(b.dir, n);
+        if (!s->uniform) ALLOC(b.spd, n);
+        ALLOC(b.road, n);
+        ALLOC(b.rng, n);
+        ALLOC(b.init, n);
+        ALLOC(b.ncount, n);
+        ALLOC(b.perm, n);
+    }
+    ALLOC(g.cnt, g.m);
+    ALLOC(g.start, g.m + 1);
+    ALLOC(g.part, g.m / kScanTile);
+    ALLOC(g.rank, n);
+    ALLOC(g.bpos, n);
+    ALLOC(g.bid, n);
+    ALLOC(s->ctr, 1);
+    ALLOC(s->hist, s->hist_cap);
+#undef ALLOC
+    if (rc != MSIM_OK) {
+        std::string e = s->err;
+        msim_destroy(s);
+        return fail(nullptr, rc, "%s", e.c_str());
+    }
+
+    // Entity AoS -> SoA.
+    std::vector<float2> pos(n), tgt(n), dir(n);
+    std::vector<uint32_t> road(n), init(n), perm(n), zeros(n, 0u);
+    std::vector<uint4> rng(n);
+    s->color0.resize(n);
+    for (uint32_t i = 0; i < n; i++) {
+        const msim_entity& e = entities[i];
+        pos[i] = make_float2(e.pos.x, e.pos.y);
+        tgt[i] = make_float2(e.target.x, e.target.y);
+        dir[i] = make_float2(e.direction.x, e.direction.y);
+        road[i] = e.roadIndex;
+        rng[i] = make_uint4(e.randomState.x, e.randomState.y, e.randomState.z, e.randomState.w);
+        init[i] = e.initialized ? 1u : 0u;
+        s->n_uninit += init[i] ? 0u : 1u;
+        perm[i] = i;
+        s->color0[i] = e.color;
+    }
+    auto up = [&](void* dst, const void* src, size_t bytes) {
+        if (rc == MSIM_OK && bytes) {
+            hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+            if (e != hipSuccess) rc = fail(s, MSIM_ERR_HIP, "upload failed: %s", hipGetErrorString(e));
+        }
+    };
+    Buffers& b0 = s->buf[0];
+    up(s->roads, roads, sizeof(msim_road) * n_roads);
+    up(s->conns, connections, sizeof(uint32_t) * n_connections);
+    up(b0.pos, pos.data(), sizeof(float2) * n);
+    up(b0.tgt, tgt.data(), sizeof(float2) * n);
+    up(b0.dir, dir.data(), sizeof(float2) * n);
+    if (!s->uniform) up(b0.spd, speed, sizeof(float) * n);
+    up(b0.road, road.data(), sizeof(uint32_t) * n);
+    up(b0.rng, rng.data(), sizeof(uint4) * n);
+    up(b0.init, init.data(), sizeof(uint32_t) * n);
+    up(b0.ncount, zeros.data(), sizeof(uint32_t) * n);
+    up(b0.perm, perm.data(), sizeof(uint32_t) * n);
+    if (rc == MSIM_OK) {
+        hipError_t e = hipMemset(g.cnt, 0, sizeof(uint32_t) * g.m);
+        if (e == hipSuccess) e = hipMemset(s->ctr, 0, sizeof(Counters));
+        if (e == hipSuccess) e = hipMemset(s->hist, 0, sizeof(uint32_t) * s->hist_cap);
+        if (e != hipSuccess) rc = fail(s, MSIM_ERR_HIP, "hipMemset failed: %s", hipGetErrorString(e));
+    }
+    // Initial binning, then renumber agents into bucket order for locality.
+    if (rc == MSIM_OK) rc = enqueue_move(s, kBinOnly, false);
+    if (rc == MSIM_OK) rc = reorder(s);
+    if (rc == MSIM_OK) rc = wait_stream(s);
+    if (rc != MSIM_OK) {
+        std::string e = s->err;
+        msim_destroy(s);
+        return fail(nullptr, rc, "%s", e.c_str());
+    }
+    *out = s;
+    return MSIM_OK;
+}
+
+extern "C" int msim_dispatch(msim_sim* s, const msim_push_consts* pc) {
+    if (!s || !pc) return fail(s, MSIM_ERR_INVALID, "NULL argument");
+    int rc = set_radius(s, pc->collisionRadius);
+    if (rc != MSIM_OK) return rc;
+    s->pc = *pc;
+    return single_dispatch(s, pc->tick);
+}
+
+extern "C" int msim_run_ticks(msim_sim* s, uint32_t n_ticks) {
+    if (!s) return fail(s, MSIM_ERR_INVALID, "NULL handle");
+    uint32_t done = 0;
+    int rc = MSIM_OK;
+    while (done < n_ticks && rc == MSIM_OK) {
+        const uint32_t t1 = s->pc.tick + 1;
+        if (s->n_uninit != 0 || (t1 & 1u) != 0u) {
+            rc = single_dispatch(s, t1);
+            if (rc == MSIM_OK) rc = single_dispatch(s, t1 + 1);
+            done++;
+            continue;
+        }
+        const uint32_t left = n_ticks - done;
+        const uint32_t until_reorder =
+            s->reorder_interval ? s->reorder_interval - s->since_reorder : UINT32_MAX;
+        if (s->graph_ticks > 0 && left - 1 >= s->graph_ticks && until_reorder >= s->graph_ticks) {
+            if (!s->gexec[s->cur]) {
+                rc = build_graph(s);
+                if (rc != MSIM_OK) break;
+            }
+            hipError_t e = hipGraphLaunch(s->gexec[s->cur], s->stream);
+            if (e != hipSuccess) return fail(s, MSIM_ERR_HIP, "hipGraphLaunch: %s", hipGetErrorString(e));
+            s->pc.tick += 2 * s->graph_ticks;
+            s->ticks_simulated += s->graph_ticks;
+            s->since_reorder += s->graph_ticks;
+            s->any_collide = true;
+            done += s->graph_ticks;
+            if (s->reorder_interval && s->since_reorder >= s->reorder_interval) rc = reorder(s);
+        } else {
+            // Direction is output-only state: write it on the final tick.
+            rc = enqueue_move(s, kMoveAll, left == 1);
+            if (rc == MSIM_OK) rc = enqueue_collide(s, false);
+            s->pc.tick += 2;
+            s->any_collide = true;
+            done++;
+            if (rc == MSIM_OK) rc = after_move(s);
+        }
+    }
+    return rc;
+}
+
+extern "C" int msim_sync(msim_sim* s) {
+    if (!s) return fail(s, MSIM_ERR_INVALID, "NULL handle");
+    return wait_stream(s);
+}
+
+namespace {
+
+struct HostState {
+    std::vector<float2> pos, tgt, dir;
+    std::vector<uint32_t> road, init, ncount, perm;
+    std::vector<uint4> rng;
+};
+
+int fetch_state(msim_sim* s, HostState& h, bool need_dir, bool need_rng) {
+    const Buffers& b = s->buf[s->cur];
+    int rc = MSIM_OK;
+    if (rc == MSIM_OK) rc = download(s, h.pos, b.pos, s->n);
+    if (rc == MSIM_OK) rc = download(s, h.tgt, b.tgt, s->n);
+    if (rc == MSIM_OK && need_dir) rc = download(s, h.dir, b.dir, s->n);
+    if (rc == MSIM_OK) rc = download(s, h.road, b.road, s->n);
+    if (rc == MSIM_OK && need_rng) rc = download(s, h.rng, b.rng, s->n);
+    if (rc == MSIM_OK) rc = download(s, h.init, b.init, s->n);
+    if (rc == MSIM_OK) rc = download(s, h.ncount, b.ncount, s->n);
+    if (rc == MSIM_OK) rc = download(s, h.perm, b.perm, s->n);
+    if (rc == MSIM_OK) rc = wait_stream(s);
+    return rc;
+}
+
+}  // namespace
+
+extern "C" int msim_read_entities(msim_sim* s, msim_entity* out, uint32_t first, uint32_t count) {
+    if (!s || (count && !out)) return fail(s, MSIM_ERR_INVALID, "NULL argument");
+    if (static_cast<uint64_t>(first) + count > s->n) {
+        return fail(s, MSIM_ERR_INVALID, "range [%u, %u) exceeds %u agents", first, first + count, s->n);
+    }
+    HostState h;
+    int rc = fetch_state(s, h, true, true);
+    if (rc != MSIM_OK) return rc;
+    for (uint32_t k = 0; k < s->n; k++) {
+        const uint32_t i = h.perm[k];
+        if (i < first || i >= first + count) continue;
+        msim_entity& e = out[i - first];
+        if (s->any_collide) {
+            e.color = h.ncount[k] ? msim_rgba{0.f, 0.f, 1.f, 1.f} : msim_rgba{0.f, 1.f, 0.f, 1.f};
+        } else {
+            e.color = s->color0[i];
+        }
+        e.randomState = msim_vec4u{h.rng[k].x, h.rng[k].y, h.rng[k].z, h.rng[k].w};
+        e.pos = msim_vec2{h.pos[k].x, h.pos[k].y};
+        e.target = msim_vec2{h.tgt[k].x, h.tgt[k].y};
+        e.direction = msim_vec2{h.dir[k].x, h.dir[k].y};
+        e.roadIndex = h.road[k];
+        e.initialized = h.init[k];
+    }
+    return MSIM_OK;
+}
+
+extern "C" int msim_read_state(msim_sim* s, float* pos, float* target, float* direction,
+                               uint32_t* road_index, uint32_t* rand_state, uint32_t* neighbour_count) {
+    if (!s) return fail(s, MSIM_ERR_INVALID, "NULL handle");
+    HostState h;
+    int rc = fetch_state(s, h, direction != nullptr, rand_state != nullptr);
+    if (rc != MSIM_OK) return rc;
+    for (uint32_t k = 0; k < s->n; k++) {
+        const uint32_t i = h.perm[k];
+        if (pos) {
+            pos[2 * i] = h.pos[k].x;
+            pos[2 * i + 1] = h.pos[k].y;
+        }
+        if (target) {
+            target[2 * i] = h.tgt[k].x;
+            target[2 * i + 1] = h.tgt[k].y;
+        }
+        if (direction) {
+            direction[2 * i] = h.dir[k].x;
+            direction[2 * i + 1] = h.dir[k].y;
+        }
+        if (road_index) road_index[i] = h.road[k];
+        if (rand_state) {
+            rand_state[4 * i] = h.rng[k].x;
+            rand_state[4 * i + 1] = h.rng[k].y;
+            rand_state[4 * i + 2] = h.rng[k].z;
+            rand_state[4 * i + 3] = h.rng[k].w;
+        }
+        if (neighbour_count) neighbour_count[i] = h.ncount[k];
+    }
+    return MSIM_OK;
+}
+
+extern "C" int msim_read_debug(msim_sim* s, uint32_t out[10]) {
+    if (!s || !out) return fail(s, MSIM_ERR_INVALID, "NULL argument");
+    Counters c{};
+    MSIM_TRY_HIP(s, hipMemcpyAsync(&c, s->ctr, sizeof(c), hipMemcpyDeviceToHost, s->stream));
+    int rc = wait_stream(s);
+    if (rc != MSIM_OK) return rc;
+    std::memset(out, 0, sizeof(uint32_t) * 10);
+    out[0] = static_cast<uint32_t>(c.inserts);
+    out[1] = static_cast<uint32_t>(c.pairs_total);
+    return MSIM_OK;
+}
+
+extern "C" int msim_read_pairs(msim_sim* s, uint64_t* total, uint32_t* per_tick, uint32_t n,
+                               uint32_t* n_out) {
+    if (!s) return fail(s, MSIM_ERR_INVALID, "NULL handle");
+    Counters c{};
+    std::vector<uint32_t> hist(s->hist_cap);
+    MSIM_TRY_HIP(s, hipMemcpyAsync(&c, s->ctr, sizeof(c), hipMemcpyDeviceToHost, s->stream));
+    MSIM_TRY_HIP(s, hipMemcpyAsync(hist.data(), s->hist, sizeof(uint32_t) * s->hist_cap,
+                                   hipMemcpyDeviceToHost, s->stream));
+    int rc = wait_stream(s);
+    if (rc != MSIM_OK) return rc;
+    if (total) *total = c.pairs_total;
+    const uint64_t avail = std::min<uint64_t>(c.collides, s->hist_cap);
+    const uint32_t k = static_cast<uint32_t>(std::min<uint64_t>(avail, per_tick ? n : 0));
+    for (uint32_t q = 0; q < k; q++) {
+        const uint64_t idx = c.collides - k + q;
+        per_tick[q] = hist[idx & (s->hist_cap - 1)];
+    }
+    if (n_out) *n_out = k;
+    return MSIM_OK;
+}
+
+extern "C" int msim_read_cells(msim_sim* s, int32_t* cell_xy) {
+    if (!s || (s->n && !cell_xy)) return fail(s, MSIM_ERR_INVALID, "NULL argument");
+    if (s->n == 0) return MSIM_OK;
+    int2* d = nullptr;
+    MSIM_TRY_HIP(s, hipMallocAsync(reinterpret_cast<void**>(&d), sizeof(int2) * s->n, s->stream));
+    hipLaunchKernelGGL(k_export_cells, dim3(agent_blocks(s->n)), dim3(kBlock), 0, s->stream,
+                       s->agents(s->cur), s->grid.inv_cell, d);
+    int rc = check_launch(s, "k_export_cells");
+    if (rc == MSIM_OK) {
+        hipError_t e = hipMemcpyAsync(cell_xy, d, sizeof(int2) * s->n, hipMemcpyDeviceToHost, s->stream);
+        if (e != hipSuccess) rc = fail(s, MSIM_ERR_HIP, "cell readback: %s", hipGetErrorString(e));
+    }
+    (void)hipFreeAsync(d, s->stream);
+    if (rc == MSIM_OK) rc = wait_stream(s);
+    return rc;
+}
+
+extern "C" int msim_get_info(msim_sim* s, msim_info* out) {
+    if (!s || !out) return fail(s, MSIM_ERR_INVALID, "NULL argument");
+    out->n_agents = s->n;
+    out->n_roads = s->n_roads;
+    out->n_connections = s->n_conns;
+    out->tick = s->pc.tick;
+    out->ticks_simulated = s->ticks_simulated;
+    out->cell_size = 1.0f / s->grid.inv_cell;
+    out->collision_radius = s->grid.radius;
+    out->table_width = s->grid.wmask + 1;
+    out->table_height = s->grid.hmask + 1;
+    out->uniform_speed = s->uniform ? 1u : 0u;
+    out->n_uninitialized = s->n_uninit;
+    out->device_bytes = s->bytes;
+    return MSIM_OK;
+}
+
+extern "C" int msim_time_ticks(msim_sim* s, uint32_t n_ticks, float* ms_total, float* kernel_ms,
+                               uint32_t n_kernel_ms) {
+    if (!s) return fail(s, MSIM_ERR_INVALID, "NULL handle");
+    if (s->n_uninit != 0) return fail(s, MSIM_ERR_STATE, "agents are not initialised yet; dispatch first");
+    if ((s->pc.tick + 1) & 1u) return fail(s, MSIM_ERR_STATE, "next dispatch is not a move dispatch");
+    constexpr int kNk = 6;
+    hipEvent_t ev[kNk + 1];
+    for (auto& e : ev) MSIM_TRY_HIP(s, hipEventCreate(&e));
+    int rc = MSIM_OK;
+    if (ms_total) {
+        (void)hipEventRecord(ev[0], s->stream);
+        rc = msim_run_ticks(s, n_ticks);
+        (void)hipEventRecord(ev[1], s->stream);
+        if (rc == MSIM_OK) rc = wait_stream(s);
+        if (rc == MSIM_OK) (void)hipEventElapsedTime(ms_total, ev[0], ev[1]);
+    }
+    if (rc == MSIM_OK && kernel_ms && n_kernel_ms) {
+        double acc[kNk] = {0, 0, 0, 0, 0, 0};
+        for (uint32_t t = 0; t < n_ticks && rc == MSIM_OK; t++) {
+            (void)hipEventRecord(ev[0], s->stream);
+            rc = enqueue_move(s, kMoveAll, t + 1 == n_ticks, ev + 1);
+            if (rc == MSIM_OK) rc = enqueue_collide(s, false, ev + 6);
+            s->pc.tick += 2;
+            s->any_collide = true;
+            if (rc == MSIM_OK) rc = after_move(s);
+            if (rc == MSIM_OK) rc = wait_stream(s);
+            for (int k = 0; k < kNk && rc == MSIM_OK; k++) {
+                float ms = 0.f;
+                (void)hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
+                acc[k] += ms;
+            }
+        }
+        for (uint32_t k = 0; k < n_kernel_ms && k < static_cast<uint32_t>(kNk); k++) {
+            kernel_ms[k] = static_cast<float>(acc[k]);
+        }
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    return rc;
+}
+
+extern "C" int msim_get_stream(msim_sim* s, void** stream) {
+    if (!s || !stream) return fail(s, MSIM_ERR_INVALID, "NULL argument");
+    *stream = s->stream;
+    return MSIM_OK;
+}
+
+extern "C" int msim_destroy(msim_sim* s) {
+    if (!s) return MSIM_OK;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    drop_graphs(s);
+    for (void* p : s->allocs) (void)hipFree(p);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+    return MSIM_OK;
+}
+
+extern "C" const char* msim_last_error(const msim_sim* s) { return s ? s->err.c_str() : g_last_error.c_str(); }
+
+extern "C" const char* msim_get_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" const char* msim_version(void) { return "msim 0.1.0 gfx950 fp-contract=off"; }

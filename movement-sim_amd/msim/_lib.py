@@ -1,0 +1,235 @@
+"""ctypes binding of libmsim.so, the C-ABI declared in include/msim/msim.h.
+
+The library is the only compute path: if it has not been built, loading
+raises instead of falling back to anything else.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from .types import ENTITY_DTYPE, PUSH_DTYPE, ROAD_DTYPE
+
+PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_ROOT, "lib", "libmsim.so")
+
+MSIM_OK = 0
+STATUS_NAMES = {1: "MSIM_ERR_INVALID", 2: "MSIM_ERR_HIP", 3: "MSIM_ERR_OOM", 4: "MSIM_ERR_STATE",
+                5: "MSIM_ERR_TIMEOUT"}
+
+
+class MsimError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Options(ctypes.Structure):
+    _fields_ = [
+        ("struct_size", ctypes.c_uint32),
+        ("cell_size", ctypes.c_float),
+        ("table_log2", ctypes.c_uint32),
+        ("block_size", ctypes.c_uint32),
+        ("reorder_interval", ctypes.c_uint32),
+        ("graph_ticks", ctypes.c_uint32),
+        ("history_capacity", ctypes.c_uint32),
+        ("device", ctypes.c_int32),
+        ("watchdog_ms", ctypes.c_uint32),
+    ]
+
+
+class Info(ctypes.Structure):
+    _fields_ = [
+        ("n_agents", ctypes.c_uint32),
+        ("n_roads", ctypes.c_uint32),
+        ("n_connections", ctypes.c_uint32),
+        ("tick", ctypes.c_uint32),
+        ("ticks_simulated", ctypes.c_uint64),
+        ("cell_size", ctypes.c_float),
+        ("collision_radius", ctypes.c_float),
+        ("table_width", ctypes.c_uint32),
+        ("table_height", ctypes.c_uint32),
+        ("uniform_speed", ctypes.c_uint32),
+        ("n_uninitialized", ctypes.c_uint32),
+        ("device_bytes", ctypes.c_uint64),
+    ]
+
+
+# Every symbol include/msim/msim.h declares, with its ctypes signature.
+_VP = ctypes.c_void_p
+_U32 = ctypes.c_uint32
+SIGNATURES = {
+    "msim_create": (ctypes.c_int, [_VP, _U32, _VP, _U32, _VP, _U32, _VP, _VP, _VP, ctypes.POINTER(_VP)]),
+    "msim_dispatch": (ctypes.c_int, [_VP, _VP]),
+    "msim_run_ticks": (ctypes.c_int, [_VP, _U32]),
+    "msim_sync": (ctypes.c_int, [_VP]),
+    "msim_read_entities": (ctypes.c_int, [_VP, _VP, _U32, _U32]),
+    "msim_read_debug": (ctypes.c_int, [_VP, _VP]),
+    "msim_read_state": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP, _VP, _VP]),
+    "msim_read_pairs": (ctypes.c_int, [_VP, _VP, _VP, _U32, _VP]),
+    "msim_read_cells": (ctypes.c_int, [_VP, _VP]),
+    "msim_get_info": (ctypes.c_int, [_VP, _VP]),
+    "msim_time_ticks": (ctypes.c_int, [_VP, _U32, _VP, _VP, _U32]),
+    "msim_get_stream": (ctypes.c_int, [_VP, ctypes.POINTER(_VP)]),
+    "msim_destroy": (ctypes.c_int, [_VP]),
+    "msim_last_error": (ctypes.c_char_p, [_VP]),
+    "msim_get_last_error": (ctypes.c_char_p, []),
+    "msim_version": (ctypes.c_char_p, []),
+}
+
+_lib = None
+
+
+def load_library(path=None):
+    """Load libmsim.so once; raise if it is missing (there is no fallback)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(
+            f"{p} is not built: run __graft_entry__.build() (or make -C movement-sim_amd/csrc) first")
+    lib = ctypes.CDLL(p)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def make_options(**kw):
+    o = Options()
+    o.struct_size = ctypes.sizeof(Options)
+    o.device = -1
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class Simulation:
+    """One handle of the plain pipeline (the HIP drop-in for the shader tick).
+
+    roads: ROAD_DTYPE array, connections: uint32 array, entities: ENTITY_DTYPE
+    array, speed: float32 array or None (constant 1.4 m/tick), pc: PUSH_DTYPE.
+    """
+
+    def __init__(self, roads, connections, entities, pc, speed=None, **options):
+        self.lib = load_library()
+        roads = np.ascontiguousarray(roads, ROAD_DTYPE)
+        conns = np.ascontiguousarray(connections, np.uint32)
+        ents = np.ascontiguousarray(entities, ENTITY_DTYPE)
+        pcv = np.ascontiguousarray(np.asarray(pc, PUSH_DTYPE).reshape(()))
+        spd = None if speed is None else np.ascontiguousarray(speed, np.float32)
+        if spd is not None and spd.shape[0] != ents.shape[0]:
+            raise ValueError("speed must have one entry per agent")
+        opts = make_options(**options) if options else None
+        h = ctypes.c_void_p()
+        rc = self.lib.msim_create(_ptr(roads), roads.shape[0], _ptr(conns), conns.shape[0], _ptr(ents),
+                                  ents.shape[0], _ptr(spd), _ptr(pcv),
+                                  None if opts is None else ctypes.byref(opts), ctypes.byref(h))
+        if rc != MSIM_OK:
+            raise MsimError(rc, self.lib.msim_get_last_error().decode())
+        self.h = h
+        self.n = ents.shape[0]
+        self.pc = pcv.copy()
+
+    def _check(self, rc):
+        if rc != MSIM_OK:
+            raise MsimError(rc, self.lib.msim_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.msim_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- driving -----------------------------------------------------------
+    def dispatch(self, tick=None):
+        """One shader dispatch (even tick = move, odd = collision)."""
+        if tick is None:
+            tick = int(self.pc["tick"]) + 1
+        self.pc["tick"] = tick
+        self._check(self.lib.msim_dispatch(self.h, _ptr(self.pc)))
+
+    def sim_tick(self):
+        """Simulator::sim_tick: tick+=1 -> dispatch, tick+=1 -> dispatch."""
+        self.dispatch()
+        self.dispatch()
+
+    def run_ticks(self, n):
+        self._check(self.lib.msim_run_ticks(self.h, n))
+        self.pc["tick"] = self.info()["tick"]
+
+    def sync(self):
+        self._check(self.lib.msim_sync(self.h))
+
+    # -- readback ----------------------------------------------------------
+    def entities(self, first=0, count=None):
+        count = self.n - first if count is None else count
+        out = np.zeros(count, ENTITY_DTYPE)
+        self._check(self.lib.msim_read_entities(self.h, _ptr(out), first, count))
+        return out
+
+    def state(self):
+        n = self.n
+        st = {
+            "pos": np.zeros((n, 2), np.float32),
+            "tgt": np.zeros((n, 2), np.float32),
+            "dir": np.zeros((n, 2), np.float32),
+            "road": np.zeros(n, np.uint32),
+            "rng": np.zeros((n, 4), np.uint32),
+            "ncount": np.zeros(n, np.uint32),
+        }
+        self._check(self.lib.msim_read_state(self.h, _ptr(st["pos"]), _ptr(st["tgt"]), _ptr(st["dir"]),
+                                             _ptr(st["road"]), _ptr(st["rng"]), _ptr(st["ncount"])))
+        return st
+
+    def debug_data(self):
+        out = np.zeros(10, np.uint32)
+        self._check(self.lib.msim_read_debug(self.h, _ptr(out)))
+        return out
+
+    def pairs(self, n=4096):
+        total = np.zeros(1, np.uint64)
+        per = np.zeros(n, np.uint32)
+        got = ctypes.c_uint32()
+        self._check(self.lib.msim_read_pairs(self.h, _ptr(total), _ptr(per), n, ctypes.byref(got)))
+        return int(total[0]), per[: got.value].copy()
+
+    def cells(self):
+        out = np.zeros((self.n, 2), np.int32)
+        self._check(self.lib.msim_read_cells(self.h, _ptr(out)))
+        return out
+
+    def info(self):
+        i = Info()
+        self._check(self.lib.msim_get_info(self.h, ctypes.byref(i)))
+        return {f: getattr(i, f) for f, _ in Info._fields_}
+
+    def time_ticks(self, n, kernels=True):
+        total = np.zeros(1, np.float32)
+        k = np.zeros(6, np.float32)
+        self._check(self.lib.msim_time_ticks(self.h, n, _ptr(total), _ptr(k) if kernels else None, 6))
+        return float(total[0]), k
+
+    def stream(self):
+        s = ctypes.c_void_p()
+        self._check(self.lib.msim_get_stream(self.h, ctypes.byref(s)))
+        return s.value

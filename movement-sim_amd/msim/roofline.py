@@ -1,0 +1,44 @@
+"""Algorithmic bytes of the fused tick, per agent, itemised per kernel.
+
+These are the bytes the algorithm has to move, not what a given kernel
+happens to move (rocprofv3 FETCH_SIZE/WRITE_SIZE give the latter):
+
+  advance   28 B  read pos 8 + target 8 + speed 4, write pos 8
+                  (24 B when all agents share the shader's constant speed)
+  scan       8 B  bucket-offset traffic per agent (counts in, starts out)
+  scatter   12 B  binned position 8 + agent id 4 written
+  collide   16 B  binned position 8 + agent id 4 read back, count 4 written
+  ---------------
+  fused     64 B  per agent-tick
+
+The arrival path (road 32 B, connections <= 24 B, xorshift state 32 B, road
+and target writes 12 B) runs for a few percent of agent-ticks and is reported
+separately as arrivals per agent-tick.
+"""
+
+HBM_PEAK_BYTES_PER_S = 8.0e12  # MI355X HBM3E
+
+B_ADVANCE = 28
+B_ADVANCE_UNIFORM_SPEED = 24
+B_SCAN = 8
+B_SCATTER = 12
+B_COLLIDE = 16
+B_FUSED = B_ADVANCE + B_SCAN + B_SCATTER + B_COLLIDE
+assert B_FUSED == 64
+
+# Order of msim_time_ticks' per-kernel durations.
+KERNELS = ("k_advance_bin", "k_scan_reduce", "k_scan_partials", "k_scan_down", "k_scatter", "k_collide")
+
+
+def kernel_bytes(name, uniform_speed=False):
+    """Algorithmic bytes per agent for one launch of the named kernel; the
+    three scan kernels share the scan's 8 B."""
+    if name == "k_advance_bin":
+        return B_ADVANCE_UNIFORM_SPEED if uniform_speed else B_ADVANCE
+    if name in ("k_scan_reduce", "k_scan_partials", "k_scan_down"):
+        return B_SCAN
+    if name == "k_scatter":
+        return B_SCATTER
+    if name == "k_collide":
+        return B_COLLIDE
+    raise KeyError(name)

@@ -1,0 +1,234 @@
+"""Parity of the HIP path (through the C-ABI) with the CPU oracle.
+
+Integer state (road index, xorshift state, neighbour counts, pair totals,
+cells) must be bit-exact; binary32 positions, targets and headings are
+compared bit for bit too (the library is built with -ffp-contract=off).
+"""
+import numpy as np
+import pytest
+
+import msim
+import orc
+from msim import scenarios as S
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+def assert_same(gpu, ref_state, what="", check_dir=True):
+    st = ref_state
+    for name, g, o in (("pos", gpu["pos"], st.pos), ("tgt", gpu["tgt"], st.tgt)):
+        gb, ob = bits(g), bits(o)
+        bad = np.flatnonzero((gb != ob).any(axis=1))
+        assert bad.size == 0, f"{what}: {name} differs for {bad.size} agents, first {bad[:5]} " \
+                              f"gpu={g[bad[:3]]} oracle={o[bad[:3]]}"
+    if check_dir:
+        assert (bits(gpu["dir"]) == bits(st.dir)).all(), f"{what}: dir differs"
+    np.testing.assert_array_equal(gpu["road"], st.road, err_msg=f"{what}: road")
+    np.testing.assert_array_equal(gpu["rng"], st.rng, err_msg=f"{what}: rng")
+    np.testing.assert_array_equal(gpu["ncount"], st.ncount, err_msg=f"{what}: neighbour counts")
+
+
+def pair(sc, threads=1, **opts):
+    g = msim.Simulation(sc.roads, sc.conns, sc.entities, sc.pc, speed=sc.speed, **opts)
+    o = orc.OracleSim(sc.roads, sc.conns, sc.entities, sc.pc, speed=sc.speed, threads=threads)
+    return g, o
+
+
+def star_scenario(n=64, seed=3):
+    roads, conns, w, h = S.star_map()
+    ents = S.reference_agents(roads, n, seed)
+    return S.Scenario("star", roads, conns, ents, None, msim.push_consts(w, h, 10.0), 0, (w, h))
+
+
+def test_star_map_reference_protocol_every_dispatch():
+    """Simulator::sim_tick protocol on the star fixture with the shader's
+    constants (speed 1.4, R = 10): state and pair count after every dispatch."""
+    sc = star_scenario()
+    g, o = pair(sc)
+    for t in range(300):
+        g.dispatch()
+        o.dispatch()
+        if t < 40 or t % 25 == 0:
+            assert_same(g.state(), o.st, f"dispatch {t}")
+    assert_same(g.state(), o.st, "end")
+    total, hist = g.pairs()
+    assert total == o.pairs_total
+    assert hist.tolist() == o.pair_hist[-len(hist):]
+    dbg = g.debug_data()
+    assert dbg[1] == o.pairs_total % 2**32
+
+
+def test_config1_full_parity_1000_ticks():
+    sc = S.config(1)
+    g, o = pair(sc)
+    for chunk in range(10):
+        g.run_ticks(100)
+        o.run_ticks(100)
+        assert_same(g.state(), o.st, f"after {100 * (chunk + 1)} ticks")
+    total, hist = g.pairs(4096)
+    assert total == o.pairs_total
+    assert hist.tolist() == o.pair_hist
+    assert o.arrivals > 0
+
+
+def test_dispatch_loop_equals_fused_graph_path():
+    sc = S.config(1, scale=0.3)
+    a = msim.Simulation(sc.roads, sc.conns, sc.entities, sc.pc, speed=sc.speed, graph_ticks=8)
+    b = msim.Simulation(sc.roads, sc.conns, sc.entities, sc.pc, speed=sc.speed)
+    a.run_ticks(77)
+    for _ in range(77):
+        b.sim_tick()
+    sa, sb = a.state(), b.state()
+    for k in sa:
+        np.testing.assert_array_equal(sa[k], sb[k], err_msg=k)
+    assert a.pairs() == b.pairs() or a.pairs()[0] == b.pairs()[0]
+
+
+@pytest.mark.parametrize("opts", [dict(reorder_interval=1), dict(reorder_interval=13, graph_ticks=5),
+                                  dict(graph_ticks=1), dict(table_log2=12), dict(table_log2=20)])
+def test_options_do_not_change_results(opts):
+    sc = S.config(1, scale=0.5)
+    g, o = pair(sc, **opts)
+    g.run_ticks(60)
+    o.run_ticks(60)
+    assert_same(g.state(), o.st, str(opts))
+    assert g.pairs()[0] == o.pairs_total
+
+
+def test_mixed_initialized_flags_and_collision_first():
+    """Agents with Entity.initialized == 0 only enter the index on their first
+    dispatch and do not search in a collision dispatch
+    (random_move.comp:863-867)."""
+    sc = S.config(1, scale=0.2)
+    rng = np.random.default_rng(5)
+    ents = sc.entities.copy()
+    ents["initialized"] = (rng.random(ents.shape[0]) < 0.5).astype(np.uint32)
+    # bunch agents so that pairs exist
+    ents["pos"][: ents.shape[0] // 2] = ents["pos"][ents.shape[0] // 2: 2 * (ents.shape[0] // 2)]
+    pc = sc.pc.copy()
+    pc["tick"] = 2  # next dispatch (tick 3) is a collision dispatch
+    sc2 = S.Scenario("mixed", sc.roads, sc.conns, ents, sc.speed, pc, 0, sc.world)
+    g, o = pair(sc2)
+    for t in range(6):
+        g.dispatch()
+        o.dispatch()
+        assert_same(g.state(), o.st, f"dispatch {t}", check_dir=(t > 0))
+    assert g.pairs()[0] == o.pairs_total
+    assert g.info()["n_uninitialized"] == 0
+
+
+def test_config2_scaled_parity():
+    sc = S.config(2, scale=0.1)
+    g, o = pair(sc, threads=8)
+    g.run_ticks(150)
+    o.run_ticks(150)
+    assert_same(g.state(), o.st, "config 2 at 10%")
+    assert g.pairs()[0] == o.pairs_total
+
+
+def test_config2_full_size_parity():
+    """1,000,000 agents on the 150k-node city graph, 30 ticks: full state and
+    every per-tick pair count."""
+    sc = S.config(2)
+    g, o = pair(sc, threads=16)
+    g.run_ticks(30)
+    o.run_ticks(30)
+    assert_same(g.state(), o.st, "config 2 full")
+    total, hist = g.pairs()
+    assert total == o.pairs_total and hist.tolist() == o.pair_hist
+
+
+def test_entity_readback_layout_and_colours():
+    sc = S.config(1, scale=0.1)
+    g = msim.Simulation(sc.roads, sc.conns, sc.entities, sc.pc, speed=sc.speed)
+    e0 = g.entities()
+    np.testing.assert_array_equal(e0["color"], sc.entities["color"])  # before any collision dispatch
+    assert (e0["initialized"] == 0).all()
+    g.run_ticks(5)
+    e = g.entities()
+    st = g.state()
+    assert (bits(e["pos"]) == bits(st["pos"])).all()
+    assert (bits(e["direction"]) == bits(st["dir"])).all()
+    assert (e["roadIndex"] == st["road"]).all() and (e["randomState"] == st["rng"]).all()
+    assert (e["initialized"] == 1).all()
+    blue = st["ncount"] > 0
+    np.testing.assert_array_equal(e["color"][blue], np.tile([0, 0, 1, 1], (blue.sum(), 1)))
+    np.testing.assert_array_equal(e["color"][~blue], np.tile([0, 1, 0, 1], ((~blue).sum(), 1)))
+    part = g.entities(10, 7)
+    np.testing.assert_array_equal(part.view(np.uint8), e[10:17].view(np.uint8))
+
+
+def test_cells_export_matches_oracle():
+    sc = S.config(1, scale=0.2)
+    g = msim.Simulation(sc.roads, sc.conns, sc.entities, sc.pc, speed=sc.speed)
+    g.run_ticks(3)
+    pos = g.state()["pos"]
+    np.testing.assert_array_equal(g.cells(), orc.cells(pos, orc.cell_size(1.0)))
+
+
+def test_heavy_cell_and_duplicates():
+    """3,000 agents stacked in one 1 m cell (every pair in range) plus sparse
+    agents: counts n-1 in the stack, pairs n(n-1)/2."""
+    sc = S.config(1, scale=0.5)
+    ents = sc.entities.copy()
+    k = 3000
+    ents["pos"][:k] = ents["pos"][0]
+    ents["initialized"] = 1
+    sc2 = S.Scenario("stack", sc.roads, sc.conns, ents, sc.speed, sc.pc, 0, sc.world)
+    g, o = pair(sc2)
+    g.dispatch(3)
+    o.dispatch(3)
+    st = g.state()
+    assert (st["ncount"][:k] >= k - 1).all()
+    assert_same(st, o.st, "stack", check_dir=False)
+    assert g.pairs()[0] == o.pairs_total >= k * (k - 1) // 2
+
+
+def test_tiny_and_empty():
+    roads, conns, w, h = S.star_map()
+    for n in (1, 2, 3):
+        ents = S.reference_agents(roads, n, 7)
+        g = msim.Simulation(roads, conns, ents, msim.push_consts(w, h, 10.0))
+        o = orc.OracleSim(roads, conns, ents, msim.push_consts(w, h, 10.0))
+        g.run_ticks(40)
+        o.run_ticks(40)
+        assert_same(g.state(), o.st, f"n={n}")
+    g = msim.Simulation(roads, conns, np.zeros(0, msim.ENTITY_DTYPE), msim.push_consts(w, h, 10.0))
+    g.run_ticks(10)
+    assert g.pairs()[0] == 0 and g.entities().shape == (0,)
+
+
+def test_radius_change_through_push_constants():
+    sc = S.config(1, scale=0.2)
+    g, o = pair(sc)
+    g.run_ticks(3)
+    o.run_ticks(3)
+    pc = g.pc.copy()
+    pc["collisionRadius"] = 0.75  # within the 1 m cell chosen at creation
+    pc["tick"] = int(pc["tick"]) + 1
+    g.pc = pc
+    g.dispatch(int(pc["tick"]))
+    o.radius = 0.75
+    o.dispatch(int(pc["tick"]))
+    g.dispatch()
+    o.dispatch()
+    assert_same(g.state(), o.st, "radius 0.75")
+    pc["collisionRadius"] = 3.0
+    with pytest.raises(msim.MsimError):
+        g.pc = pc
+        g.dispatch(int(pc["tick"]) + 2)
+
+
+def test_two_handles_are_deterministic():
+    sc = S.config(1, scale=0.3)
+    a = msim.Simulation(sc.roads, sc.conns, sc.entities, sc.pc, speed=sc.speed)
+    b = msim.Simulation(sc.roads, sc.conns, sc.entities, sc.pc, speed=sc.speed, reorder_interval=3)
+    a.run_ticks(40)
+    b.run_ticks(40)
+    sa, sb = a.state(), b.state()
+    for k in sa:
+        np.testing.assert_array_equal(sa[k], sb[k], err_msg=k)
