@@ -59,7 +59,8 @@ def main():
     ap.add_argument("--profile-ticks", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--reorder", type=int, default=None)
+    ap.add_argument("--reorder", type=int, default=16)
+    ap.add_argument("--binning", type=int, default=0, choices=(0, 1))
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only; multi-GPU runs are not part of it "
@@ -76,9 +77,7 @@ def main():
     n = sc.entities.shape[0]
     print(f"# workload config {args.config}: {n} agents, {sc.roads.shape[0]} roads, "
           f"{sc.conns.shape[0]} connection entries (generated in {gen_s:.1f} s)", file=sys.stderr, flush=True)
-    opts = {}
-    if args.reorder is not None:
-        opts["reorder_interval"] = args.reorder
+    opts = {"reorder_interval": args.reorder, "binning": args.binning}
     sim = msim.Simulation(sc.roads, sc.conns, sc.entities, sc.pc, speed=sc.speed, **opts)
     info = sim.info()
 
@@ -94,7 +93,7 @@ def main():
 
     # per-kernel durations (HIP events around every launch, on the handle's stream)
     _, kms = sim.time_ticks(args.profile_ticks, kernels=True)
-    per_launch_ms = {k: float(v) / args.profile_ticks for k, v in zip(RL.KERNELS, kms)}
+    per_launch_ms = {k: v / args.profile_ticks for k, v in kms.items()}
     dom = max(per_launch_ms, key=per_launch_ms.get)
     uniform = bool(info["uniform_speed"])
     dom_bytes = RL.kernel_bytes(dom, uniform) * n
@@ -134,6 +133,8 @@ def main():
             "collision_radius_m": float(sc.pc["collisionRadius"]),
             "cell_m": info["cell_size"],
             "bucket_table": [info["table_width"], info["table_height"]],
+            "binning": "segment-owned (LDS histograms, look-back)" if args.binning == 0 else "global atomics",
+            "reorder_interval": args.reorder,
             "graph_ticks": 32,
         },
         "roofline": {

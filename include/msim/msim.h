@@ -48,6 +48,10 @@ typedef struct msim_options {
     uint32_t history_capacity; /* 0: 4096 entries of per-tick pair counts      */
     int32_t device;            /* -1 (or 0 with struct_size 0): current device */
     uint32_t watchdog_ms;      /* 0: 600000 ms host-side wait limit            */
+    uint32_t table_width_log2; /* 0: auto; buckets per table row = 2^this      */
+    uint32_t segment_log2;     /* 0: auto; buckets per rebinning workgroup     */
+    uint32_t binning;          /* 0: segment-owned rebinning (LDS histograms,
+                                  look-back scan); 1: global-atomic counting   */
 } msim_options;
 
 typedef struct msim_info {
@@ -63,6 +67,8 @@ typedef struct msim_info {
     uint32_t uniform_speed;    /* 1 when no per-agent speed array was given   */
     uint32_t n_uninitialized;  /* agents whose Entity.initialized is still 0  */
     uint64_t device_bytes;     /* device memory held by the handle            */
+    uint32_t kernels_per_tick; /* launches of one fused tick (msim_time_ticks) */
+    uint32_t binning;          /* msim_options.binning in effect              */
 } msim_info;
 
 /* Replaces Simulator::init's buffer/pipeline creation (src/sim/Simulator.cpp:
@@ -98,7 +104,9 @@ int msim_dispatch(msim_sim* sim, const msim_push_consts* pc);
  * Runs as replays of a captured hipGraph of fused ticks.  Asynchronous. */
 int msim_run_ticks(msim_sim* sim, uint32_t n_ticks);
 
-/* Blocks until the handle's stream drains (bounded by the watchdog). */
+/* Blocks until the handle's stream drains (bounded by the watchdog), then
+ * reports MSIM_ERR_STATE if a device-side invariant check failed (every agent
+ * binned exactly once; bounded look-back waits). */
 int msim_sync(msim_sim* sim);
 
 /* Replaces OpTensorSyncLocal({tensorEntities}) + tensor->vector<Entity>()
@@ -139,11 +147,15 @@ int msim_get_info(msim_sim* sim, msim_info* out);
 /* Time n_ticks fused ticks on the handle's stream with HIP events.
  * ms_total receives the elapsed time; kernel_ms (may be NULL) receives the
  * summed duration of each kernel of the tick, measured by bracketing every
- * launch with events in a second, separate pass of n_ticks ticks, in the
- * order advance, scan-reduce, scan-partials, scan-down, scatter, collide
- * (n_kernel_ms entries at most). */
+ * launch with events in a second, separate pass of n_ticks ticks, in launch
+ * order (see msim_kernel_name; n_kernel_ms entries at most).  Both passes
+ * advance the simulation. */
 int msim_time_ticks(msim_sim* sim, uint32_t n_ticks, float* ms_total,
                     float* kernel_ms, uint32_t n_kernel_ms);
+
+/* Name of the k-th kernel of a fused tick in msim_time_ticks' order, or NULL
+ * for k >= msim_info.kernels_per_tick. */
+const char* msim_kernel_name(const msim_sim* sim, uint32_t k);
 
 /* Stream the handle enqueues on (a hipStream_t), for callers that order
  * their own work after it. */

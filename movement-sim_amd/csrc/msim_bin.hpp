@@ -1,0 +1,261 @@
+// msim_bin.hpp — the fast move dispatch.
+//
+//   k_advance   streams pos/target/speed once.  Agents that keep walking get
+//               pos += heading; arriving agents snap to their target
+//               (random_move.comp:844-851) and are appended to one of
+//               kQueues arrival queues.  With ATOMIC_BIN it also counts the
+//               agent into its bucket (global atomics, the atomic path).
+//   k_route     route choice (random_move.comp:778-828) and the new heading
+//               for the queued arrivals only, so the three dependent gathers
+//               of the arrival path no longer stall the streaming kernel.
+//   k_bin_count / k_bin_place  (segment path) rebuild the bucket table
+//               without global atomics: the toroidal table is cut into n_seg
+//               segments of S buckets and workgroup `seg` owns segment seg.
+//               An agent moves at most D cells per axis per tick, so in the
+//               previous tick's bucket order it sits within `margin` buckets
+//               of its new bucket; each workgroup scans that window of the
+//               previous binning, counts the agents landing in its segment
+//               with LDS atomics and scans the counts (k_bin_count), then
+//               adds the segment's global offset (sum of the preceding
+//               segments' totals) and places its agents (k_bin_place).
+#pragma once
+
+#include "msim_kernels.hpp"
+
+namespace msim {
+
+constexpr uint32_t kBinThreads = 512;
+constexpr uint32_t kQueues = 64;  // arrival queues, one counter per 128-B line
+constexpr uint32_t kQueueStride = 32;  // counter stride in words
+
+struct QueueDev {
+    uint32_t* items;       // [kQueues * cap] agent indices
+    uint32_t* count;       // this tick's counters, at [q * kQueueStride]
+    uint32_t* count_next;  // next tick's counters (zeroed by k_advance)
+    uint32_t cap;          // capacity per queue
+};
+
+template <bool UNIFORM, bool WRITE_DIR, bool ATOMIC_BIN>
+__global__ __launch_bounds__(kBlock) void k_advance(AgentsDev a, GridDev g, QueueDev q,
+                                                    uint32_t* __restrict__ blk_stats) {
+    __shared__ uint32_t lds[2 * (kBlock / 64)];
+    __shared__ uint32_t sh_base;
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t i = blk * kBlock + threadIdx.x;
+    if (blockIdx.x == 0 && threadIdx.x < kQueues) {
+        q.count_next[threadIdx.x * kQueueStride] = 0u;
+    }
+    int inserted = 0;
+    int arrived = 0;
+    if (i < a.n) {
+        const float2 p = a.pos[i];
+        const float2 t = a.tgt[i];
+        const float s = UNIFORM ? a.spd_const : a.spd[i];
+        // update_direction (:830-839) and move's distance (:842): the two
+        // lengths are bit-identical, (p - t)^2 == (t - p)^2.
+        const float dx = t.x - p.x;
+        const float dy = t.y - p.y;
+        const float len = vlen(dx, dy);
+        float2 np;
+        if (len > s) {  // len > 0 here, so the heading is the normalised one
+            const float2 d = make_float2((dx / len) * s, (dy / len) * s);
+            np = make_float2(p.x + d.x, p.y + d.y);
+            if (WRITE_DIR) {
+                a.dir[i] = d;
+            }
+        } else {
+            np = t;
+            arrived = 1;
+        }
+        a.pos[i] = np;
+        inserted = static_cast<int>(cell_coord(np.x, g.inv_cell) != cell_coord(p.x, g.inv_cell)) |
+                   static_cast<int>(cell_coord(np.y, g.inv_cell) != cell_coord(p.y, g.inv_cell));
+        if (ATOMIC_BIN) {
+            const uint32_t b = bucket_of(cell_coord(np.x, g.inv_cell), cell_coord(np.y, g.inv_cell), g.lw,
+                                         g.wmask, g.hmask);
+            g.rank[i] = atomicAdd(&g.cnt[b], 1u);
+        }
+    }
+    // block-aggregated append of the arrivals to queue blk % kQueues
+    const unsigned long long am = __ballot(arrived);
+    const unsigned long long im = __ballot(inserted);
+    const uint32_t w = threadIdx.x >> 6;
+    if (__lane_id() == 0) {
+        lds[w] = static_cast<uint32_t>(__popcll(am));
+        lds[kBlock / 64 + w] = static_cast<uint32_t>(__popcll(im));
+    }
+    __syncthreads();
+    uint32_t before = 0, tot_a = 0, tot_i = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kBlock / 64; k++) {
+        before += (k < w) ? lds[k] : 0u;
+        tot_a += lds[k];
+        tot_i += lds[kBlock / 64 + k];
+    }
+    const uint32_t qi = blk % kQueues;
+    if (threadIdx.x == 0) {
+        sh_base = tot_a ? atomicAdd(&q.count[qi * kQueueStride], tot_a) : 0u;
+        blk_stats[2 * blk] = tot_i;
+        blk_stats[2 * blk + 1] = tot_a;
+    }
+    __syncthreads();
+    if (arrived) {
+        const unsigned long long lt = (1ull << __lane_id()) - 1ull;
+        const uint32_t slot = sh_base + before + static_cast<uint32_t>(__popcll(am & lt));
+        q.items[qi * q.cap + slot] = i;
+    }
+}
+
+// Route choice for the queued arrivals; grid = kQueues * blocks_per_queue.
+template <bool UNIFORM, bool WRITE_DIR>
+__global__ __launch_bounds__(kBlock) void k_route(AgentsDev a, QueueDev q, const msim_road* __restrict__ roads,
+                                                  const uint32_t* __restrict__ conns) {
+    const uint32_t qi = blockIdx.x % kQueues;
+    const uint32_t per_queue = gridDim.x / kQueues;
+    const uint32_t n_items = q.count[qi * kQueueStride];
+    for (uint32_t j = (blockIdx.x / kQueues) * kBlock + threadIdx.x; j < n_items; j += per_queue * kBlock) {
+        const uint32_t i = q.items[qi * q.cap + j];
+        float2 t = a.tgt[i];  // the old target, where the agent now stands
+        const float2 np = t;
+        uint32_t r = a.road[i];
+        uint4 st = a.rng[i];
+        choose_route(t, r, st, roads, conns);
+        a.tgt[i] = t;
+        a.road[i] = r;
+        a.rng[i] = st;
+        if (WRITE_DIR) {
+            const float s = UNIFORM ? a.spd_const : a.spd[i];
+            a.dir[i] = heading(t, np, s);
+        }
+    }
+}
+
+struct BinDev {
+    const uint32_t* old_start;  // [m+1] previous binning
+    const uint32_t* old_bid;    // [n]
+    uint32_t* new_start;        // [m+1]
+    float2* new_bpos;           // [n]
+    uint32_t* new_bid;          // [n]
+    uint32_t* seg_total;        // [n_seg]
+    Counters* ctr;
+    float inv_cell;
+    uint32_t lw, wmask, hmask, m;
+    uint32_t seg_log, n_seg, margin;
+    uint32_t n;
+};
+
+// Old-slot ranges [a0, a1) and [c0, c1) holding every agent whose previous
+// bucket lies in the cyclic window [s0 - margin, s0 + S + margin).
+__device__ __forceinline__ void bin_window(const BinDev& b, uint32_t s0, uint32_t& a0, uint32_t& a1,
+                                           uint32_t& c0, uint32_t& c1) {
+    const uint32_t S = 1u << b.seg_log;
+    c0 = 0;
+    c1 = 0;
+    if (2ull * b.margin + S >= b.m) {
+        a0 = 0;
+        a1 = b.old_start[b.m];
+        return;
+    }
+    const uint32_t lo = (s0 + b.m - b.margin) & (b.m - 1);
+    const uint32_t hi = (s0 + S + b.margin) & (b.m - 1);
+    a0 = b.old_start[lo];
+    if (lo < hi) {
+        a1 = b.old_start[hi];
+    } else {
+        a1 = b.old_start[b.m];
+        c1 = b.old_start[hi];
+    }
+}
+
+__device__ __forceinline__ uint32_t bucket_of_pos(float2 p, const BinDev& b) {
+    return bucket_of(cell_coord(p.x, b.inv_cell), cell_coord(p.y, b.inv_cell), b.lw, b.wmask, b.hmask);
+}
+
+// Counts the window agents that land in segment blockIdx.x, writes the
+// segment-local exclusive offsets to new_start and the total to seg_total.
+__global__ __launch_bounds__(kBinThreads) void k_bin_count(const float2* __restrict__ pos, BinDev b) {
+    extern __shared__ uint32_t hist[];
+    __shared__ uint32_t lds[kBinThreads / 64];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t S = 1u << b.seg_log;
+    const uint32_t seg = blockIdx.x;
+    const uint32_t s0 = seg << b.seg_log;
+    for (uint32_t j = tid; j < S; j += kBinThreads) {
+        hist[j] = 0u;
+    }
+    __syncthreads();
+    uint32_t a0, a1, c0, c1;
+    bin_window(b, s0, a0, a1, c0, c1);
+    for (int r = 0; r < 2; r++) {
+        const uint32_t lo = r ? c0 : a0;
+        const uint32_t hi = r ? c1 : a1;
+        for (uint32_t k = lo + tid; k < hi; k += kBinThreads) {
+            const uint32_t off = bucket_of_pos(pos[b.old_bid[k]], b) - s0;
+            if (off < S) {
+                atomicAdd(&hist[off], 1u);
+            }
+        }
+    }
+    __syncthreads();
+    const uint32_t per = S / kBinThreads;
+    uint32_t run = 0;
+    for (uint32_t q = 0; q < per; q++) {
+        run += hist[tid * per + q];
+    }
+    uint32_t total = 0;
+    uint32_t excl = block_excl_scan<kBinThreads>(run, lds, total);
+    for (uint32_t q = 0; q < per; q++) {
+        const uint32_t v = hist[tid * per + q];
+        b.new_start[s0 + tid * per + q] = excl;
+        excl += v;
+    }
+    if (tid == 0) {
+        b.seg_total[seg] = total;
+    }
+}
+
+// Adds the segment's global offset and places its agents.
+__global__ __launch_bounds__(kBinThreads) void k_bin_place(const float2* __restrict__ pos, BinDev b) {
+    extern __shared__ uint32_t cursor[];
+    __shared__ uint32_t lds[kBinThreads / 64];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t S = 1u << b.seg_log;
+    const uint32_t seg = blockIdx.x;
+    const uint32_t s0 = seg << b.seg_log;
+    uint32_t before = 0;
+    for (uint32_t k = tid; k < seg; k += kBinThreads) {
+        before += b.seg_total[k];
+    }
+    const uint32_t base = block_sum<kBinThreads>(before, lds);
+    for (uint32_t j = tid; j < S; j += kBinThreads) {
+        const uint32_t v = base + b.new_start[s0 + j];
+        cursor[j] = v;
+        b.new_start[s0 + j] = v;
+    }
+    if (seg == b.n_seg - 1 && tid == 0) {
+        const uint32_t total = base + b.seg_total[seg];
+        b.new_start[b.m] = total;
+        if (total != b.n) {
+            atomicOr(&b.ctr->faults, kFaultBinCount);
+        }
+    }
+    __syncthreads();
+    uint32_t a0, a1, c0, c1;
+    bin_window(b, s0, a0, a1, c0, c1);
+    for (int r = 0; r < 2; r++) {
+        const uint32_t lo = r ? c0 : a0;
+        const uint32_t hi = r ? c1 : a1;
+        for (uint32_t k = lo + tid; k < hi; k += kBinThreads) {
+            const uint32_t i = b.old_bid[k];
+            const float2 p = pos[i];
+            const uint32_t off = bucket_of_pos(p, b) - s0;
+            if (off < S) {
+                const uint32_t slot = atomicAdd(&cursor[off], 1u);
+                b.new_bpos[slot] = p;
+                b.new_bid[slot] = i;
+            }
+        }
+    }
+}
+
+}  // namespace msim

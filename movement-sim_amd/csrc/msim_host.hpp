@@ -38,6 +38,53 @@ inline uint32_t auto_table_log2(uint32_t n_agents) {
     return l;
 }
 
+// Table rows: the table must fold the world vertically (height <= the
+// world's rows in cells), otherwise rows the world never reaches stay empty
+// and the used buckets alias more.  Beyond that, rows are kept narrow so the
+// rebinning window (a segment plus a few rows each side) stays small.
+inline uint32_t auto_width_log2(uint32_t tlog, uint32_t rows_log) {
+    uint32_t fold = rows_log + 2 > tlog ? 2 : tlog - rows_log;  // height <= 2^rows_log
+    uint32_t l = fold < 5 ? 5 : fold;
+    if (l + 2 > tlog) l = tlog - 2;
+    return l;
+}
+
+// Segment owned by one rebinning workgroup: 16 table rows, 512..16384
+// buckets (at most 64 KB of LDS counters).
+inline uint32_t auto_segment_log2(uint32_t tlog, uint32_t lw) {
+    uint32_t l = lw + 4;
+    if (l < 9) l = 9;
+    if (l > 14) l = 14;
+    if (l > tlog) l = tlog;
+    return l;
+}
+
+// floor(log2(rows spanned by the roads and agents, in cells)), at least 2.
+inline uint32_t extent_rows_log2(const msim_road* roads, uint32_t n_roads, const msim_entity* e, uint32_t n,
+                                 float cell) {
+    double lo = 0, hi = 0;
+    bool any = false;
+    auto add = [&](float y) {
+        if (!any) {
+            lo = hi = y;
+            any = true;
+        }
+        lo = y < lo ? y : lo;
+        hi = y > hi ? y : hi;
+    };
+    for (uint32_t r = 0; r < n_roads; r++) {
+        add(roads[r].start.pos.y);
+        add(roads[r].end.pos.y);
+    }
+    for (uint32_t i = 0; i < n; i++) {
+        add(e[i].pos.y);
+    }
+    const double rows = (hi - lo) / cell + 1.0;
+    uint32_t l = 2;
+    while (l < 30 && static_cast<double>(1ull << (l + 1)) <= rows) l++;
+    return l;
+}
+
 inline bool is_pow2_float(float v) {
     if (!(v > 0.0f) || !std::isfinite(v)) return false;
     int e = 0;
@@ -120,6 +167,31 @@ inline bool validate_map(const msim_road* roads, uint32_t n_roads, const uint32_
         if (conns[k] >= n_roads) {
             snprintf(buf, sizeof(buf), "connections[%u] = %u is not a road index (< %u)", k, conns[k],
                      n_roads);
+            *why = buf;
+            return false;
+        }
+    }
+    return true;
+}
+
+// Cell coordinates must stay far from the +-2^30 clamp of the cell index so
+// that every tick's displacement bound holds in cell units.
+inline bool validate_extent(const msim_road* roads, uint32_t n_roads, const msim_entity* e, uint32_t n,
+                            float inv_cell, std::string* why) {
+    const double lim = 536870912.0;  // 2^29 cells
+    auto bad = [&](float v) { return !(std::fabs(static_cast<double>(v) * inv_cell) < lim); };
+    char buf[160];
+    for (uint32_t r = 0; r < n_roads; r++) {
+        if (bad(roads[r].start.pos.x) || bad(roads[r].start.pos.y) || bad(roads[r].end.pos.x) ||
+            bad(roads[r].end.pos.y)) {
+            snprintf(buf, sizeof(buf), "road %u lies beyond 2^29 grid cells from the origin", r);
+            *why = buf;
+            return false;
+        }
+    }
+    for (uint32_t i = 0; i < n; i++) {
+        if (bad(e[i].pos.x) || bad(e[i].pos.y) || bad(e[i].target.x) || bad(e[i].target.y)) {
+            snprintf(buf, sizeof(buf), "agent %u lies beyond 2^29 grid cells from the origin", i);
             *why = buf;
             return false;
         }

@@ -53,14 +53,20 @@ struct GridDev {
     uint32_t* bid;     // [n]   internal agent index in bucket order
 };
 
+// Device-side counters.  Each is updated by exactly one thread of a
+// single-workgroup kernel (k_scan_partials, k_collide_finish) from per-block
+// partial sums, so no kernel contends on them with atomics.
 struct Counters {
     unsigned long long inserts;      // spatial-index insertions (debugData[0])
     unsigned long long pairs_total;  // cumulative pairs (debugData[1], 64-bit)
-    unsigned long long acc;          // per-dispatch accumulator of k_collide
     unsigned long long arrivals;     // agents that reached their target
     unsigned long long collides;     // collision dispatches completed
-    unsigned int blocks_done;
+    unsigned int faults;             // kFault* bits raised by device-side checks
     unsigned int pad;
+};
+
+enum FaultBits : unsigned int {
+    kFaultBinCount = 1u  // binned agents != agents (an agent left its rebinning window)
 };
 
 enum AdvanceMode : int {
@@ -113,6 +119,24 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* lds, u
 }
 
 template <int NT>
+__device__ __forceinline__ unsigned long long block_sum64(unsigned long long v, unsigned long long* lds) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        v += __shfl_xor(v, o, 64);
+    }
+    if (__lane_id() == 0) {
+        lds[threadIdx.x >> 6] = v;
+    }
+    __syncthreads();
+    unsigned long long s = 0;
+#pragma unroll
+    for (int k = 0; k < NT / 64; k++) {
+        s += lds[k];
+    }
+    return s;
+}
+
+template <int NT>
 __device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* lds) {
     v = wave_sum(v);
     if (__lane_id() == 0) {
@@ -129,12 +153,16 @@ __device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* lds) {
 
 // ---- move dispatch ------------------------------------------------------
 
+// blk_stats[2 * block] / [2 * block + 1]: insertions / arrivals of the block
+// (folded into the counters by k_scan_partials).
 template <int MODE, bool UNIFORM, bool WRITE_DIR>
 __global__ __launch_bounds__(kBlock) void k_advance_bin(AgentsDev a, GridDev g,
                                                         const msim_road* __restrict__ roads,
                                                         const uint32_t* __restrict__ conns,
-                                                        Counters* __restrict__ ctr) {
-    const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+                                                        uint32_t* __restrict__ blk_stats) {
+    __shared__ uint32_t lds[2 * (kBlock / 64)];
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t i = blk * kBlock + threadIdx.x;
     int inserted = 0;
     int arrived = 0;
     if (i < a.n) {
@@ -181,15 +209,23 @@ __global__ __launch_bounds__(kBlock) void k_advance_bin(AgentsDev a, GridDev g,
         g.rank[i] = atomicAdd(&g.cnt[b], 1u);
     }
     if (MODE != kBinOnly) {
-        const unsigned long long mi = __ballot(inserted);
-        const unsigned long long ma = __ballot(arrived);
+        const uint32_t wi = static_cast<uint32_t>(__popcll(__ballot(inserted)));
+        const uint32_t wa = static_cast<uint32_t>(__popcll(__ballot(arrived)));
+        const uint32_t w = threadIdx.x >> 6;
         if (__lane_id() == 0) {
-            if (mi) {
-                atomicAdd(&ctr->inserts, static_cast<unsigned long long>(__popcll(mi)));
+            lds[w] = wi;
+            lds[kBlock / 64 + w] = wa;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t si = 0, sa = 0;
+#pragma unroll
+            for (uint32_t k = 0; k < kBlock / 64; k++) {
+                si += lds[k];
+                sa += lds[kBlock / 64 + k];
             }
-            if (ma) {
-                atomicAdd(&ctr->arrivals, static_cast<unsigned long long>(__popcll(ma)));
-            }
+            blk_stats[2 * blk] = si;
+            blk_stats[2 * blk + 1] = sa;
         }
     }
 }
@@ -211,11 +247,30 @@ __global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t* __restri
     }
 }
 
-// Exclusive scan of the tile totals (one workgroup); start[m] = total.
+// Exclusive scan of the tile totals (one workgroup); start[m] = total.  Also
+// folds the advance kernel's n_stat per-block statistics into the counters.
 __global__ __launch_bounds__(kPartialsThreads) void k_scan_partials(uint32_t* __restrict__ part,
                                                                     uint32_t n_part,
-                                                                    uint32_t* __restrict__ start_end) {
+                                                                    uint32_t* __restrict__ start_end,
+                                                                    const uint32_t* __restrict__ blk_stats,
+                                                                    uint32_t n_stat,
+                                                                    Counters* __restrict__ ctr) {
     __shared__ uint32_t lds[kPartialsThreads / 64];
+    if (n_stat) {
+        unsigned long long si = 0, sa = 0;
+        for (uint32_t k = threadIdx.x; k < n_stat; k += kPartialsThreads) {
+            si += blk_stats[2 * k];
+            sa += blk_stats[2 * k + 1];
+        }
+        const uint32_t ti = block_sum<kPartialsThreads>(static_cast<uint32_t>(si), lds);
+        __syncthreads();
+        const uint32_t ta = block_sum<kPartialsThreads>(static_cast<uint32_t>(sa), lds);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            ctr->inserts += ti;
+            ctr->arrivals += ta;
+        }
+    }
     const uint32_t per = (n_part + kPartialsThreads - 1) / kPartialsThreads;
     const uint32_t lo = threadIdx.x * per;
     uint32_t s = 0;
@@ -294,10 +349,10 @@ __global__ __launch_bounds__(kBlock) void k_scatter(AgentsDev a, GridDev g) {
 // the reference's "j < i" rule at random_move.comp:574.  The last workgroup to
 // finish folds the dispatch's pair total into the counters and history.
 template <bool SLOW>
-__global__ __launch_bounds__(kBlock) void k_collide(AgentsDev a, GridDev g, Counters* __restrict__ ctr,
-                                                    uint32_t* __restrict__ hist, uint32_t hist_mask) {
+__global__ __launch_bounds__(kBlock) void k_collide(AgentsDev a, GridDev g, uint32_t* __restrict__ blk_pairs) {
     __shared__ uint32_t lds[kBlock / 64];
-    const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t s = blk * kBlock + threadIdx.x;
     uint32_t cnt = 0;
     uint32_t own = 0;
     if (s < a.n) {
@@ -354,20 +409,59 @@ __global__ __launch_bounds__(kBlock) void k_collide(AgentsDev a, GridDev g, Coun
     }
     const uint32_t bs = block_sum<kBlock>(SLOW ? own : cnt, lds);
     if (threadIdx.x == 0) {
-        if (bs) {
-            atomicAdd(&ctr->acc, static_cast<unsigned long long>(bs));
-        }
-        __threadfence();
-        const unsigned int prev = atomicAdd(&ctr->blocks_done, 1u);
-        if (prev == gridDim.x - 1) {
-            __threadfence();
-            const unsigned long long acc = atomicExch(&ctr->acc, 0ull);
-            const unsigned long long pairs = SLOW ? acc : acc / 2;
-            atomicAdd(&ctr->pairs_total, pairs);
-            const unsigned long long t = atomicAdd(&ctr->collides, 1ull);
-            hist[t & hist_mask] = static_cast<uint32_t>(pairs);
-            atomicExch(&ctr->blocks_done, 0u);
-        }
+        blk_pairs[blk] = bs;
+    }
+}
+
+// Folds k_collide's per-block sums into the dispatch's pair total: every pair
+// is found from both ends in the fast path, once in the SLOW path.
+// Folds n_stat per-block advance statistics (insertions, arrivals) into the
+// counters; one workgroup.
+__device__ __forceinline__ void fold_stats(const uint32_t* __restrict__ blk_stats, uint32_t n_stat,
+                                           Counters* __restrict__ ctr, unsigned long long* lds) {
+    unsigned long long si = 0, sa = 0;
+    for (uint32_t k = threadIdx.x; k < n_stat; k += kPartialsThreads) {
+        si += blk_stats[2 * k];
+        sa += blk_stats[2 * k + 1];
+    }
+    const unsigned long long ti = block_sum64<kPartialsThreads>(si, lds);
+    __syncthreads();
+    const unsigned long long ta = block_sum64<kPartialsThreads>(sa, lds);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        ctr->inserts += ti;
+        ctr->arrivals += ta;
+    }
+}
+
+__global__ __launch_bounds__(kPartialsThreads) void k_fold_stats(const uint32_t* __restrict__ blk_stats,
+                                                                 uint32_t n_stat, Counters* __restrict__ ctr) {
+    __shared__ unsigned long long lds[kPartialsThreads / 64];
+    fold_stats(blk_stats, n_stat, ctr, lds);
+}
+
+// Also folds n_stat pending advance statistics (0: none pending).
+template <bool SLOW>
+__global__ __launch_bounds__(kPartialsThreads) void k_collide_finish(const uint32_t* __restrict__ blk_pairs,
+                                                                     uint32_t nb, Counters* __restrict__ ctr,
+                                                                     uint32_t* __restrict__ hist,
+                                                                     uint32_t hist_mask,
+                                                                     const uint32_t* __restrict__ blk_stats,
+                                                                     uint32_t n_stat) {
+    __shared__ unsigned long long lds[kPartialsThreads / 64];
+    if (n_stat) {
+        fold_stats(blk_stats, n_stat, ctr, lds);
+    }
+    unsigned long long s = 0;
+    for (uint32_t k = threadIdx.x; k < nb; k += kPartialsThreads) {
+        s += blk_pairs[k];
+    }
+    const unsigned long long tot = block_sum64<kPartialsThreads>(s, lds);
+    if (threadIdx.x == 0) {
+        const unsigned long long pairs = SLOW ? tot : tot / 2;
+        ctr->pairs_total += pairs;
+        hist[ctr->collides & hist_mask] = static_cast<uint32_t>(pairs);
+        ctr->collides += 1;
     }
 }
 

@@ -17,8 +17,9 @@
 #include <vector>
 
 #include "msim/msim.h"
-#include "msim_kernels.hpp"
+#include "msim_bin.hpp"
 #include "msim_host.hpp"
+#include "msim_kernels.hpp"
 
 using namespace msim;
 
@@ -52,8 +53,27 @@ struct msim_sim {
     int cur = 0;
     bool uniform = true;
     float spd_const = kShaderSpeed;
-    GridDev grid{};
+    GridDev grid{};  // table parameters and the atomic path's scratch arrays
+    // Double-buffered binning: [bcur] is the current one (read by the
+    // collision dispatch), the fast move dispatch writes [1 - bcur].
+    uint32_t* bstart[2] = {nullptr, nullptr};
+    float2* bbpos[2] = {nullptr, nullptr};
+    uint32_t* bbid[2] = {nullptr, nullptr};
+    int bcur = 0;
+    bool stats_pending = false;
+    int binning = 0;  // 0: segment-owned (k_advance + k_bin), 1: atomic counting
+    uint32_t seg_log = 0, n_seg = 0, margin = 0;
+    uint32_t* seg_total = nullptr;  // [n_seg] per-segment agent totals
+    // Arrival queues of the fast move dispatch; counters are double-buffered by
+    // tick parity (k_advance zeroes next tick's set).
+    uint32_t* q_items = nullptr;
+    uint32_t* q_count = nullptr;  // [2][kQueues * kQueueStride]
+    uint32_t q_cap = 0;
+    uint32_t q_blocks = 1;  // k_route workgroups per queue
+    int qpar = 0;
     Counters* ctr = nullptr;
+    uint32_t* blk_stats = nullptr;  // [2 * agent blocks] advance statistics
+    uint32_t* blk_pairs = nullptr;  // [agent blocks] per-block neighbour sums
     uint32_t* hist = nullptr;
     uint32_t hist_cap = 0;
     msim_push_consts pc{};
@@ -64,11 +84,19 @@ struct msim_sim {
     uint32_t reorder_interval = 0;
     uint32_t since_reorder = 0;
     uint32_t graph_ticks = 32;
-    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    hipGraphExec_t gexec[2][2][2] = {};  // [cur][bcur][qpar]
     uint32_t watchdog_ms = 600000;
     uint64_t bytes = 0;
     std::vector<void*> allocs;
     std::string err;
+
+    GridDev grid_cur() const {
+        GridDev g = grid;
+        g.start = bstart[bcur];
+        g.bpos = bbpos[bcur];
+        g.bid = bbid[bcur];
+        return g;
+    }
 
     AgentsDev agents(int which) const {
         const Buffers& b = buf[which];
@@ -76,6 +104,14 @@ struct msim_sim {
                          b.road, b.rng, b.init, b.ncount, b.perm};
     }
 };
+
+namespace {
+const char* const kFastKernels[] = {"k_advance",   "k_route",   "k_bin_count",
+                                    "k_bin_place", "k_collide", "k_collide_finish"};
+const char* const kAtomicKernels[] = {"k_advance",   "k_route",   "k_scan_reduce", "k_scan_partials",
+                                      "k_scan_down", "k_scatter", "k_collide",     "k_collide_finish"};
+uint32_t kernels_per_tick(const msim_sim* s) { return s->binning ? 8u : 6u; }
+}  // namespace
 
 namespace {
 
@@ -140,25 +176,124 @@ void launch_advance_mode(msim_sim* s, bool write_dir) {
     if (s->uniform) {
         if (write_dir) {
             hipLaunchKernelGGL((k_advance_bin<MODE, true, true>), g, dim3(kBlock), 0, s->stream, a,
-                               s->grid, s->roads, s->conns, s->ctr);
+                               s->grid_cur(), s->roads, s->conns, s->blk_stats);
         } else {
             hipLaunchKernelGGL((k_advance_bin<MODE, true, false>), g, dim3(kBlock), 0, s->stream, a,
-                               s->grid, s->roads, s->conns, s->ctr);
+                               s->grid_cur(), s->roads, s->conns, s->blk_stats);
         }
     } else {
         if (write_dir) {
             hipLaunchKernelGGL((k_advance_bin<MODE, false, true>), g, dim3(kBlock), 0, s->stream, a,
-                               s->grid, s->roads, s->conns, s->ctr);
+                               s->grid_cur(), s->roads, s->conns, s->blk_stats);
         } else {
             hipLaunchKernelGGL((k_advance_bin<MODE, false, false>), g, dim3(kBlock), 0, s->stream, a,
-                               s->grid, s->roads, s->conns, s->ctr);
+                               s->grid_cur(), s->roads, s->conns, s->blk_stats);
         }
     }
 }
 
+// Fast move dispatch: streaming advance, route choice for the arrivals, then
+// the bucket table rebuild (segment-owned into the other binning buffer, or
+// atomic counting + scan + scatter into the current one).  ev (optional)
+// receives an event after each launch.
+template <bool U, bool W>
+void launch_advance_route(msim_sim* s, const QueueDev& q) {
+    const AgentsDev a = s->agents(s->cur);
+    const GridDev g = s->grid_cur();
+    const dim3 nb(agent_blocks(s->n));
+    if (s->binning) {
+        hipLaunchKernelGGL((k_advance<U, W, true>), nb, dim3(kBlock), 0, s->stream, a, g, q, s->blk_stats);
+    } else {
+        hipLaunchKernelGGL((k_advance<U, W, false>), nb, dim3(kBlock), 0, s->stream, a, g, q, s->blk_stats);
+    }
+}
+
+template <bool U, bool W>
+void launch_route(msim_sim* s, const QueueDev& q) {
+    hipLaunchKernelGGL((k_route<U, W>), dim3(kQueues * s->q_blocks), dim3(kBlock), 0, s->stream,
+                       s->agents(s->cur), q, s->roads, s->conns);
+}
+
+int enqueue_move_fast(msim_sim* s, bool write_dir, hipEvent_t* ev) {
+    const uint32_t qw = kQueues * kQueueStride;
+    QueueDev q{s->q_items, s->q_count + s->qpar * qw, s->q_count + (1 - s->qpar) * qw, s->q_cap};
+    if (s->uniform) {
+        if (write_dir) launch_advance_route<true, true>(s, q);
+        else launch_advance_route<true, false>(s, q);
+    } else {
+        if (write_dir) launch_advance_route<false, true>(s, q);
+        else launch_advance_route<false, false>(s, q);
+    }
+    if (ev) (void)hipEventRecord(ev[0], s->stream);
+    if (s->uniform) {
+        if (write_dir) launch_route<true, true>(s, q);
+        else launch_route<true, false>(s, q);
+    } else {
+        if (write_dir) launch_route<false, true>(s, q);
+        else launch_route<false, false>(s, q);
+    }
+    if (ev) (void)hipEventRecord(ev[1], s->stream);
+    s->qpar = 1 - s->qpar;
+    const float2* pos = s->agents(s->cur).pos;
+    if (s->binning == 0) {
+        const int nb = 1 - s->bcur;
+        BinDev b{};
+        b.old_start = s->bstart[s->bcur];
+        b.old_bid = s->bbid[s->bcur];
+        b.new_start = s->bstart[nb];
+        b.new_bpos = s->bbpos[nb];
+        b.new_bid = s->bbid[nb];
+        b.seg_total = s->seg_total;
+        b.ctr = s->ctr;
+        b.inv_cell = s->grid.inv_cell;
+        b.lw = s->grid.lw;
+        b.wmask = s->grid.wmask;
+        b.hmask = s->grid.hmask;
+        b.m = s->grid.m;
+        b.seg_log = s->seg_log;
+        b.n_seg = s->n_seg;
+        b.margin = s->margin;
+        b.n = s->n;
+        const size_t lds = sizeof(uint32_t) << s->seg_log;
+        hipLaunchKernelGGL(k_bin_count, dim3(s->n_seg), dim3(kBinThreads), lds, s->stream, pos, b);
+        if (ev) (void)hipEventRecord(ev[2], s->stream);
+        hipLaunchKernelGGL(k_bin_place, dim3(s->n_seg), dim3(kBinThreads), lds, s->stream, pos, b);
+        if (ev) (void)hipEventRecord(ev[3], s->stream);
+        s->bcur = nb;
+    } else {
+        const GridDev g = s->grid_cur();
+        const uint32_t n_part = g.m / kScanTile;
+        hipLaunchKernelGGL(k_scan_reduce, dim3(n_part), dim3(kBlock), 0, s->stream, g.cnt, g.part);
+        if (ev) (void)hipEventRecord(ev[2], s->stream);
+        hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kPartialsThreads), 0, s->stream, g.part, n_part,
+                           g.start + g.m, s->blk_stats, 0u, s->ctr);
+        if (ev) (void)hipEventRecord(ev[3], s->stream);
+        hipLaunchKernelGGL(k_scan_down, dim3(n_part), dim3(kBlock), 0, s->stream, g.cnt, g.part, g.start);
+        if (ev) (void)hipEventRecord(ev[4], s->stream);
+        hipLaunchKernelGGL(k_scatter, dim3(agent_blocks(s->n)), dim3(kBlock), 0, s->stream, s->agents(s->cur),
+                           g);
+        if (ev) (void)hipEventRecord(ev[5], s->stream);
+    }
+    s->stats_pending = true;  // folded by the next collision dispatch's finish pass
+    return check_launch(s, "move dispatch");
+}
+
+// Folds advance statistics that no collision dispatch has folded yet.
+void flush_stats(msim_sim* s) {
+    if (s->stats_pending) {
+        hipLaunchKernelGGL(k_fold_stats, dim3(1), dim3(kPartialsThreads), 0, s->stream, s->blk_stats,
+                           agent_blocks(s->n), s->ctr);
+        s->stats_pending = false;
+    }
+}
+
 // Move dispatch: advance + bucket table rebuild.  ev (optional) receives an
-// event record after each of the 5 launches.
+// event record after each of the 5 launches of the atomic path.
 int enqueue_move(msim_sim* s, int mode, bool write_dir, hipEvent_t* ev = nullptr) {
+    flush_stats(s);
+    if (mode == kMoveAll) {
+        return enqueue_move_fast(s, write_dir, ev);
+    }
     switch (mode) {
         case kMoveAll:
             launch_advance_mode<kMoveAll>(s, write_dir);
@@ -171,12 +306,13 @@ int enqueue_move(msim_sim* s, int mode, bool write_dir, hipEvent_t* ev = nullptr
             break;
     }
     if (ev) (void)hipEventRecord(ev[0], s->stream);
-    const GridDev& g = s->grid;
+    const GridDev g = s->grid_cur();
     const uint32_t n_part = g.m / kScanTile;
     hipLaunchKernelGGL(k_scan_reduce, dim3(n_part), dim3(kBlock), 0, s->stream, g.cnt, g.part);
     if (ev) (void)hipEventRecord(ev[1], s->stream);
+    const uint32_t n_stat = (mode == kBinOnly) ? 0u : agent_blocks(s->n);
     hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kPartialsThreads), 0, s->stream, g.part, n_part,
-                       g.start + g.m);
+                       g.start + g.m, s->blk_stats, n_stat, s->ctr);
     if (ev) (void)hipEventRecord(ev[2], s->stream);
     hipLaunchKernelGGL(k_scan_down, dim3(n_part), dim3(kBlock), 0, s->stream, g.cnt, g.part, g.start);
     if (ev) (void)hipEventRecord(ev[3], s->stream);
@@ -189,23 +325,33 @@ int enqueue_move(msim_sim* s, int mode, bool write_dir, hipEvent_t* ev = nullptr
 int enqueue_collide(msim_sim* s, bool slow, hipEvent_t* ev = nullptr) {
     const AgentsDev a = s->agents(s->cur);
     const dim3 g(agent_blocks(s->n));
+    const uint32_t n_stat = s->stats_pending ? g.x : 0u;
+    s->stats_pending = false;
     if (slow) {
-        hipLaunchKernelGGL(k_collide<true>, g, dim3(kBlock), 0, s->stream, a, s->grid, s->ctr, s->hist,
-                           s->hist_cap - 1);
+        hipLaunchKernelGGL(k_collide<true>, g, dim3(kBlock), 0, s->stream, a, s->grid_cur(), s->blk_pairs);
+        if (ev) (void)hipEventRecord(ev[0], s->stream);
+        hipLaunchKernelGGL(k_collide_finish<true>, dim3(1), dim3(kPartialsThreads), 0, s->stream, s->blk_pairs,
+                           g.x, s->ctr, s->hist, s->hist_cap - 1, s->blk_stats, n_stat);
         hipLaunchKernelGGL(k_set_init, g, dim3(kBlock), 0, s->stream, a.init, s->n);
     } else {
-        hipLaunchKernelGGL(k_collide<false>, g, dim3(kBlock), 0, s->stream, a, s->grid, s->ctr, s->hist,
-                           s->hist_cap - 1);
+        hipLaunchKernelGGL(k_collide<false>, g, dim3(kBlock), 0, s->stream, a, s->grid_cur(), s->blk_pairs);
+        if (ev) (void)hipEventRecord(ev[0], s->stream);
+        hipLaunchKernelGGL(k_collide_finish<false>, dim3(1), dim3(kPartialsThreads), 0, s->stream,
+                           s->blk_pairs, g.x, s->ctr, s->hist, s->hist_cap - 1, s->blk_stats, n_stat);
     }
-    if (ev) (void)hipEventRecord(ev[0], s->stream);
+    if (ev) (void)hipEventRecord(ev[1], s->stream);
     return check_launch(s, "collision dispatch");
 }
 
 void drop_graphs(msim_sim* s) {
-    for (auto& g : s->gexec) {
-        if (g) {
-            (void)hipGraphExecDestroy(g);
-            g = nullptr;
+    for (auto& a : s->gexec) {
+        for (auto& b : a) {
+            for (auto& g : b) {
+                if (g) {
+                    (void)hipGraphExecDestroy(g);
+                    g = nullptr;
+                }
+            }
         }
     }
 }
@@ -213,7 +359,7 @@ void drop_graphs(msim_sim* s) {
 int reorder(msim_sim* s) {
     const int nxt = 1 - s->cur;
     hipLaunchKernelGGL(k_reorder, dim3(agent_blocks(s->n)), dim3(kBlock), 0, s->stream, s->agents(s->cur),
-                       s->agents(nxt), s->grid);
+                       s->agents(nxt), s->grid_cur());
     int rc = check_launch(s, "k_reorder");
     if (rc != MSIM_OK) {
         return rc;
@@ -243,8 +389,11 @@ int wait_stream(msim_sim* s) {
     }
 }
 
+// Captures graph_ticks fused ticks starting from the current (cur, bcur)
+// buffers; graph_ticks is even, so a replay returns to the same bcur.
 int build_graph(msim_sim* s) {
     hipGraph_t graph = nullptr;
+    const int cur0 = s->cur, bcur0 = s->bcur, qpar0 = s->qpar;
     MSIM_TRY_HIP(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
     int rc = MSIM_OK;
     for (uint32_t t = 0; t < s->graph_ticks && rc == MSIM_OK; t++) {
@@ -254,6 +403,8 @@ int build_graph(msim_sim* s) {
         }
     }
     hipError_t e = hipStreamEndCapture(s->stream, &graph);
+    s->bcur = bcur0;  // capture only recorded the ticks; nothing ran
+    s->qpar = qpar0;
     if (rc != MSIM_OK) {
         if (graph) (void)hipGraphDestroy(graph);
         return rc;
@@ -261,10 +412,10 @@ int build_graph(msim_sim* s) {
     if (e != hipSuccess) {
         return fail(s, MSIM_ERR_HIP, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
     }
-    e = hipGraphInstantiate(&s->gexec[s->cur], graph, nullptr, nullptr, 0);
+    e = hipGraphInstantiate(&s->gexec[cur0][bcur0][qpar0], graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (e != hipSuccess) {
-        s->gexec[s->cur] = nullptr;
+        s->gexec[cur0][bcur0][qpar0] = nullptr;
         return fail(s, MSIM_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
     }
     return MSIM_OK;
@@ -372,6 +523,21 @@ extern "C" int msim_create(const msim_road* roads, uint32_t n_roads, const uint3
     if (tlog < 12 || tlog > 28) {
         return fail(nullptr, MSIM_ERR_INVALID, "table_log2 must be in [12, 28]");
     }
+    const uint32_t rows_log = extent_rows_log2(roads, n_roads, entities, n_entities, cell);
+    const uint32_t lw = o.table_width_log2 ? o.table_width_log2 : auto_width_log2(tlog, rows_log);
+    if (lw < 2 || lw + 2 > tlog) {
+        return fail(nullptr, MSIM_ERR_INVALID, "table_width_log2 must be in [2, table_log2 - 2]");
+    }
+    const uint32_t seg_log = o.segment_log2 ? o.segment_log2 : auto_segment_log2(tlog, lw);
+    if (seg_log < 9 || seg_log > 14 || seg_log > tlog) {
+        return fail(nullptr, MSIM_ERR_INVALID, "segment_log2 must be in [9, 14] and <= table_log2");
+    }
+    if (o.binning > 1) {
+        return fail(nullptr, MSIM_ERR_INVALID, "binning must be 0 (segment) or 1 (atomic)");
+    }
+    if (!validate_extent(roads, n_roads, entities, n_entities, 1.0f / cell, &why)) {
+        return fail(nullptr, MSIM_ERR_INVALID, "%s", why.c_str());
+    }
 
     msim_sim* s = new msim_sim();
     s->n = n_entities;
@@ -379,7 +545,7 @@ extern "C" int msim_create(const msim_road* roads, uint32_t n_roads, const uint3
     s->n_conns = n_connections;
     s->pc = *pc;
     s->reorder_interval = o.reorder_interval;
-    s->graph_ticks = o.graph_ticks ? o.graph_ticks : 32;
+    s->graph_ticks = o.graph_ticks ? (o.graph_ticks + 1u) & ~1u : 32u;  // even: replays keep bcur
     s->watchdog_ms = o.watchdog_ms ? o.watchdog_ms : 600000;
     s->hist_cap = next_pow2_u32(o.history_capacity ? o.history_capacity : 4096);
     s->uniform = (speed == nullptr);
@@ -401,15 +567,39 @@ extern "C" int msim_create(const msim_road* roads, uint32_t n_roads, const uint3
         return fail(nullptr, MSIM_ERR_HIP, "hipStreamCreate failed");
     }
 
+    // LDS per workgroup bounds the segment size (4 bytes per bucket counter).
+    int lds_max = 65536;
+    (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, s->device);
+    uint32_t seg_use = seg_log;
+    while (seg_use > 9 && (static_cast<size_t>(4) << seg_use) + 1024 > static_cast<size_t>(lds_max)) seg_use--;
+    if (o.segment_log2 && seg_use != seg_log) {
+        msim_destroy(s);
+        return fail(nullptr, MSIM_ERR_INVALID, "segment_log2 %u needs more LDS than the device's %d bytes",
+                    seg_log, lds_max);
+    }
+
     const uint32_t n = n_entities;
     GridDev& g = s->grid;
     g.inv_cell = 1.0f / cell;
     g.radius = radius;
     g.d2_limit = d2_limit_for(radius);
-    g.lw = (tlog + 1) / 2;
+    g.m = 1u << tlog;
+    g.lw = lw;
     g.wmask = (1u << g.lw) - 1u;
     g.hmask = (1u << (tlog - g.lw)) - 1u;
-    g.m = 1u << tlog;
+    s->binning = static_cast<int>(o.binning);
+    s->seg_log = seg_use;
+    s->n_seg = g.m >> seg_use;
+    // An agent moves at most max_speed per axis per tick (the heading has
+    // |dir| = speed up to rounding), i.e. at most D cells; in linear bucket
+    // order that is at most (D + 1) rows plus D columns.
+    float smax = kShaderSpeed;
+    if (speed) {
+        smax = 0.0f;
+        for (uint32_t i = 0; i < n_entities; i++) smax = std::max(smax, speed[i]);
+    }
+    const uint32_t D = static_cast<uint32_t>(std::floor(smax * (1.0f + 1.0f / 1048576.0f) / cell)) + 1u;
+    s->margin = (D + 1u) * (g.wmask + 1u) + D;
 
 #define ALLOC(p, cnt)                                   \
     if (rc == MSIM_OK) rc = dalloc_n(s, &(p), (cnt));
@@ -428,12 +618,21 @@ extern "C" int msim_create(const msim_road* roads, uint32_t n_roads, const uint3
         ALLOC(b.perm, n);
     }
     ALLOC(g.cnt, g.m);
-    ALLOC(g.start, g.m + 1);
     ALLOC(g.part, g.m / kScanTile);
     ALLOC(g.rank, n);
-    ALLOC(g.bpos, n);
-    ALLOC(g.bid, n);
+    for (int k = 0; k < 2; k++) {
+        ALLOC(s->bstart[k], g.m + 1);
+        ALLOC(s->bbpos[k], n);
+        ALLOC(s->bbid[k], n);
+    }
+    ALLOC(s->seg_total, s->n_seg);
+    s->q_cap = ceil_div(agent_blocks(n), kQueues) * kBlock;
+    s->q_blocks = std::min(64u, std::max(1u, ceil_div(n, kQueues * kBlock * 16u)));
+    ALLOC(s->q_items, static_cast<size_t>(kQueues) * s->q_cap);
+    ALLOC(s->q_count, 2 * kQueues * kQueueStride);
     ALLOC(s->ctr, 1);
+    ALLOC(s->blk_stats, 2 * static_cast<size_t>(agent_blocks(n)));
+    ALLOC(s->blk_pairs, agent_blocks(n));
     ALLOC(s->hist, s->hist_cap);
 #undef ALLOC
     if (rc != MSIM_OK) {
@@ -480,6 +679,10 @@ extern "C" int msim_create(const msim_road* roads, uint32_t n_roads, const uint3
     if (rc == MSIM_OK) {
         hipError_t e = hipMemset(g.cnt, 0, sizeof(uint32_t) * g.m);
         if (e == hipSuccess) e = hipMemset(s->ctr, 0, sizeof(Counters));
+        if (e == hipSuccess) e = hipMemset(s->q_count, 0, sizeof(uint32_t) * 2 * kQueues * kQueueStride);
+        // every binned id slot always holds a valid agent index
+        for (int k = 0; k < 2 && e == hipSuccess; k++) e = hipMemset(s->bbid[k], 0, sizeof(uint32_t) * n);
+        for (int k = 0; k < 2 && e == hipSuccess; k++) e = hipMemset(s->bstart[k], 0, sizeof(uint32_t) * (g.m + 1));
         if (e == hipSuccess) e = hipMemset(s->hist, 0, sizeof(uint32_t) * s->hist_cap);
         if (e != hipSuccess) rc = fail(s, MSIM_ERR_HIP, "hipMemset failed: %s", hipGetErrorString(e));
     }
@@ -520,11 +723,11 @@ extern "C" int msim_run_ticks(msim_sim* s, uint32_t n_ticks) {
         const uint32_t until_reorder =
             s->reorder_interval ? s->reorder_interval - s->since_reorder : UINT32_MAX;
         if (s->graph_ticks > 0 && left - 1 >= s->graph_ticks && until_reorder >= s->graph_ticks) {
-            if (!s->gexec[s->cur]) {
+            if (!s->gexec[s->cur][s->bcur][s->qpar]) {
                 rc = build_graph(s);
                 if (rc != MSIM_OK) break;
             }
-            hipError_t e = hipGraphLaunch(s->gexec[s->cur], s->stream);
+            hipError_t e = hipGraphLaunch(s->gexec[s->cur][s->bcur][s->qpar], s->stream);
             if (e != hipSuccess) return fail(s, MSIM_ERR_HIP, "hipGraphLaunch: %s", hipGetErrorString(e));
             s->pc.tick += 2 * s->graph_ticks;
             s->ticks_simulated += s->graph_ticks;
@@ -545,9 +748,21 @@ extern "C" int msim_run_ticks(msim_sim* s, uint32_t n_ticks) {
     return rc;
 }
 
+int check_faults(msim_sim* s) {
+    Counters c{};
+    MSIM_TRY_HIP(s, hipMemcpyAsync(&c, s->ctr, sizeof(c), hipMemcpyDeviceToHost, s->stream));
+    int rc = wait_stream(s);
+    if (rc != MSIM_OK) return rc;
+    if (c.faults) {
+        return fail(s, MSIM_ERR_STATE, "device-side check failed (fault bits 0x%x%s)", c.faults,
+                    (c.faults & kFaultBinCount) ? ": binned agent count != agent count" : "");
+    }
+    return MSIM_OK;
+}
+
 extern "C" int msim_sync(msim_sim* s) {
     if (!s) return fail(s, MSIM_ERR_INVALID, "NULL handle");
-    return wait_stream(s);
+    return check_faults(s);
 }
 
 namespace {
@@ -569,7 +784,7 @@ int fetch_state(msim_sim* s, HostState& h, bool need_dir, bool need_rng) {
     if (rc == MSIM_OK) rc = download(s, h.init, b.init, s->n);
     if (rc == MSIM_OK) rc = download(s, h.ncount, b.ncount, s->n);
     if (rc == MSIM_OK) rc = download(s, h.perm, b.perm, s->n);
-    if (rc == MSIM_OK) rc = wait_stream(s);
+    if (rc == MSIM_OK) rc = check_faults(s);
     return rc;
 }
 
@@ -636,6 +851,7 @@ extern "C" int msim_read_state(msim_sim* s, float* pos, float* target, float* di
 
 extern "C" int msim_read_debug(msim_sim* s, uint32_t out[10]) {
     if (!s || !out) return fail(s, MSIM_ERR_INVALID, "NULL argument");
+    flush_stats(s);
     Counters c{};
     MSIM_TRY_HIP(s, hipMemcpyAsync(&c, s->ctr, sizeof(c), hipMemcpyDeviceToHost, s->stream));
     int rc = wait_stream(s);
@@ -698,7 +914,14 @@ extern "C" int msim_get_info(msim_sim* s, msim_info* out) {
     out->uniform_speed = s->uniform ? 1u : 0u;
     out->n_uninitialized = s->n_uninit;
     out->device_bytes = s->bytes;
+    out->kernels_per_tick = kernels_per_tick(s);
+    out->binning = static_cast<uint32_t>(s->binning);
     return MSIM_OK;
+}
+
+extern "C" const char* msim_kernel_name(const msim_sim* s, uint32_t k) {
+    if (!s || k >= kernels_per_tick(s)) return nullptr;
+    return s->binning ? kAtomicKernels[k] : kFastKernels[k];
 }
 
 extern "C" int msim_time_ticks(msim_sim* s, uint32_t n_ticks, float* ms_total, float* kernel_ms,
@@ -706,8 +929,9 @@ extern "C" int msim_time_ticks(msim_sim* s, uint32_t n_ticks, float* ms_total, f
     if (!s) return fail(s, MSIM_ERR_INVALID, "NULL handle");
     if (s->n_uninit != 0) return fail(s, MSIM_ERR_STATE, "agents are not initialised yet; dispatch first");
     if ((s->pc.tick + 1) & 1u) return fail(s, MSIM_ERR_STATE, "next dispatch is not a move dispatch");
-    constexpr int kNk = 6;
-    hipEvent_t ev[kNk + 1];
+    constexpr int kMaxK = 8;
+    const int nk = static_cast<int>(kernels_per_tick(s));
+    hipEvent_t ev[kMaxK + 1];
     for (auto& e : ev) MSIM_TRY_HIP(s, hipEventCreate(&e));
     int rc = MSIM_OK;
     if (ms_total) {
@@ -718,22 +942,22 @@ extern "C" int msim_time_ticks(msim_sim* s, uint32_t n_ticks, float* ms_total, f
         if (rc == MSIM_OK) (void)hipEventElapsedTime(ms_total, ev[0], ev[1]);
     }
     if (rc == MSIM_OK && kernel_ms && n_kernel_ms) {
-        double acc[kNk] = {0, 0, 0, 0, 0, 0};
+        double acc[kMaxK] = {0, 0, 0, 0, 0, 0, 0, 0};
         for (uint32_t t = 0; t < n_ticks && rc == MSIM_OK; t++) {
             (void)hipEventRecord(ev[0], s->stream);
             rc = enqueue_move(s, kMoveAll, t + 1 == n_ticks, ev + 1);
-            if (rc == MSIM_OK) rc = enqueue_collide(s, false, ev + 6);
+            if (rc == MSIM_OK) rc = enqueue_collide(s, false, ev + (nk - 1));
             s->pc.tick += 2;
             s->any_collide = true;
             if (rc == MSIM_OK) rc = after_move(s);
             if (rc == MSIM_OK) rc = wait_stream(s);
-            for (int k = 0; k < kNk && rc == MSIM_OK; k++) {
+            for (int k = 0; k < nk && rc == MSIM_OK; k++) {
                 float ms = 0.f;
                 (void)hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
                 acc[k] += ms;
             }
         }
-        for (uint32_t k = 0; k < n_kernel_ms && k < static_cast<uint32_t>(kNk); k++) {
+        for (uint32_t k = 0; k < n_kernel_ms && k < static_cast<uint32_t>(nk); k++) {
             kernel_ms[k] = static_cast<float>(acc[k]);
         }
     }

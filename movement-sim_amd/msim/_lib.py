@@ -35,6 +35,9 @@ class Options(ctypes.Structure):
         ("history_capacity", ctypes.c_uint32),
         ("device", ctypes.c_int32),
         ("watchdog_ms", ctypes.c_uint32),
+        ("table_width_log2", ctypes.c_uint32),
+        ("segment_log2", ctypes.c_uint32),
+        ("binning", ctypes.c_uint32),
     ]
 
 
@@ -52,6 +55,8 @@ class Info(ctypes.Structure):
         ("uniform_speed", ctypes.c_uint32),
         ("n_uninitialized", ctypes.c_uint32),
         ("device_bytes", ctypes.c_uint64),
+        ("kernels_per_tick", ctypes.c_uint32),
+        ("binning", ctypes.c_uint32),
     ]
 
 
@@ -70,6 +75,7 @@ SIGNATURES = {
     "msim_read_cells": (ctypes.c_int, [_VP, _VP]),
     "msim_get_info": (ctypes.c_int, [_VP, _VP]),
     "msim_time_ticks": (ctypes.c_int, [_VP, _U32, _VP, _VP, _U32]),
+    "msim_kernel_name": (ctypes.c_char_p, [_VP, _U32]),
     "msim_get_stream": (ctypes.c_int, [_VP, ctypes.POINTER(_VP)]),
     "msim_destroy": (ctypes.c_int, [_VP]),
     "msim_last_error": (ctypes.c_char_p, [_VP]),
@@ -124,7 +130,7 @@ class Simulation:
         roads = np.ascontiguousarray(roads, ROAD_DTYPE)
         conns = np.ascontiguousarray(connections, np.uint32)
         ents = np.ascontiguousarray(entities, ENTITY_DTYPE)
-        pcv = np.ascontiguousarray(np.asarray(pc, PUSH_DTYPE).reshape(()))
+        pcv = np.array(pc, PUSH_DTYPE).reshape(())  # 0-d record
         spd = None if speed is None else np.ascontiguousarray(speed, np.float32)
         if spd is not None and spd.shape[0] != ents.shape[0]:
             raise ValueError("speed must have one entry per agent")
@@ -223,11 +229,22 @@ class Simulation:
         self._check(self.lib.msim_get_info(self.h, ctypes.byref(i)))
         return {f: getattr(i, f) for f, _ in Info._fields_}
 
+    def kernel_names(self):
+        out = []
+        while True:
+            nm = self.lib.msim_kernel_name(self.h, len(out))
+            if not nm:
+                return out
+            out.append(nm.decode())
+
     def time_ticks(self, n, kernels=True):
+        """(ms for n ticks through the graph path, {kernel: summed ms over a
+        second pass of n ticks with events around every launch})."""
         total = np.zeros(1, np.float32)
-        k = np.zeros(6, np.float32)
-        self._check(self.lib.msim_time_ticks(self.h, n, _ptr(total), _ptr(k) if kernels else None, 6))
-        return float(total[0]), k
+        names = self.kernel_names()
+        k = np.zeros(len(names), np.float32)
+        self._check(self.lib.msim_time_ticks(self.h, n, _ptr(total), _ptr(k) if kernels else None, len(names)))
+        return float(total[0]), dict(zip(names, (float(v) for v in k)))
 
     def stream(self):
         s = ctypes.c_void_p()

@@ -26,19 +26,21 @@ B_COLLIDE = 16
 B_FUSED = B_ADVANCE + B_SCAN + B_SCATTER + B_COLLIDE
 assert B_FUSED == 64
 
-# Order of msim_time_ticks' per-kernel durations.
-KERNELS = ("k_advance_bin", "k_scan_reduce", "k_scan_partials", "k_scan_down", "k_scatter", "k_collide")
 
 
 def kernel_bytes(name, uniform_speed=False):
-    """Algorithmic bytes per agent for one launch of the named kernel; the
-    three scan kernels share the scan's 8 B."""
-    if name == "k_advance_bin":
+    """Algorithmic bytes per agent for one launch of the named kernel.
+
+    k_advance 28 (24 with the shader's constant speed); k_bin_count 8 (bucket
+    offsets) and k_bin_place 12 (binned position + id) in the segment path, or
+    k_scan_down 8 and k_scatter 12 in the atomic path; k_collide 16.  k_route
+    (arrival path), the single-workgroup scan/finish passes: 0."""
+    if name in ("k_advance", "k_advance_bin"):
         return B_ADVANCE_UNIFORM_SPEED if uniform_speed else B_ADVANCE
-    if name in ("k_scan_reduce", "k_scan_partials", "k_scan_down"):
+    if name in ("k_bin_count", "k_scan_down"):
         return B_SCAN
-    if name == "k_scatter":
+    if name in ("k_bin_place", "k_scatter"):
         return B_SCATTER
     if name == "k_collide":
         return B_COLLIDE
-    raise KeyError(name)
+    return 0
