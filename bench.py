@@ -59,8 +59,7 @@ def main():
     ap.add_argument("--profile-ticks", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--reorder", type=int, default=16)
-    ap.add_argument("--binning", type=int, default=0, choices=(0, 1))
+    ap.add_argument("--untuned", action="store_true", help="library defaults instead of msim/tuned.json")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only; multi-GPU runs are not part of it "
@@ -77,7 +76,9 @@ def main():
     n = sc.entities.shape[0]
     print(f"# workload config {args.config}: {n} agents, {sc.roads.shape[0]} roads, "
           f"{sc.conns.shape[0]} connection entries (generated in {gen_s:.1f} s)", file=sys.stderr, flush=True)
-    opts = {"reorder_interval": args.reorder, "binning": args.binning}
+    from msim import autotune
+
+    opts = dict(autotune.DEFAULT) if args.untuned else autotune.tuned_options(sc)
     sim = msim.Simulation(sc.roads, sc.conns, sc.entities, sc.pc, speed=sc.speed, **opts)
     info = sim.info()
 
@@ -133,8 +134,7 @@ def main():
             "collision_radius_m": float(sc.pc["collisionRadius"]),
             "cell_m": info["cell_size"],
             "bucket_table": [info["table_width"], info["table_height"]],
-            "binning": "segment-owned (LDS histograms, look-back)" if args.binning == 0 else "global atomics",
-            "reorder_interval": args.reorder,
+            "options": opts,
             "graph_ticks": 32,
         },
         "roofline": {

@@ -51,7 +51,12 @@ typedef struct msim_options {
     uint32_t table_width_log2; /* 0: auto; buckets per table row = 2^this      */
     uint32_t segment_log2;     /* 0: auto; buckets per rebinning workgroup     */
     uint32_t binning;          /* 0: segment-owned rebinning (LDS histograms,
-                                  look-back scan); 1: global-atomic counting   */
+                                  per-segment offsets); 1: global-atomic counting */
+    uint32_t route_queue;      /* 0: arrivals routed inside the advance kernel;
+                                  1: queued and routed by a separate pass       */
+    uint32_t collide_wave_uniform; /* 0: every lane walks its own candidates;
+                                  1: waves inside one bucket range share one
+                                  candidate walk (readlane broadcast)           */
 } msim_options;
 
 typedef struct msim_info {

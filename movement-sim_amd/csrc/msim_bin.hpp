@@ -35,9 +35,13 @@ struct QueueDev {
     uint32_t cap;          // capacity per queue
 };
 
-template <bool UNIFORM, bool WRITE_DIR, bool ATOMIC_BIN>
+// ROUTE_INLINE: arrivals take the route choice in this kernel (no queue, no
+// k_route launch); better when arrivals are cheap to gather (small maps).
+template <bool UNIFORM, bool WRITE_DIR, bool ATOMIC_BIN, bool ROUTE_INLINE>
 __global__ __launch_bounds__(kBlock) void k_advance(AgentsDev a, GridDev g, QueueDev q,
-                                                    uint32_t* __restrict__ blk_stats) {
+                                                    unsigned long long* __restrict__ part_stats,
+                                                    const msim_road* __restrict__ roads,
+                                                    const uint32_t* __restrict__ conns) {
     __shared__ uint32_t lds[2 * (kBlock / 64)];
     __shared__ uint32_t sh_base;
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
@@ -47,6 +51,7 @@ __global__ __launch_bounds__(kBlock) void k_advance(AgentsDev a, GridDev g, Queu
     }
     int inserted = 0;
     int arrived = 0;
+    uint32_t bkey = 0;
     if (i < a.n) {
         const float2 p = a.pos[i];
         const float2 t = a.tgt[i];
@@ -66,14 +71,31 @@ __global__ __launch_bounds__(kBlock) void k_advance(AgentsDev a, GridDev g, Queu
         } else {
             np = t;
             arrived = 1;
+            if (ROUTE_INLINE) {
+                float2 nt = t;
+                uint32_t r = a.road[i];
+                uint4 st = a.rng[i];
+                choose_route(nt, r, st, roads, conns);
+                a.tgt[i] = nt;
+                a.road[i] = r;
+                a.rng[i] = st;
+                if (WRITE_DIR) {
+                    a.dir[i] = heading(nt, np, s);
+                }
+            }
         }
         a.pos[i] = np;
         inserted = static_cast<int>(cell_coord(np.x, g.inv_cell) != cell_coord(p.x, g.inv_cell)) |
                    static_cast<int>(cell_coord(np.y, g.inv_cell) != cell_coord(p.y, g.inv_cell));
         if (ATOMIC_BIN) {
-            const uint32_t b = bucket_of(cell_coord(np.x, g.inv_cell), cell_coord(np.y, g.inv_cell), g.lw,
-                                         g.wmask, g.hmask);
-            g.rank[i] = atomicAdd(&g.cnt[b], 1u);
+            bkey = bucket_of(cell_coord(np.x, g.inv_cell), cell_coord(np.y, g.inv_cell), g.lw, g.wmask,
+                             g.hmask);
+        }
+    }
+    if (ATOMIC_BIN) {
+        const uint32_t r = wave_rank_add(g.cnt, bkey, i < a.n);
+        if (i < a.n) {
+            g.rank[i] = r;
         }
     }
     // block-aggregated append of the arrivals to queue blk % kQueues
@@ -94,12 +116,12 @@ __global__ __launch_bounds__(kBlock) void k_advance(AgentsDev a, GridDev g, Queu
     }
     const uint32_t qi = blk % kQueues;
     if (threadIdx.x == 0) {
-        sh_base = tot_a ? atomicAdd(&q.count[qi * kQueueStride], tot_a) : 0u;
-        blk_stats[2 * blk] = tot_i;
-        blk_stats[2 * blk + 1] = tot_a;
+        sh_base = (tot_a && !ROUTE_INLINE) ? atomicAdd(&q.count[qi * kQueueStride], tot_a) : 0u;
+        slot_add(part_stats, blk, kWordInserts, tot_i);
+        slot_add(part_stats, blk, kWordArrivals, tot_a);
     }
     __syncthreads();
-    if (arrived) {
+    if (!ROUTE_INLINE && arrived) {
         const unsigned long long lt = (1ull << __lane_id()) - 1ull;
         const uint32_t slot = sh_base + before + static_cast<uint32_t>(__popcll(am & lt));
         q.items[qi * q.cap + slot] = i;
@@ -171,6 +193,44 @@ __device__ __forceinline__ uint32_t bucket_of_pos(float2 p, const BinDev& b) {
     return bucket_of(cell_coord(p.x, b.inv_cell), cell_coord(p.y, b.inv_cell), b.lw, b.wmask, b.hmask);
 }
 
+constexpr uint32_t kBinUnroll = 4;
+
+// One pass over the window slots [lo, hi): every agent whose new bucket lies
+// in the segment bumps its LDS counter (PLACE: and is written to the slot the
+// counter hands out).  The trip count is block-uniform, so the wave-level rank
+// helper always runs with the whole wave; kBinUnroll independent loads per
+// thread keep enough requests in flight.
+template <bool PLACE>
+__device__ __forceinline__ void bin_pass(const float2* __restrict__ pos, const BinDev& b, uint32_t s0,
+                                         uint32_t lo, uint32_t hi, uint32_t* table) {
+    const uint32_t S = 1u << b.seg_log;
+    for (uint32_t base = lo; base < hi; base += kBinUnroll * kBinThreads) {
+        uint32_t id[kBinUnroll];
+        float2 p[kBinUnroll];
+#pragma unroll
+        for (uint32_t u = 0; u < kBinUnroll; u++) {
+            const uint32_t k = base + u * kBinThreads + threadIdx.x;
+            id[u] = k < hi ? b.old_bid[k] : 0u;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kBinUnroll; u++) {
+            const uint32_t k = base + u * kBinThreads + threadIdx.x;
+            p[u] = k < hi ? pos[id[u]] : make_float2(0.0f, 0.0f);
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kBinUnroll; u++) {
+            const uint32_t k = base + u * kBinThreads + threadIdx.x;
+            const uint32_t off = bucket_of_pos(p[u], b) - s0;
+            const bool mine = (k < hi) && (off < S);
+            const uint32_t slot = wave_rank_add(table, off, mine);
+            if (PLACE && mine) {
+                b.new_bpos[slot] = p[u];
+                b.new_bid[slot] = id[u];
+            }
+        }
+    }
+}
+
 // Counts the window agents that land in segment blockIdx.x, writes the
 // segment-local exclusive offsets to new_start and the total to seg_total.
 __global__ __launch_bounds__(kBinThreads) void k_bin_count(const float2* __restrict__ pos, BinDev b) {
@@ -186,16 +246,8 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_count(const float2* __restr
     __syncthreads();
     uint32_t a0, a1, c0, c1;
     bin_window(b, s0, a0, a1, c0, c1);
-    for (int r = 0; r < 2; r++) {
-        const uint32_t lo = r ? c0 : a0;
-        const uint32_t hi = r ? c1 : a1;
-        for (uint32_t k = lo + tid; k < hi; k += kBinThreads) {
-            const uint32_t off = bucket_of_pos(pos[b.old_bid[k]], b) - s0;
-            if (off < S) {
-                atomicAdd(&hist[off], 1u);
-            }
-        }
-    }
+    bin_pass<false>(pos, b, s0, a0, a1, hist);
+    bin_pass<false>(pos, b, s0, c0, c1, hist);
     __syncthreads();
     const uint32_t per = S / kBinThreads;
     uint32_t run = 0;
@@ -242,20 +294,8 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_place(const float2* __restr
     __syncthreads();
     uint32_t a0, a1, c0, c1;
     bin_window(b, s0, a0, a1, c0, c1);
-    for (int r = 0; r < 2; r++) {
-        const uint32_t lo = r ? c0 : a0;
-        const uint32_t hi = r ? c1 : a1;
-        for (uint32_t k = lo + tid; k < hi; k += kBinThreads) {
-            const uint32_t i = b.old_bid[k];
-            const float2 p = pos[i];
-            const uint32_t off = bucket_of_pos(p, b) - s0;
-            if (off < S) {
-                const uint32_t slot = atomicAdd(&cursor[off], 1u);
-                b.new_bpos[slot] = p;
-                b.new_bid[slot] = i;
-            }
-        }
-    }
+    bin_pass<true>(pos, b, s0, a0, a1, cursor);
+    bin_pass<true>(pos, b, s0, c0, c1, cursor);
 }
 
 }  // namespace msim

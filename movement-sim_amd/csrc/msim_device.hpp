@@ -121,16 +121,17 @@ __device__ __forceinline__ uint32_t bucket_of(int32_t cx, int32_t cy, uint32_t l
     return ((static_cast<uint32_t>(cy) & hmask) << lw) | (static_cast<uint32_t>(cx) & wmask);
 }
 
-// random_move.comp:551-562 quad_tree_in_range, with the final
-// sqrt(d2) < R replaced by the exactly equivalent d2 < d2_limit, where
-// d2_limit is the smallest binary32 x with sqrtf(x) >= R (sqrtf is correctly
-// rounded and monotone, so {x : sqrtf(x) < R} = {x : x < d2_limit}).
-__device__ __forceinline__ bool in_range(float2 pj, float2 pi, float radius, float d2_limit) {
+// random_move.comp:551-562 quad_tree_in_range(pj, pi, R), exactly:
+//   |dx| > R -> false; |dy| > R -> false; sqrt(dx*dx + dy*dy) < R.
+// sqrtf is correctly rounded and non-decreasing, so sqrt(d2) < R <=>
+// d2 < d2_limit (the smallest binary32 x with sqrtf(x) >= R).  The two box
+// tests are implied: |dx| > R gives fl(dx*dx) >= fl(R*R), so d2 >= fl(R*R),
+// and the host checks fl(R*R) >= d2_limit when it picks d2_limit.
+__device__ __forceinline__ bool in_range(float2 pj, float2 pi, float d2_limit) {
     const float ddx = pj.x - pi.x;
     const float ddy = pj.y - pi.y;
     const float d2 = ddx * ddx + ddy * ddy;
-    return static_cast<int>(fabsf(ddx) <= radius) & static_cast<int>(fabsf(ddy) <= radius) &
-           static_cast<int>(d2 < d2_limit);
+    return d2 < d2_limit;
 }
 
 // Workgroup index remapped so that each XCD (workgroups are dealt round-robin

@@ -122,6 +122,14 @@ inline float d2_limit_for(float r) {
     return bits_to_float(lo);
 }
 
+// The neighbour predicate drops quad_tree_in_range's box tests, which is exact
+// when every |dx| > r already fails d2 < limit, i.e. when fl(r*r) >= limit.
+inline bool box_tests_implied(float r, float limit) {
+    volatile float rr = r;
+    const float sq = rr * rr;
+    return sq >= limit;
+}
+
 // Every read random_move.comp:778-828 can make must stay inside the arrays:
 // an endpoint with count >= 3 may read connections[index + count] (the draw
 // can return the top offset), count == 2 reads connections[index + 1].

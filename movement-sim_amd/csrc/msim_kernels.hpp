@@ -69,6 +69,53 @@ enum FaultBits : unsigned int {
     kFaultBinCount = 1u  // binned agents != agents (an agent left its rebinning window)
 };
 
+// Per-dispatch partial sums are accumulated into kSlots 64-bit slots, one per
+// 128-byte line, by one atomic per workgroup on slot (block % kSlots); the
+// single-workgroup finish/fold passes read the kSlots values and clear them.
+constexpr uint32_t kSlots = 64;
+constexpr uint32_t kSlotStride = 16;  // 64-bit words between slots
+enum SlotWord : uint32_t { kWordPairs = 0, kWordInserts = 0, kWordArrivals = 1 };
+
+__device__ __forceinline__ void slot_add(unsigned long long* part, uint32_t blk, uint32_t word,
+                                         unsigned long long v) {
+    if (v) {
+        atomicAdd(&part[(blk % kSlots) * kSlotStride + word], v);
+    }
+}
+
+// Adds 1 to table[key] for every lane with `active` set and returns the lane's
+// pre-increment value (its rank within the key).  Lanes of a wave that sit
+// in the same heavy cell share keys: when the active keys form at most 8
+// runs, the wave issues one atomic per distinct key; otherwise one per lane.
+// Must be called by the whole wave (inactive lanes pass active = false).
+__device__ __forceinline__ uint32_t wave_rank_add(uint32_t* table, uint32_t key, bool active) {
+    const uint32_t lane = __lane_id();
+    const unsigned long long act = __ballot(active);
+    const uint32_t prev = __shfl_up(key, 1, 64);
+    const bool prev_active = lane ? ((act >> (lane - 1)) & 1ull) != 0 : false;
+    const unsigned long long heads = __ballot(active && (!prev_active || key != prev));
+    if (__popcll(heads) > 8) {
+        return active ? atomicAdd(&table[key], 1u) : 0u;
+    }
+    uint32_t rank = 0;
+    unsigned long long todo = act;
+    while (todo) {
+        const int leader = __ffsll(static_cast<long long>(todo)) - 1;
+        const uint32_t k = __shfl(key, leader, 64);
+        const unsigned long long peers = __ballot(active && key == k) & todo;
+        uint32_t base = 0;
+        if (static_cast<int>(lane) == leader) {
+            base = atomicAdd(&table[k], static_cast<uint32_t>(__popcll(peers)));
+        }
+        base = __shfl(base, leader, 64);
+        if ((peers >> lane) & 1ull) {
+            rank = base + static_cast<uint32_t>(__popcll(peers & ((1ull << lane) - 1ull)));
+        }
+        todo &= ~peers;
+    }
+    return rank;
+}
+
 enum AdvanceMode : int {
     kMoveAll = 0,   // every agent is initialised: advance all
     kMoveInit = 1,  // per-agent Entity.initialized: 0 -> insert only, set to 1
@@ -153,13 +200,13 @@ __device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* lds) {
 
 // ---- move dispatch ------------------------------------------------------
 
-// blk_stats[2 * block] / [2 * block + 1]: insertions / arrivals of the block
-// (folded into the counters by k_scan_partials).
+// Insertions / arrivals of the block go to part_stats (kWordInserts,
+// kWordArrivals of the block's slot).
 template <int MODE, bool UNIFORM, bool WRITE_DIR>
 __global__ __launch_bounds__(kBlock) void k_advance_bin(AgentsDev a, GridDev g,
                                                         const msim_road* __restrict__ roads,
                                                         const uint32_t* __restrict__ conns,
-                                                        uint32_t* __restrict__ blk_stats) {
+                                                        unsigned long long* __restrict__ part_stats) {
     __shared__ uint32_t lds[2 * (kBlock / 64)];
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t i = blk * kBlock + threadIdx.x;
@@ -224,8 +271,8 @@ __global__ __launch_bounds__(kBlock) void k_advance_bin(AgentsDev a, GridDev g,
                 si += lds[k];
                 sa += lds[kBlock / 64 + k];
             }
-            blk_stats[2 * blk] = si;
-            blk_stats[2 * blk + 1] = sa;
+            slot_add(part_stats, blk, kWordInserts, si);
+            slot_add(part_stats, blk, kWordArrivals, sa);
         }
     }
 }
@@ -247,30 +294,11 @@ __global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t* __restri
     }
 }
 
-// Exclusive scan of the tile totals (one workgroup); start[m] = total.  Also
-// folds the advance kernel's n_stat per-block statistics into the counters.
+// Exclusive scan of the tile totals (one workgroup); start[m] = total.
 __global__ __launch_bounds__(kPartialsThreads) void k_scan_partials(uint32_t* __restrict__ part,
                                                                     uint32_t n_part,
-                                                                    uint32_t* __restrict__ start_end,
-                                                                    const uint32_t* __restrict__ blk_stats,
-                                                                    uint32_t n_stat,
-                                                                    Counters* __restrict__ ctr) {
+                                                                    uint32_t* __restrict__ start_end) {
     __shared__ uint32_t lds[kPartialsThreads / 64];
-    if (n_stat) {
-        unsigned long long si = 0, sa = 0;
-        for (uint32_t k = threadIdx.x; k < n_stat; k += kPartialsThreads) {
-            si += blk_stats[2 * k];
-            sa += blk_stats[2 * k + 1];
-        }
-        const uint32_t ti = block_sum<kPartialsThreads>(static_cast<uint32_t>(si), lds);
-        __syncthreads();
-        const uint32_t ta = block_sum<kPartialsThreads>(static_cast<uint32_t>(sa), lds);
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            ctr->inserts += ti;
-            ctr->arrivals += ta;
-        }
-    }
     const uint32_t per = (n_part + kPartialsThreads - 1) / kPartialsThreads;
     const uint32_t lo = threadIdx.x * per;
     uint32_t s = 0;
@@ -348,17 +376,71 @@ __global__ __launch_bounds__(kBlock) void k_scatter(AgentsDev a, GridDev g) {
 // when agent i searches (Entity.initialized was 1 at dispatch start), which is
 // the reference's "j < i" rule at random_move.comp:574.  The last workgroup to
 // finish folds the dispatch's pair total into the counters and history.
-template <bool SLOW>
-__global__ __launch_bounds__(kBlock) void k_collide(AgentsDev a, GridDev g, uint32_t* __restrict__ blk_pairs) {
+// UNIF enables the wave-uniform path below (heavy cells).
+template <bool SLOW, bool UNIF>
+__global__ __launch_bounds__(kBlock) void k_collide(AgentsDev a, GridDev g, const float2* __restrict__ bpos,
+                                                    unsigned long long* __restrict__ part_pairs) {
     __shared__ uint32_t lds[kBlock / 64];
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t s = blk * kBlock + threadIdx.x;
+    const bool valid = s < a.n;
     uint32_t cnt = 0;
     uint32_t own = 0;
-    if (s < a.n) {
-        const float2 me = g.bpos[s];
-        const int32_t cx = cell_coord(me.x, g.inv_cell);
-        const int32_t cy = cell_coord(me.y, g.inv_cell);
+    const float T = g.d2_limit;
+    const float2 me = valid ? bpos[s] : make_float2(0.0f, 0.0f);
+    const int32_t cx = cell_coord(me.x, g.inv_cell);
+    const int32_t cy = cell_coord(me.y, g.inv_cell);
+    const uint32_t brow = static_cast<uint32_t>(cy) & g.hmask;
+    const uint32_t bcol = static_cast<uint32_t>(cx) & g.wmask;
+
+    // Wave-uniform path: every valid lane sits in one table row, in columns
+    // [c_lo, c_lo + 2] away from the row ends.  Then the union of the lanes'
+    // 3x3 neighbourhoods is 3 contiguous bucket ranges shared by the wave;
+    // each candidate is fetched once per wave (scalar load) and tested by all
+    // lanes.  A lane's own slot is among the candidates and always passes
+    // (distance 0), so it is subtracted afterwards.
+    bool uniform = false;
+    if (!SLOW && UNIF) {
+        const unsigned long long act = __ballot(valid);
+        const int first = act ? __ffsll(static_cast<long long>(act)) - 1 : 0;
+        const uint32_t row0 = __shfl(brow, first, 64);
+        uint32_t lo = valid ? bcol : 0xFFFFFFFFu;
+        uint32_t hi = valid ? bcol : 0u;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            lo = min(lo, static_cast<uint32_t>(__shfl_xor(static_cast<int>(lo), o, 64)));
+            hi = max(hi, static_cast<uint32_t>(__shfl_xor(static_cast<int>(hi), o, 64)));
+        }
+        const bool same_row = __ballot(valid && brow != row0) == 0ull;
+        uniform = act != 0ull && same_row && lo >= 1u && hi + 2u <= g.wmask + 1u && hi - lo <= 2u;
+        if (uniform) {
+            const uint32_t r0 = __builtin_amdgcn_readfirstlane(row0);
+            const uint32_t c_lo = __builtin_amdgcn_readfirstlane(lo) - 1u;
+            const uint32_t c_hi = __builtin_amdgcn_readfirstlane(hi) + 2u;  // exclusive
+            const uint32_t lane = __lane_id();
+#pragma unroll 1
+            for (int dy = -1; dy <= 1; dy++) {
+                const uint32_t rowb = ((r0 + static_cast<uint32_t>(dy)) & g.hmask) << g.lw;
+                const uint32_t k0 = __builtin_amdgcn_readfirstlane(g.start[rowb + c_lo]);
+                const uint32_t k1 = __builtin_amdgcn_readfirstlane(g.start[rowb + c_hi]);
+                // 64 candidates per coalesced load, broadcast lane by lane
+                for (uint32_t base = k0; base < k1; base += 64u) {
+                    const uint32_t kk = base + lane;
+                    const float2 qv = kk < k1 ? bpos[kk] : make_float2(0.0f, 0.0f);
+                    const uint32_t m = min(64u, k1 - base);
+                    const int qx = __float_as_int(qv.x);
+                    const int qy = __float_as_int(qv.y);
+                    for (uint32_t j = 0; j < m; j++) {
+                        const float2 q = make_float2(__int_as_float(__builtin_amdgcn_readlane(qx, j)),
+                                                     __int_as_float(__builtin_amdgcn_readlane(qy, j)));
+                        cnt += in_range(q, me, T) ? 1u : 0u;
+                    }
+                }
+            }
+            cnt = valid ? cnt - 1u : 0u;  // the lane's own slot
+        }
+    }
+    if (!uniform && valid) {
         const uint32_t me_id = g.bid[s];
         uint32_t me_ext = 0;
         bool me_search = true;
@@ -366,8 +448,6 @@ __global__ __launch_bounds__(kBlock) void k_collide(AgentsDev a, GridDev g, uint
             me_ext = a.perm[me_id];
             me_search = a.init[me_id] != 0u;
         }
-        const float R = g.radius;
-        const float T = g.d2_limit;
 #pragma unroll 1
         for (int dy = -1; dy <= 1; dy++) {
             const uint32_t rowb = (static_cast<uint32_t>(cy + dy) & g.hmask) << g.lw;
@@ -388,12 +468,22 @@ __global__ __launch_bounds__(kBlock) void k_collide(AgentsDev a, GridDev g, uint
             for (int seg = 0; seg < 2; seg++) {
                 const uint32_t lo = seg ? r2 : r0;
                 const uint32_t hi = seg ? r3 : r1;
+                if (!SLOW) {
+                    // four independent candidate loads in flight per step
+                    uint32_t k = lo;
+                    for (; k + 4u <= hi; k += 4u) {
+                        const float2 q0 = bpos[k], q1 = bpos[k + 1], q2 = bpos[k + 2], q3 = bpos[k + 3];
+                        cnt += (in_range(q0, me, T) ? 1u : 0u) + (in_range(q1, me, T) ? 1u : 0u) +
+                               (in_range(q2, me, T) ? 1u : 0u) + (in_range(q3, me, T) ? 1u : 0u);
+                    }
+                    for (; k < hi; k++) {
+                        cnt += in_range(bpos[k], me, T) ? 1u : 0u;
+                    }
+                    continue;
+                }
                 for (uint32_t k = lo; k < hi; k++) {
-                    const float2 q = g.bpos[k];
-                    const bool hit = in_range(q, me, R, T) & (k != s);
-                    if (!SLOW) {
-                        cnt += hit ? 1u : 0u;
-                    } else if (hit) {
+                    const float2 q = bpos[k];
+                    if (in_range(q, me, T) && k != s) {
                         const uint32_t j = g.bid[k];
                         const uint32_t je = a.perm[j];
                         const bool counted = (je < me_ext) ? me_search : (a.init[j] != 0u);
@@ -405,63 +495,16 @@ __global__ __launch_bounds__(kBlock) void k_collide(AgentsDev a, GridDev g, uint
                 }
             }
         }
-        a.ncount[me_id] = cnt;
+    }
+    if (!SLOW && !uniform && valid) {
+        cnt -= 1u;  // the agent's own slot passed the predicate
+    }
+    if (valid) {
+        a.ncount[g.bid[s]] = cnt;
     }
     const uint32_t bs = block_sum<kBlock>(SLOW ? own : cnt, lds);
     if (threadIdx.x == 0) {
-        blk_pairs[blk] = bs;
-    }
-}
-
-// Folds k_collide's per-block sums into the dispatch's pair total: every pair
-// is found from both ends in the fast path, once in the SLOW path.
-// Folds n_stat per-block advance statistics (insertions, arrivals) into the
-// counters; one workgroup.
-__device__ __forceinline__ void fold_stats(const uint32_t* __restrict__ blk_stats, uint32_t n_stat,
-                                           Counters* __restrict__ ctr, unsigned long long* lds) {
-    unsigned long long si = 0, sa = 0;
-    for (uint32_t k = threadIdx.x; k < n_stat; k += kPartialsThreads) {
-        si += blk_stats[2 * k];
-        sa += blk_stats[2 * k + 1];
-    }
-    const unsigned long long ti = block_sum64<kPartialsThreads>(si, lds);
-    __syncthreads();
-    const unsigned long long ta = block_sum64<kPartialsThreads>(sa, lds);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        ctr->inserts += ti;
-        ctr->arrivals += ta;
-    }
-}
-
-__global__ __launch_bounds__(kPartialsThreads) void k_fold_stats(const uint32_t* __restrict__ blk_stats,
-                                                                 uint32_t n_stat, Counters* __restrict__ ctr) {
-    __shared__ unsigned long long lds[kPartialsThreads / 64];
-    fold_stats(blk_stats, n_stat, ctr, lds);
-}
-
-// Also folds n_stat pending advance statistics (0: none pending).
-template <bool SLOW>
-__global__ __launch_bounds__(kPartialsThreads) void k_collide_finish(const uint32_t* __restrict__ blk_pairs,
-                                                                     uint32_t nb, Counters* __restrict__ ctr,
-                                                                     uint32_t* __restrict__ hist,
-                                                                     uint32_t hist_mask,
-                                                                     const uint32_t* __restrict__ blk_stats,
-                                                                     uint32_t n_stat) {
-    __shared__ unsigned long long lds[kPartialsThreads / 64];
-    if (n_stat) {
-        fold_stats(blk_stats, n_stat, ctr, lds);
-    }
-    unsigned long long s = 0;
-    for (uint32_t k = threadIdx.x; k < nb; k += kPartialsThreads) {
-        s += blk_pairs[k];
-    }
-    const unsigned long long tot = block_sum64<kPartialsThreads>(s, lds);
-    if (threadIdx.x == 0) {
-        const unsigned long long pairs = SLOW ? tot : tot / 2;
-        ctr->pairs_total += pairs;
-        hist[ctr->collides & hist_mask] = static_cast<uint32_t>(pairs);
-        ctr->collides += 1;
+        slot_add(part_pairs, blk, kWordPairs, bs);
     }
 }
 
@@ -469,6 +512,61 @@ __global__ __launch_bounds__(kBlock) void k_set_init(uint32_t* init, uint32_t n)
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i < n) {
         init[i] = 1u;
+    }
+}
+
+// Sums word w of the kSlots slots (threads 0..kSlots-1) and clears them;
+// the result is valid in thread 0.
+__device__ __forceinline__ unsigned long long take_slots(unsigned long long* part, uint32_t w) {
+    unsigned long long v = 0;
+    if (threadIdx.x < kSlots) {
+        unsigned long long* p = &part[threadIdx.x * kSlotStride + w];
+        v = *p;
+        *p = 0ull;
+    }
+    if (threadIdx.x < 64) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            v += __shfl_xor(v, o, 64);
+        }
+    }
+    return v;
+}
+
+// Folds the pending advance statistics (insertions, arrivals) into the
+// counters; launched with one wave.
+__global__ __launch_bounds__(64) void k_fold_stats(unsigned long long* __restrict__ part_stats,
+                                                   Counters* __restrict__ ctr) {
+    const unsigned long long ins = take_slots(part_stats, kWordInserts);
+    const unsigned long long arr = take_slots(part_stats, kWordArrivals);
+    if (threadIdx.x == 0) {
+        ctr->inserts += ins;
+        ctr->arrivals += arr;
+    }
+}
+
+// Folds k_collide's slots into the dispatch's pair total (every pair is found
+// from both ends in the fast path, once in the SLOW path) and, when
+// fold_stats is set, the pending advance statistics; launched with one wave.
+template <bool SLOW>
+__global__ __launch_bounds__(64) void k_collide_finish(unsigned long long* __restrict__ part_pairs,
+                                                       Counters* __restrict__ ctr, uint32_t* __restrict__ hist,
+                                                       uint32_t hist_mask,
+                                                       unsigned long long* __restrict__ part_stats,
+                                                       uint32_t fold_stats) {
+    const unsigned long long tot = take_slots(part_pairs, kWordPairs);
+    unsigned long long ins = 0, arr = 0;
+    if (fold_stats) {
+        ins = take_slots(part_stats, kWordInserts);
+        arr = take_slots(part_stats, kWordArrivals);
+    }
+    if (threadIdx.x == 0) {
+        const unsigned long long pairs = SLOW ? tot : tot / 2;
+        ctr->pairs_total += pairs;
+        hist[ctr->collides & hist_mask] = static_cast<uint32_t>(pairs);
+        ctr->collides += 1;
+        ctr->inserts += ins;
+        ctr->arrivals += arr;
     }
 }
 

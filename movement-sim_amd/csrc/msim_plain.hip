@@ -61,7 +61,9 @@ struct msim_sim {
     uint32_t* bbid[2] = {nullptr, nullptr};
     int bcur = 0;
     bool stats_pending = false;
-    int binning = 0;  // 0: segment-owned (k_advance + k_bin), 1: atomic counting
+    int binning = 0;  // 0: segment-owned (k_bin_count + k_bin_place), 1: atomic counting
+    bool route_inline = false;  // arrivals routed inside k_advance instead of k_route
+    bool collide_uniform = false;  // wave-uniform neighbour path for waves in one bucket range
     uint32_t seg_log = 0, n_seg = 0, margin = 0;
     uint32_t* seg_total = nullptr;  // [n_seg] per-segment agent totals
     // Arrival queues of the fast move dispatch; counters are double-buffered by
@@ -72,8 +74,8 @@ struct msim_sim {
     uint32_t q_blocks = 1;  // k_route workgroups per queue
     int qpar = 0;
     Counters* ctr = nullptr;
-    uint32_t* blk_stats = nullptr;  // [2 * agent blocks] advance statistics
-    uint32_t* blk_pairs = nullptr;  // [agent blocks] per-block neighbour sums
+    unsigned long long* part_stats = nullptr;  // [kSlots * kSlotStride] advance statistics
+    unsigned long long* part_pairs = nullptr;  // [kSlots * kSlotStride] neighbour sums
     uint32_t* hist = nullptr;
     uint32_t hist_cap = 0;
     msim_push_consts pc{};
@@ -106,11 +108,22 @@ struct msim_sim {
 };
 
 namespace {
-const char* const kFastKernels[] = {"k_advance",   "k_route",   "k_bin_count",
-                                    "k_bin_place", "k_collide", "k_collide_finish"};
-const char* const kAtomicKernels[] = {"k_advance",   "k_route",   "k_scan_reduce", "k_scan_partials",
-                                      "k_scan_down", "k_scatter", "k_collide",     "k_collide_finish"};
-uint32_t kernels_per_tick(const msim_sim* s) { return s->binning ? 8u : 6u; }
+const char* const kSegKernels[] = {"k_bin_count", "k_bin_place"};
+const char* const kAtomicKernels[] = {"k_scan_reduce", "k_scan_partials", "k_scan_down", "k_scatter"};
+uint32_t kernels_per_tick(const msim_sim* s) { return (s->route_inline ? 1u : 2u) + (s->binning ? 4u : 2u) + 2u; }
+// Name of launch k of a fused tick.
+const char* tick_kernel_name(const msim_sim* s, uint32_t k) {
+    if (k == 0) return "k_advance";
+    if (!s->route_inline) {
+        if (k == 1) return "k_route";
+        k--;
+    }
+    k--;
+    const uint32_t nbin = s->binning ? 4u : 2u;
+    if (k < nbin) return s->binning ? kAtomicKernels[k] : kSegKernels[k];
+    k -= nbin;
+    return k == 0 ? "k_collide" : "k_collide_finish";
+}
 }  // namespace
 
 namespace {
@@ -176,18 +189,18 @@ void launch_advance_mode(msim_sim* s, bool write_dir) {
     if (s->uniform) {
         if (write_dir) {
             hipLaunchKernelGGL((k_advance_bin<MODE, true, true>), g, dim3(kBlock), 0, s->stream, a,
-                               s->grid_cur(), s->roads, s->conns, s->blk_stats);
+                               s->grid_cur(), s->roads, s->conns, s->part_stats);
         } else {
             hipLaunchKernelGGL((k_advance_bin<MODE, true, false>), g, dim3(kBlock), 0, s->stream, a,
-                               s->grid_cur(), s->roads, s->conns, s->blk_stats);
+                               s->grid_cur(), s->roads, s->conns, s->part_stats);
         }
     } else {
         if (write_dir) {
             hipLaunchKernelGGL((k_advance_bin<MODE, false, true>), g, dim3(kBlock), 0, s->stream, a,
-                               s->grid_cur(), s->roads, s->conns, s->blk_stats);
+                               s->grid_cur(), s->roads, s->conns, s->part_stats);
         } else {
             hipLaunchKernelGGL((k_advance_bin<MODE, false, false>), g, dim3(kBlock), 0, s->stream, a,
-                               s->grid_cur(), s->roads, s->conns, s->blk_stats);
+                               s->grid_cur(), s->roads, s->conns, s->part_stats);
         }
     }
 }
@@ -196,15 +209,26 @@ void launch_advance_mode(msim_sim* s, bool write_dir) {
 // the bucket table rebuild (segment-owned into the other binning buffer, or
 // atomic counting + scan + scatter into the current one).  ev (optional)
 // receives an event after each launch.
-template <bool U, bool W>
-void launch_advance_route(msim_sim* s, const QueueDev& q) {
+template <bool U, bool W, bool RI>
+void launch_advance_ri(msim_sim* s, const QueueDev& q) {
     const AgentsDev a = s->agents(s->cur);
     const GridDev g = s->grid_cur();
     const dim3 nb(agent_blocks(s->n));
     if (s->binning) {
-        hipLaunchKernelGGL((k_advance<U, W, true>), nb, dim3(kBlock), 0, s->stream, a, g, q, s->blk_stats);
+        hipLaunchKernelGGL((k_advance<U, W, true, RI>), nb, dim3(kBlock), 0, s->stream, a, g, q, s->part_stats,
+                           s->roads, s->conns);
     } else {
-        hipLaunchKernelGGL((k_advance<U, W, false>), nb, dim3(kBlock), 0, s->stream, a, g, q, s->blk_stats);
+        hipLaunchKernelGGL((k_advance<U, W, false, RI>), nb, dim3(kBlock), 0, s->stream, a, g, q,
+                           s->part_stats, s->roads, s->conns);
+    }
+}
+
+template <bool U, bool W>
+void launch_advance_route(msim_sim* s, const QueueDev& q) {
+    if (s->route_inline) {
+        launch_advance_ri<U, W, true>(s, q);
+    } else {
+        launch_advance_ri<U, W, false>(s, q);
     }
 }
 
@@ -225,14 +249,16 @@ int enqueue_move_fast(msim_sim* s, bool write_dir, hipEvent_t* ev) {
         else launch_advance_route<false, false>(s, q);
     }
     if (ev) (void)hipEventRecord(ev[0], s->stream);
-    if (s->uniform) {
-        if (write_dir) launch_route<true, true>(s, q);
-        else launch_route<true, false>(s, q);
-    } else {
-        if (write_dir) launch_route<false, true>(s, q);
-        else launch_route<false, false>(s, q);
+    if (!s->route_inline) {
+        if (s->uniform) {
+            if (write_dir) launch_route<true, true>(s, q);
+            else launch_route<true, false>(s, q);
+        } else {
+            if (write_dir) launch_route<false, true>(s, q);
+            else launch_route<false, false>(s, q);
+        }
+        ev = ev ? ev + 1 : nullptr;
     }
-    if (ev) (void)hipEventRecord(ev[1], s->stream);
     s->qpar = 1 - s->qpar;
     const float2* pos = s->agents(s->cur).pos;
     if (s->binning == 0) {
@@ -256,23 +282,23 @@ int enqueue_move_fast(msim_sim* s, bool write_dir, hipEvent_t* ev) {
         b.n = s->n;
         const size_t lds = sizeof(uint32_t) << s->seg_log;
         hipLaunchKernelGGL(k_bin_count, dim3(s->n_seg), dim3(kBinThreads), lds, s->stream, pos, b);
-        if (ev) (void)hipEventRecord(ev[2], s->stream);
+        if (ev) (void)hipEventRecord(ev[1], s->stream);
         hipLaunchKernelGGL(k_bin_place, dim3(s->n_seg), dim3(kBinThreads), lds, s->stream, pos, b);
-        if (ev) (void)hipEventRecord(ev[3], s->stream);
+        if (ev) (void)hipEventRecord(ev[2], s->stream);
         s->bcur = nb;
     } else {
         const GridDev g = s->grid_cur();
         const uint32_t n_part = g.m / kScanTile;
         hipLaunchKernelGGL(k_scan_reduce, dim3(n_part), dim3(kBlock), 0, s->stream, g.cnt, g.part);
-        if (ev) (void)hipEventRecord(ev[2], s->stream);
+        if (ev) (void)hipEventRecord(ev[1], s->stream);
         hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kPartialsThreads), 0, s->stream, g.part, n_part,
-                           g.start + g.m, s->blk_stats, 0u, s->ctr);
-        if (ev) (void)hipEventRecord(ev[3], s->stream);
+                           g.start + g.m);
+        if (ev) (void)hipEventRecord(ev[2], s->stream);
         hipLaunchKernelGGL(k_scan_down, dim3(n_part), dim3(kBlock), 0, s->stream, g.cnt, g.part, g.start);
-        if (ev) (void)hipEventRecord(ev[4], s->stream);
+        if (ev) (void)hipEventRecord(ev[3], s->stream);
         hipLaunchKernelGGL(k_scatter, dim3(agent_blocks(s->n)), dim3(kBlock), 0, s->stream, s->agents(s->cur),
                            g);
-        if (ev) (void)hipEventRecord(ev[5], s->stream);
+        if (ev) (void)hipEventRecord(ev[4], s->stream);
     }
     s->stats_pending = true;  // folded by the next collision dispatch's finish pass
     return check_launch(s, "move dispatch");
@@ -281,8 +307,7 @@ int enqueue_move_fast(msim_sim* s, bool write_dir, hipEvent_t* ev) {
 // Folds advance statistics that no collision dispatch has folded yet.
 void flush_stats(msim_sim* s) {
     if (s->stats_pending) {
-        hipLaunchKernelGGL(k_fold_stats, dim3(1), dim3(kPartialsThreads), 0, s->stream, s->blk_stats,
-                           agent_blocks(s->n), s->ctr);
+        hipLaunchKernelGGL(k_fold_stats, dim3(1), dim3(64), 0, s->stream, s->part_stats, s->ctr);
         s->stats_pending = false;
     }
 }
@@ -310,9 +335,9 @@ int enqueue_move(msim_sim* s, int mode, bool write_dir, hipEvent_t* ev = nullptr
     const uint32_t n_part = g.m / kScanTile;
     hipLaunchKernelGGL(k_scan_reduce, dim3(n_part), dim3(kBlock), 0, s->stream, g.cnt, g.part);
     if (ev) (void)hipEventRecord(ev[1], s->stream);
-    const uint32_t n_stat = (mode == kBinOnly) ? 0u : agent_blocks(s->n);
     hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kPartialsThreads), 0, s->stream, g.part, n_part,
-                       g.start + g.m, s->blk_stats, n_stat, s->ctr);
+                       g.start + g.m);
+    s->stats_pending = (mode != kBinOnly);
     if (ev) (void)hipEventRecord(ev[2], s->stream);
     hipLaunchKernelGGL(k_scan_down, dim3(n_part), dim3(kBlock), 0, s->stream, g.cnt, g.part, g.start);
     if (ev) (void)hipEventRecord(ev[3], s->stream);
@@ -325,19 +350,26 @@ int enqueue_move(msim_sim* s, int mode, bool write_dir, hipEvent_t* ev = nullptr
 int enqueue_collide(msim_sim* s, bool slow, hipEvent_t* ev = nullptr) {
     const AgentsDev a = s->agents(s->cur);
     const dim3 g(agent_blocks(s->n));
-    const uint32_t n_stat = s->stats_pending ? g.x : 0u;
+    const uint32_t fold = s->stats_pending ? 1u : 0u;
     s->stats_pending = false;
     if (slow) {
-        hipLaunchKernelGGL(k_collide<true>, g, dim3(kBlock), 0, s->stream, a, s->grid_cur(), s->blk_pairs);
+        hipLaunchKernelGGL((k_collide<true, false>), g, dim3(kBlock), 0, s->stream, a, s->grid_cur(),
+                           static_cast<const float2*>(s->bbpos[s->bcur]), s->part_pairs);
         if (ev) (void)hipEventRecord(ev[0], s->stream);
-        hipLaunchKernelGGL(k_collide_finish<true>, dim3(1), dim3(kPartialsThreads), 0, s->stream, s->blk_pairs,
-                           g.x, s->ctr, s->hist, s->hist_cap - 1, s->blk_stats, n_stat);
+        hipLaunchKernelGGL(k_collide_finish<true>, dim3(1), dim3(64), 0, s->stream, s->part_pairs, s->ctr,
+                           s->hist, s->hist_cap - 1, s->part_stats, fold);
         hipLaunchKernelGGL(k_set_init, g, dim3(kBlock), 0, s->stream, a.init, s->n);
     } else {
-        hipLaunchKernelGGL(k_collide<false>, g, dim3(kBlock), 0, s->stream, a, s->grid_cur(), s->blk_pairs);
+        if (s->collide_uniform) {
+            hipLaunchKernelGGL((k_collide<false, true>), g, dim3(kBlock), 0, s->stream, a, s->grid_cur(),
+                               static_cast<const float2*>(s->bbpos[s->bcur]), s->part_pairs);
+        } else {
+            hipLaunchKernelGGL((k_collide<false, false>), g, dim3(kBlock), 0, s->stream, a, s->grid_cur(),
+                               static_cast<const float2*>(s->bbpos[s->bcur]), s->part_pairs);
+        }
         if (ev) (void)hipEventRecord(ev[0], s->stream);
-        hipLaunchKernelGGL(k_collide_finish<false>, dim3(1), dim3(kPartialsThreads), 0, s->stream,
-                           s->blk_pairs, g.x, s->ctr, s->hist, s->hist_cap - 1, s->blk_stats, n_stat);
+        hipLaunchKernelGGL(k_collide_finish<false>, dim3(1), dim3(64), 0, s->stream, s->part_pairs, s->ctr,
+                           s->hist, s->hist_cap - 1, s->part_stats, fold);
     }
     if (ev) (void)hipEventRecord(ev[1], s->stream);
     return check_launch(s, "collision dispatch");
@@ -440,8 +472,13 @@ int set_radius(msim_sim* s, float r) {
                     (double)(1.0f / s->grid.inv_cell));
     }
     if (r != s->grid.radius) {
+        const float lim = d2_limit_for(r);
+        if (!box_tests_implied(r, lim)) {
+            return fail(s, MSIM_ERR_INVALID, "collisionRadius %a: box tests not implied by the squared limit",
+                        (double)r);
+        }
         s->grid.radius = r;
-        s->grid.d2_limit = d2_limit_for(r);
+        s->grid.d2_limit = lim;
         drop_graphs(s);
     }
     return MSIM_OK;
@@ -506,8 +543,12 @@ extern "C" int msim_create(const msim_road* roads, uint32_t n_roads, const uint3
         o.device = -1;
     }
     const float radius = pc->collisionRadius;
-    if (!(radius > 0.0f) || !std::isfinite(radius)) {
+    if (!(radius > 0.0f) || !std::isfinite(radius) || !(radius * radius < 3.0e38f)) {
         return fail(nullptr, MSIM_ERR_INVALID, "collisionRadius must be positive and finite");
+    }
+    if (!box_tests_implied(radius, d2_limit_for(radius))) {
+        return fail(nullptr, MSIM_ERR_INVALID, "collisionRadius %a: box tests not implied by the squared limit",
+                    (double)radius);
     }
     float cell = o.cell_size;
     if (cell == 0.0f) {
@@ -588,6 +629,8 @@ extern "C" int msim_create(const msim_road* roads, uint32_t n_roads, const uint3
     g.wmask = (1u << g.lw) - 1u;
     g.hmask = (1u << (tlog - g.lw)) - 1u;
     s->binning = static_cast<int>(o.binning);
+    s->route_inline = o.route_queue == 0;
+    s->collide_uniform = o.collide_wave_uniform != 0;
     s->seg_log = seg_use;
     s->n_seg = g.m >> seg_use;
     // An agent moves at most max_speed per axis per tick (the heading has
@@ -631,8 +674,8 @@ extern "C" int msim_create(const msim_road* roads, uint32_t n_roads, const uint3
     ALLOC(s->q_items, static_cast<size_t>(kQueues) * s->q_cap);
     ALLOC(s->q_count, 2 * kQueues * kQueueStride);
     ALLOC(s->ctr, 1);
-    ALLOC(s->blk_stats, 2 * static_cast<size_t>(agent_blocks(n)));
-    ALLOC(s->blk_pairs, agent_blocks(n));
+    ALLOC(s->part_stats, kSlots * kSlotStride);
+    ALLOC(s->part_pairs, kSlots * kSlotStride);
     ALLOC(s->hist, s->hist_cap);
 #undef ALLOC
     if (rc != MSIM_OK) {
@@ -680,6 +723,8 @@ extern "C" int msim_create(const msim_road* roads, uint32_t n_roads, const uint3
         hipError_t e = hipMemset(g.cnt, 0, sizeof(uint32_t) * g.m);
         if (e == hipSuccess) e = hipMemset(s->ctr, 0, sizeof(Counters));
         if (e == hipSuccess) e = hipMemset(s->q_count, 0, sizeof(uint32_t) * 2 * kQueues * kQueueStride);
+        if (e == hipSuccess) e = hipMemset(s->part_stats, 0, sizeof(unsigned long long) * kSlots * kSlotStride);
+        if (e == hipSuccess) e = hipMemset(s->part_pairs, 0, sizeof(unsigned long long) * kSlots * kSlotStride);
         // every binned id slot always holds a valid agent index
         for (int k = 0; k < 2 && e == hipSuccess; k++) e = hipMemset(s->bbid[k], 0, sizeof(uint32_t) * n);
         for (int k = 0; k < 2 && e == hipSuccess; k++) e = hipMemset(s->bstart[k], 0, sizeof(uint32_t) * (g.m + 1));
@@ -921,7 +966,7 @@ extern "C" int msim_get_info(msim_sim* s, msim_info* out) {
 
 extern "C" const char* msim_kernel_name(const msim_sim* s, uint32_t k) {
     if (!s || k >= kernels_per_tick(s)) return nullptr;
-    return s->binning ? kAtomicKernels[k] : kFastKernels[k];
+    return tick_kernel_name(s, k);
 }
 
 extern "C" int msim_time_ticks(msim_sim* s, uint32_t n_ticks, float* ms_total, float* kernel_ms,

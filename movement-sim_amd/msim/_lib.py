@@ -38,6 +38,8 @@ class Options(ctypes.Structure):
         ("table_width_log2", ctypes.c_uint32),
         ("segment_log2", ctypes.c_uint32),
         ("binning", ctypes.c_uint32),
+        ("route_queue", ctypes.c_uint32),
+        ("collide_wave_uniform", ctypes.c_uint32),
     ]
 
 

@@ -1,0 +1,123 @@
+"""Autotuner for the plain pipeline's launch and layout parameters.
+
+Coordinate descent over the msim_options knobs that change speed but never
+results (every value is covered by the parity tests): bucket-table row width,
+rebinning segment size, binning mode, inline vs queued route choice, spatial
+reorder interval and the neighbour walk.  Each candidate is timed on the real
+workload with HIP events around graph-replayed ticks; the winner is cached in
+a JSON file keyed by the workload's shape.
+
+    python -m msim.autotune --config 2          (from movement-sim_amd/)
+"""
+import argparse
+import json
+import os
+import time
+
+import numpy as np
+
+from ._lib import Simulation, MsimError
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CACHE = os.path.join(HERE, "tuned.json")
+
+# Knob -> candidate values (0 = the library's automatic choice where it has one).
+SPACE = {
+    "table_width_log2": [0, 5, 6, 7, 8, 9, 10, 11, 12],
+    "segment_log2": [0, 9, 10, 11, 12, 13, 14],
+    "binning": [0, 1],
+    "route_queue": [0, 1],
+    "reorder_interval": [8, 16, 32, 64],
+    "collide_wave_uniform": [0, 1],
+}
+DEFAULT = {"table_width_log2": 0, "segment_log2": 0, "binning": 0, "route_queue": 0,
+           "reorder_interval": 16, "collide_wave_uniform": 0}
+
+
+def workload_key(sc):
+    n = int(sc.entities.shape[0])
+    w, h = (int(round(v)) for v in sc.world)
+    uni = "u" if sc.speed is None else "s"
+    return f"n{n}-r{int(sc.roads.shape[0])}-w{w}x{h}-R{float(sc.pc['collisionRadius']):g}-{uni}"
+
+
+def time_options(sc, opts, warmup, ticks):
+    """Microseconds per tick through the graph path (HIP events)."""
+    try:
+        sim = Simulation(sc.roads, sc.conns, sc.entities, sc.pc, speed=sc.speed, **opts)
+    except MsimError:
+        return float("inf")
+    try:
+        sim.run_ticks(warmup)
+        sim.sync()
+        ms, _ = sim.time_ticks(ticks, kernels=False)
+        sim.sync()
+        return 1e3 * ms / ticks
+    except MsimError:
+        return float("inf")
+    finally:
+        sim.close()
+
+
+def tune(sc, warmup=20, ticks=100, space=None, log=print):
+    space = SPACE if space is None else space
+    best = dict(DEFAULT)
+    best_us = time_options(sc, best, warmup, ticks)
+    log(f"  start {best} -> {best_us:.1f} us/tick")
+    trials = [(dict(best), best_us)]
+    for knob, values in space.items():
+        for v in values:
+            if v == best[knob]:
+                continue
+            cand = dict(best)
+            cand[knob] = v
+            us = time_options(sc, cand, warmup, ticks)
+            trials.append((cand, us))
+            log(f"  {knob}={v}: {us:.1f} us/tick" + (" (new best)" if us < best_us else ""))
+            if us < best_us:
+                best, best_us = cand, us
+    return best, best_us, trials
+
+
+def load_cache(path=CACHE):
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return {}
+
+
+def tuned_options(sc, path=CACHE):
+    """Cached best options for this workload shape, or the defaults."""
+    ent = load_cache(path).get(workload_key(sc))
+    return dict(ent["options"]) if ent else dict(DEFAULT)
+
+
+def main():
+    import sys
+    sys.path.insert(0, os.path.dirname(HERE))
+    from msim import scenarios as S
+
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--config", type=int, nargs="+", default=[2])
+    ap.add_argument("--ticks", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--out", default=CACHE)
+    args = ap.parse_args()
+    cache = load_cache(args.out)
+    for k in args.config:
+        sc = S.config(k)
+        key = workload_key(sc)
+        print(f"config {k} ({key})", flush=True)
+        t0 = time.time()
+        best, us, trials = tune(sc, args.warmup, args.ticks, log=lambda m: print(m, flush=True))
+        n = sc.entities.shape[0]
+        cache[key] = {"config": k, "options": best, "us_per_tick": us, "agent_ticks_per_s": n / us * 1e6,
+                      "default_us_per_tick": trials[0][1], "trials": len(trials),
+                      "tuned_s": round(time.time() - t0, 1)}
+        print(json.dumps(cache[key]), flush=True)
+        with open(args.out, "w") as f:
+            json.dump(cache, f, indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    main()

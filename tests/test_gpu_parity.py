@@ -94,7 +94,9 @@ def test_dispatch_loop_equals_fused_graph_path():
                                   dict(graph_ticks=1), dict(table_log2=12), dict(table_log2=20),
                                   dict(binning=1), dict(binning=1, reorder_interval=5),
                                   dict(segment_log2=9, table_width_log2=3),
-                                  dict(table_log2=16, segment_log2=13, table_width_log2=8)])
+                                  dict(table_log2=16, segment_log2=13, table_width_log2=8),
+                                  dict(route_queue=1), dict(collide_wave_uniform=1),
+                                  dict(route_queue=1, binning=1, collide_wave_uniform=1)])
 def test_options_do_not_change_results(opts):
     sc = S.config(1, scale=0.5)
     g, o = pair(sc, **opts)
@@ -252,3 +254,32 @@ def test_config3_full_size_parity():
     assert_same(g.state(), o.st, "config 3 full")
     total, hist = g.pairs()
     assert total == o.pairs_total and hist.tolist() == o.pair_hist
+
+
+@pytest.mark.parametrize("radius", [1.0, 0.75, 10.0, 2.5])
+def test_boundary_pairs_match_brute_force(radius):
+    """Crafted pairs at exactly R, one ulp inside, stacked duplicates and
+    cell-edge positions, through one collision dispatch, against the oracle's
+    O(N^2) pass."""
+    roads, conns, w, h = S.star_map()
+    rng = np.random.default_rng(int(radius * 10))
+    n = 6000
+    r32 = np.float32(radius)
+    p = rng.uniform(2, 98, (n, 2)).astype(np.float32)
+    p[:500, 0] = np.floor(p[:500, 0])  # on cell edges
+    p[500:1000] = p[1000:1500]  # duplicates
+    p[1500:2000, 0] = p[2000:2500, 0] + r32  # exactly R apart
+    p[1500:2000, 1] = p[2000:2500, 1]
+    p[2500:3000, 0] = np.nextafter(p[3000:3500, 0] + r32, np.float32(-1))  # just inside
+    p[2500:3000, 1] = p[3000:3500, 1]
+    ents = S.reference_agents(roads, n, 3)
+    ents["pos"] = p
+    ents["initialized"] = 1
+    pc = msim.push_consts(w, h, radius, tick=2)
+    g = msim.Simulation(roads, conns, ents, pc)
+    g.dispatch(3)
+    st = g.state()
+    c = np.zeros(n, np.uint32)
+    pairs = orc.L.orc_collide_brute(n, orc._p(np.ascontiguousarray(p)), radius, None, orc._p(c))
+    np.testing.assert_array_equal(st["ncount"], c)
+    assert g.pairs()[0] == pairs
