@@ -56,7 +56,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--config", type=int, default=2, choices=(1, 2, 3))
-    ap.add_argument("--profile-ticks", type=int, default=200)
+    ap.add_argument("--profile-ticks", type=int, default=240)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--untuned", action="store_true", help="library defaults instead of msim/tuned.json")
@@ -92,7 +92,8 @@ def main():
     value = n * args.steps / el
     print(f"# timed {args.steps} ticks in {el:.3f} s", file=sys.stderr, flush=True)
 
-    # per-kernel durations (HIP events around every launch, on the handle's stream)
+    # per-kernel durations: HIP events right before and after one launch per
+    # tick, on the handle's stream, each kernel probed in turn
     _, kms = sim.time_ticks(args.profile_ticks, kernels=True)
     per_launch_ms = {k: v / args.profile_ticks for k, v in kms.items()}
     dom = max(per_launch_ms, key=per_launch_ms.get)

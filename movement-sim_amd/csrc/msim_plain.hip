@@ -64,6 +64,11 @@ struct msim_sim {
     int binning = 0;  // 0: segment-owned (k_bin_count + k_bin_place), 1: atomic counting
     bool route_inline = false;  // arrivals routed inside k_advance instead of k_route
     bool collide_uniform = false;  // wave-uniform neighbour path for waves in one bucket range
+    // msim_time_ticks probe: launch number probe_kernel of the tick is
+    // bracketed by probe_ev[0] / probe_ev[1]; -1 = no probe.
+    int probe_kernel = -1;
+    int launch_no = 0;
+    hipEvent_t probe_ev[2] = {nullptr, nullptr};
     uint32_t seg_log = 0, n_seg = 0, margin = 0;
     uint32_t* seg_total = nullptr;  // [n_seg] per-segment agent totals
     // Arrival queues of the fast move dispatch; counters are double-buffered by
@@ -205,6 +210,18 @@ void launch_advance_mode(msim_sim* s, bool write_dir) {
     }
 }
 
+// msim_time_ticks brackets one launch of a tick with events: probe_pre /
+// probe_post surround every launch of the fast tick and fire for launch
+// number probe_kernel only.
+void probe_pre(msim_sim* s) {
+    if (s->probe_kernel == s->launch_no) (void)hipEventRecord(s->probe_ev[0], s->stream);
+}
+
+void probe_post(msim_sim* s) {
+    if (s->probe_kernel == s->launch_no) (void)hipEventRecord(s->probe_ev[1], s->stream);
+    s->launch_no++;
+}
+
 // Fast move dispatch: streaming advance, route choice for the arrivals, then
 // the bucket table rebuild (segment-owned into the other binning buffer, or
 // atomic counting + scan + scatter into the current one).  ev (optional)
@@ -241,6 +258,8 @@ void launch_route(msim_sim* s, const QueueDev& q) {
 int enqueue_move_fast(msim_sim* s, bool write_dir, hipEvent_t* ev) {
     const uint32_t qw = kQueues * kQueueStride;
     QueueDev q{s->q_items, s->q_count + s->qpar * qw, s->q_count + (1 - s->qpar) * qw, s->q_cap};
+    s->launch_no = 0;
+    probe_pre(s);
     if (s->uniform) {
         if (write_dir) launch_advance_route<true, true>(s, q);
         else launch_advance_route<true, false>(s, q);
@@ -248,8 +267,9 @@ int enqueue_move_fast(msim_sim* s, bool write_dir, hipEvent_t* ev) {
         if (write_dir) launch_advance_route<false, true>(s, q);
         else launch_advance_route<false, false>(s, q);
     }
-    if (ev) (void)hipEventRecord(ev[0], s->stream);
+    probe_post(s);
     if (!s->route_inline) {
+        probe_pre(s);
         if (s->uniform) {
             if (write_dir) launch_route<true, true>(s, q);
             else launch_route<true, false>(s, q);
@@ -257,7 +277,7 @@ int enqueue_move_fast(msim_sim* s, bool write_dir, hipEvent_t* ev) {
             if (write_dir) launch_route<false, true>(s, q);
             else launch_route<false, false>(s, q);
         }
-        ev = ev ? ev + 1 : nullptr;
+        probe_post(s);
     }
     s->qpar = 1 - s->qpar;
     const float2* pos = s->agents(s->cur).pos;
@@ -281,24 +301,30 @@ int enqueue_move_fast(msim_sim* s, bool write_dir, hipEvent_t* ev) {
         b.margin = s->margin;
         b.n = s->n;
         const size_t lds = sizeof(uint32_t) << s->seg_log;
+        probe_pre(s);
         hipLaunchKernelGGL(k_bin_count, dim3(s->n_seg), dim3(kBinThreads), lds, s->stream, pos, b);
-        if (ev) (void)hipEventRecord(ev[1], s->stream);
+        probe_post(s);
+        probe_pre(s);
         hipLaunchKernelGGL(k_bin_place, dim3(s->n_seg), dim3(kBinThreads), lds, s->stream, pos, b);
-        if (ev) (void)hipEventRecord(ev[2], s->stream);
+        probe_post(s);
         s->bcur = nb;
     } else {
         const GridDev g = s->grid_cur();
         const uint32_t n_part = g.m / kScanTile;
+        probe_pre(s);
         hipLaunchKernelGGL(k_scan_reduce, dim3(n_part), dim3(kBlock), 0, s->stream, g.cnt, g.part);
-        if (ev) (void)hipEventRecord(ev[1], s->stream);
+        probe_post(s);
+        probe_pre(s);
         hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kPartialsThreads), 0, s->stream, g.part, n_part,
                            g.start + g.m);
-        if (ev) (void)hipEventRecord(ev[2], s->stream);
+        probe_post(s);
+        probe_pre(s);
         hipLaunchKernelGGL(k_scan_down, dim3(n_part), dim3(kBlock), 0, s->stream, g.cnt, g.part, g.start);
-        if (ev) (void)hipEventRecord(ev[3], s->stream);
+        probe_post(s);
+        probe_pre(s);
         hipLaunchKernelGGL(k_scatter, dim3(agent_blocks(s->n)), dim3(kBlock), 0, s->stream, s->agents(s->cur),
                            g);
-        if (ev) (void)hipEventRecord(ev[4], s->stream);
+        probe_post(s);
     }
     s->stats_pending = true;  // folded by the next collision dispatch's finish pass
     return check_launch(s, "move dispatch");
@@ -353,25 +379,30 @@ int enqueue_collide(msim_sim* s, bool slow, hipEvent_t* ev = nullptr) {
     const uint32_t fold = s->stats_pending ? 1u : 0u;
     s->stats_pending = false;
     if (slow) {
+        probe_pre(s);
         hipLaunchKernelGGL((k_collide<true, false>), g, dim3(kBlock), 0, s->stream, a, s->grid_cur(),
                            static_cast<const float2*>(s->bbpos[s->bcur]), s->part_pairs);
-        if (ev) (void)hipEventRecord(ev[0], s->stream);
+        probe_post(s);
+        probe_pre(s);
         hipLaunchKernelGGL(k_collide_finish<true>, dim3(1), dim3(64), 0, s->stream, s->part_pairs, s->ctr,
                            s->hist, s->hist_cap - 1, s->part_stats, fold);
         hipLaunchKernelGGL(k_set_init, g, dim3(kBlock), 0, s->stream, a.init, s->n);
     } else {
         if (s->collide_uniform) {
+            probe_pre(s);
             hipLaunchKernelGGL((k_collide<false, true>), g, dim3(kBlock), 0, s->stream, a, s->grid_cur(),
                                static_cast<const float2*>(s->bbpos[s->bcur]), s->part_pairs);
         } else {
+            probe_pre(s);
             hipLaunchKernelGGL((k_collide<false, false>), g, dim3(kBlock), 0, s->stream, a, s->grid_cur(),
                                static_cast<const float2*>(s->bbpos[s->bcur]), s->part_pairs);
         }
-        if (ev) (void)hipEventRecord(ev[0], s->stream);
+        probe_post(s);
+        probe_pre(s);
         hipLaunchKernelGGL(k_collide_finish<false>, dim3(1), dim3(64), 0, s->stream, s->part_pairs, s->ctr,
                            s->hist, s->hist_cap - 1, s->part_stats, fold);
     }
-    if (ev) (void)hipEventRecord(ev[1], s->stream);
+    probe_post(s);
     return check_launch(s, "collision dispatch");
 }
 
@@ -987,25 +1018,31 @@ extern "C" int msim_time_ticks(msim_sim* s, uint32_t n_ticks, float* ms_total, f
         if (rc == MSIM_OK) (void)hipEventElapsedTime(ms_total, ev[0], ev[1]);
     }
     if (rc == MSIM_OK && kernel_ms && n_kernel_ms) {
+        // Kernel k of a tick is timed alone: an event right before and right
+        // after its launch, the tick's other kernels running untimed.
         double acc[kMaxK] = {0, 0, 0, 0, 0, 0, 0, 0};
         for (uint32_t t = 0; t < n_ticks && rc == MSIM_OK; t++) {
-            (void)hipEventRecord(ev[0], s->stream);
-            rc = enqueue_move(s, kMoveAll, t + 1 == n_ticks, ev + 1);
-            if (rc == MSIM_OK) rc = enqueue_collide(s, false, ev + (nk - 1));
+            const int k = static_cast<int>(t % static_cast<uint32_t>(nk));
+            s->probe_kernel = k;
+            s->probe_ev[0] = ev[0];
+            s->probe_ev[1] = ev[1];
+            rc = enqueue_move(s, kMoveAll, t + 1 == n_ticks);
+            if (rc == MSIM_OK) rc = enqueue_collide(s, false);
+            s->probe_kernel = -1;
             s->pc.tick += 2;
             s->any_collide = true;
             if (rc == MSIM_OK) rc = after_move(s);
             if (rc == MSIM_OK) rc = wait_stream(s);
-            for (int k = 0; k < nk && rc == MSIM_OK; k++) {
+            if (rc == MSIM_OK) {
                 float ms = 0.f;
-                (void)hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
-                acc[k] += ms;
+                if (hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) acc[k] += ms * nk;
             }
         }
         for (uint32_t k = 0; k < n_kernel_ms && k < static_cast<uint32_t>(nk); k++) {
             kernel_ms[k] = static_cast<float>(acc[k]);
         }
     }
+    (void)hipGetLastError();
     for (auto& e : ev) (void)hipEventDestroy(e);
     return rc;
 }
