@@ -44,7 +44,8 @@ typedef struct msim_options {
     uint32_t table_log2;       /* 0: auto (about one bucket per agent)         */
     uint32_t block_size;       /* 0: 256 threads (4 wave64s)                   */
     uint32_t reorder_interval; /* 0: never; k: spatially renumber every k ticks*/
-    uint32_t graph_ticks;      /* 0: 32 ticks per captured hipGraph            */
+    uint32_t graph_ticks;      /* ticks per captured hipGraph segment, rounded
+                                  to whole reorder periods (0: 32; batched: 100)*/
     uint32_t history_capacity; /* 0: 4096 entries of per-tick pair counts      */
     int32_t device;            /* -1 (or 0 with struct_size 0): current device */
     uint32_t watchdog_ms;      /* 0: 600000 ms host-side wait limit            */
@@ -167,6 +168,50 @@ const char* msim_kernel_name(const msim_sim* sim, uint32_t k);
 int msim_get_stream(msim_sim* sim, void** stream);
 
 int msim_destroy(msim_sim* sim);
+
+/* ---- batched form ---------------------------------------------------------
+ * Many independent scenarios (own map, own agents, same PushConsts) advanced
+ * together: every kernel launch of a tick covers all scenarios, and ticks run
+ * as captured hipGraph segments.  No reference counterpart (the reference runs
+ * one map); each scenario's state is bit-identical to a plain msim_sim run of
+ * it.  Arrays are packed scenario after scenario; road_off / conn_off /
+ * agent_off (n_scen + 1 entries each) delimit them.  Road indices inside
+ * roads, connections and entities are local to their scenario, and each
+ * scenario's connection block must satisfy msim_create's rules on its own
+ * (including the guard entry).  All agents must share one Entity.initialized
+ * value; pc->tick must be odd (the next dispatch is a move), and only
+ * pc->collisionRadius and pc->tick are used.  speed may be NULL (1.4 m/tick for
+ * all).  Options: cell_size, reorder_interval (default 20), graph_ticks
+ * (default 100), device and watchdog_ms apply; the table layout is chosen per
+ * scenario.  msim_batch_last_error(NULL) reports a failed create. */
+typedef struct msim_batch msim_batch;
+
+int msim_batch_create(uint32_t n_scen, const uint32_t* road_off, const msim_road* roads,
+                      const uint32_t* conn_off, const uint32_t* connections,
+                      const uint32_t* agent_off, const msim_entity* entities, const float* speed,
+                      const msim_push_consts* pc, const msim_options* opt, msim_batch** out);
+
+/* n_ticks sim_ticks of every scenario (graph segments of graph_ticks).  With
+ * every agent uninitialised, the first tick is the reference's insert-only
+ * move dispatch followed by a collision dispatch. */
+int msim_batch_run_ticks(msim_batch* b, uint32_t n_ticks);
+
+int msim_batch_sync(msim_batch* b);
+
+/* One scenario's state in its local agent order (any pointer may be NULL);
+ * road indices are local to the scenario. */
+int msim_batch_read_state(msim_batch* b, uint32_t scen, float* pos, float* target, float* direction,
+                          uint32_t* road_index, uint32_t* rand_state, uint32_t* neighbour_count);
+
+/* Cumulative pair totals (n_scen entries) and the pairs of each scenario's
+ * last collision dispatch (n_scen entries); either may be NULL. */
+int msim_batch_read_pairs(msim_batch* b, uint64_t* totals, uint32_t* last);
+
+/* Elapsed milliseconds of n_ticks batched ticks (HIP events). */
+int msim_batch_time_ticks(msim_batch* b, uint32_t n_ticks, float* ms_total);
+
+int msim_batch_destroy(msim_batch* b);
+const char* msim_batch_last_error(const msim_batch* b);
 
 const char* msim_last_error(const msim_sim* sim);
 const char* msim_get_last_error(void);

@@ -38,7 +38,7 @@ struct QueueDev {
 // ROUTE_INLINE: arrivals take the route choice in this kernel (no queue, no
 // k_route launch); better when arrivals are cheap to gather (small maps).
 template <bool UNIFORM, bool WRITE_DIR, bool ATOMIC_BIN, bool ROUTE_INLINE>
-__global__ __launch_bounds__(kBlock) void k_advance(AgentsDev a, GridDev g, QueueDev q,
+static __global__ __launch_bounds__(kBlock) void k_advance(AgentsDev a, GridDev g, QueueDev q,
                                                     unsigned long long* __restrict__ part_stats,
                                                     const msim_road* __restrict__ roads,
                                                     const uint32_t* __restrict__ conns) {
@@ -130,7 +130,7 @@ __global__ __launch_bounds__(kBlock) void k_advance(AgentsDev a, GridDev g, Queu
 
 // Route choice for the queued arrivals; grid = kQueues * blocks_per_queue.
 template <bool UNIFORM, bool WRITE_DIR>
-__global__ __launch_bounds__(kBlock) void k_route(AgentsDev a, QueueDev q, const msim_road* __restrict__ roads,
+static __global__ __launch_bounds__(kBlock) void k_route(AgentsDev a, QueueDev q, const msim_road* __restrict__ roads,
                                                   const uint32_t* __restrict__ conns) {
     const uint32_t qi = blockIdx.x % kQueues;
     const uint32_t per_queue = gridDim.x / kQueues;
@@ -166,31 +166,44 @@ struct BinDev {
     uint32_t n;
 };
 
+// One bucket table region and the previous binning of its agents.
+struct TableView {
+    const uint32_t* old_start;  // [m+1] slot offsets (absolute)
+    const uint32_t* old_bid;    // slot -> internal agent
+    float inv_cell;
+    uint32_t lw, wmask, hmask, m, seg_log, margin;
+};
+
+__device__ __forceinline__ TableView table_view(const BinDev& b) {
+    return TableView{b.old_start, b.old_bid, b.inv_cell, b.lw, b.wmask, b.hmask, b.m, b.seg_log, b.margin};
+}
+
 // Old-slot ranges [a0, a1) and [c0, c1) holding every agent whose previous
 // bucket lies in the cyclic window [s0 - margin, s0 + S + margin).
-__device__ __forceinline__ void bin_window(const BinDev& b, uint32_t s0, uint32_t& a0, uint32_t& a1,
+__device__ __forceinline__ void bin_window(const TableView& t, uint32_t s0, uint32_t& a0, uint32_t& a1,
                                            uint32_t& c0, uint32_t& c1) {
-    const uint32_t S = 1u << b.seg_log;
+    const uint32_t S = 1u << t.seg_log;
     c0 = 0;
     c1 = 0;
-    if (2ull * b.margin + S >= b.m) {
-        a0 = 0;
-        a1 = b.old_start[b.m];
+    if (2ull * t.margin + S >= t.m) {
+        a0 = t.old_start[0];
+        a1 = t.old_start[t.m];
         return;
     }
-    const uint32_t lo = (s0 + b.m - b.margin) & (b.m - 1);
-    const uint32_t hi = (s0 + S + b.margin) & (b.m - 1);
-    a0 = b.old_start[lo];
+    const uint32_t lo = (s0 + t.m - t.margin) & (t.m - 1);
+    const uint32_t hi = (s0 + S + t.margin) & (t.m - 1);
+    a0 = t.old_start[lo];
     if (lo < hi) {
-        a1 = b.old_start[hi];
+        a1 = t.old_start[hi];
     } else {
-        a1 = b.old_start[b.m];
-        c1 = b.old_start[hi];
+        a1 = t.old_start[t.m];
+        c0 = t.old_start[0];
+        c1 = t.old_start[hi];
     }
 }
 
-__device__ __forceinline__ uint32_t bucket_of_pos(float2 p, const BinDev& b) {
-    return bucket_of(cell_coord(p.x, b.inv_cell), cell_coord(p.y, b.inv_cell), b.lw, b.wmask, b.hmask);
+__device__ __forceinline__ uint32_t bucket_of_pos(float2 p, const TableView& t) {
+    return bucket_of(cell_coord(p.x, t.inv_cell), cell_coord(p.y, t.inv_cell), t.lw, t.wmask, t.hmask);
 }
 
 constexpr uint32_t kBinUnroll = 4;
@@ -201,16 +214,17 @@ constexpr uint32_t kBinUnroll = 4;
 // helper always runs with the whole wave; kBinUnroll independent loads per
 // thread keep enough requests in flight.
 template <bool PLACE>
-__device__ __forceinline__ void bin_pass(const float2* __restrict__ pos, const BinDev& b, uint32_t s0,
-                                         uint32_t lo, uint32_t hi, uint32_t* table) {
-    const uint32_t S = 1u << b.seg_log;
+__device__ __forceinline__ void bin_pass(const float2* __restrict__ pos, const TableView& t, uint32_t s0,
+                                         uint32_t lo, uint32_t hi, uint32_t* table, float2* __restrict__ out_pos,
+                                         uint32_t* __restrict__ out_bid) {
+    const uint32_t S = 1u << t.seg_log;
     for (uint32_t base = lo; base < hi; base += kBinUnroll * kBinThreads) {
         uint32_t id[kBinUnroll];
         float2 p[kBinUnroll];
 #pragma unroll
         for (uint32_t u = 0; u < kBinUnroll; u++) {
             const uint32_t k = base + u * kBinThreads + threadIdx.x;
-            id[u] = k < hi ? b.old_bid[k] : 0u;
+            id[u] = k < hi ? t.old_bid[k] : 0u;
         }
 #pragma unroll
         for (uint32_t u = 0; u < kBinUnroll; u++) {
@@ -220,34 +234,32 @@ __device__ __forceinline__ void bin_pass(const float2* __restrict__ pos, const B
 #pragma unroll
         for (uint32_t u = 0; u < kBinUnroll; u++) {
             const uint32_t k = base + u * kBinThreads + threadIdx.x;
-            const uint32_t off = bucket_of_pos(p[u], b) - s0;
+            const uint32_t off = bucket_of_pos(p[u], t) - s0;
             const bool mine = (k < hi) && (off < S);
             const uint32_t slot = wave_rank_add(table, off, mine);
             if (PLACE && mine) {
-                b.new_bpos[slot] = p[u];
-                b.new_bid[slot] = id[u];
+                out_pos[slot] = p[u];
+                out_bid[slot] = id[u];
             }
         }
     }
 }
 
-// Counts the window agents that land in segment blockIdx.x, writes the
-// segment-local exclusive offsets to new_start and the total to seg_total.
-__global__ __launch_bounds__(kBinThreads) void k_bin_count(const float2* __restrict__ pos, BinDev b) {
-    extern __shared__ uint32_t hist[];
-    __shared__ uint32_t lds[kBinThreads / 64];
+// Segment count: window agents landing in [s0, s0 + S) counted in LDS, local
+// exclusive offsets written to start_out[s0 + j]; returns the segment total.
+__device__ __forceinline__ uint32_t bin_count_segment(const float2* __restrict__ pos, const TableView& t,
+                                                      uint32_t s0, uint32_t* hist, uint32_t* lds,
+                                                      uint32_t* __restrict__ start_out) {
     const uint32_t tid = threadIdx.x;
-    const uint32_t S = 1u << b.seg_log;
-    const uint32_t seg = blockIdx.x;
-    const uint32_t s0 = seg << b.seg_log;
+    const uint32_t S = 1u << t.seg_log;
     for (uint32_t j = tid; j < S; j += kBinThreads) {
         hist[j] = 0u;
     }
     __syncthreads();
     uint32_t a0, a1, c0, c1;
-    bin_window(b, s0, a0, a1, c0, c1);
-    bin_pass<false>(pos, b, s0, a0, a1, hist);
-    bin_pass<false>(pos, b, s0, c0, c1, hist);
+    bin_window(t, s0, a0, a1, c0, c1);
+    bin_pass<false>(pos, t, s0, a0, a1, hist, nullptr, nullptr);
+    bin_pass<false>(pos, t, s0, c0, c1, hist, nullptr, nullptr);
     __syncthreads();
     const uint32_t per = S / kBinThreads;
     uint32_t run = 0;
@@ -258,44 +270,59 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_count(const float2* __restr
     uint32_t excl = block_excl_scan<kBinThreads>(run, lds, total);
     for (uint32_t q = 0; q < per; q++) {
         const uint32_t v = hist[tid * per + q];
-        b.new_start[s0 + tid * per + q] = excl;
+        start_out[s0 + tid * per + q] = excl;
         excl += v;
     }
-    if (tid == 0) {
+    return total;
+}
+
+// Segment place: start_out[s0 + j] += base (absolute slots), agents placed.
+__device__ __forceinline__ void bin_place_segment(const float2* __restrict__ pos, const TableView& t, uint32_t s0,
+                                                  uint32_t base, uint32_t* cursor, uint32_t* __restrict__ start_out,
+                                                  float2* __restrict__ out_pos, uint32_t* __restrict__ out_bid) {
+    const uint32_t S = 1u << t.seg_log;
+    for (uint32_t j = threadIdx.x; j < S; j += kBinThreads) {
+        const uint32_t v = base + start_out[s0 + j];
+        cursor[j] = v;
+        start_out[s0 + j] = v;
+    }
+    __syncthreads();
+    uint32_t a0, a1, c0, c1;
+    bin_window(t, s0, a0, a1, c0, c1);
+    bin_pass<true>(pos, t, s0, a0, a1, cursor, out_pos, out_bid);
+    bin_pass<true>(pos, t, s0, c0, c1, cursor, out_pos, out_bid);
+}
+
+// Counts the window agents that land in segment blockIdx.x, writes the
+// segment-local exclusive offsets to new_start and the total to seg_total.
+static __global__ __launch_bounds__(kBinThreads) void k_bin_count(const float2* __restrict__ pos, BinDev b) {
+    extern __shared__ uint32_t hist[];
+    __shared__ uint32_t lds[kBinThreads / 64];
+    const uint32_t seg = blockIdx.x;
+    const uint32_t total = bin_count_segment(pos, table_view(b), seg << b.seg_log, hist, lds, b.new_start);
+    if (threadIdx.x == 0) {
         b.seg_total[seg] = total;
     }
 }
 
 // Adds the segment's global offset and places its agents.
-__global__ __launch_bounds__(kBinThreads) void k_bin_place(const float2* __restrict__ pos, BinDev b) {
+static __global__ __launch_bounds__(kBinThreads) void k_bin_place(const float2* __restrict__ pos, BinDev b) {
     extern __shared__ uint32_t cursor[];
     __shared__ uint32_t lds[kBinThreads / 64];
-    const uint32_t tid = threadIdx.x;
-    const uint32_t S = 1u << b.seg_log;
     const uint32_t seg = blockIdx.x;
-    const uint32_t s0 = seg << b.seg_log;
     uint32_t before = 0;
-    for (uint32_t k = tid; k < seg; k += kBinThreads) {
+    for (uint32_t k = threadIdx.x; k < seg; k += kBinThreads) {
         before += b.seg_total[k];
     }
     const uint32_t base = block_sum<kBinThreads>(before, lds);
-    for (uint32_t j = tid; j < S; j += kBinThreads) {
-        const uint32_t v = base + b.new_start[s0 + j];
-        cursor[j] = v;
-        b.new_start[s0 + j] = v;
-    }
-    if (seg == b.n_seg - 1 && tid == 0) {
+    if (seg == b.n_seg - 1 && threadIdx.x == 0) {
         const uint32_t total = base + b.seg_total[seg];
         b.new_start[b.m] = total;
         if (total != b.n) {
             atomicOr(&b.ctr->faults, kFaultBinCount);
         }
     }
-    __syncthreads();
-    uint32_t a0, a1, c0, c1;
-    bin_window(b, s0, a0, a1, c0, c1);
-    bin_pass<true>(pos, b, s0, a0, a1, cursor);
-    bin_pass<true>(pos, b, s0, c0, c1, cursor);
+    bin_place_segment(pos, table_view(b), seg << b.seg_log, base, cursor, b.new_start, b.new_bpos, b.new_bid);
 }
 
 }  // namespace msim

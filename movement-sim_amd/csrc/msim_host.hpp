@@ -85,6 +85,15 @@ inline uint32_t extent_rows_log2(const msim_road* roads, uint32_t n_roads, const
     return l;
 }
 
+// Ticks per captured segment: whole reorder periods (at least one), about
+// graph_ticks long; graph_ticks itself when reordering is off.
+inline uint32_t segment_ticks(uint32_t graph_ticks, uint32_t reorder_interval) {
+    if (reorder_interval == 0) return graph_ticks;
+    uint32_t k = (graph_ticks + reorder_interval / 2) / reorder_interval;
+    if (k == 0) k = 1;
+    return k * reorder_interval;
+}
+
 inline bool is_pow2_float(float v) {
     if (!(v > 0.0f) || !std::isfinite(v)) return false;
     int e = 0;

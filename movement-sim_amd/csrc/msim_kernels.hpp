@@ -203,7 +203,7 @@ __device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* lds) {
 // Insertions / arrivals of the block go to part_stats (kWordInserts,
 // kWordArrivals of the block's slot).
 template <int MODE, bool UNIFORM, bool WRITE_DIR>
-__global__ __launch_bounds__(kBlock) void k_advance_bin(AgentsDev a, GridDev g,
+static __global__ __launch_bounds__(kBlock) void k_advance_bin(AgentsDev a, GridDev g,
                                                         const msim_road* __restrict__ roads,
                                                         const uint32_t* __restrict__ conns,
                                                         unsigned long long* __restrict__ part_stats) {
@@ -278,7 +278,7 @@ __global__ __launch_bounds__(kBlock) void k_advance_bin(AgentsDev a, GridDev g,
 }
 
 // Per-tile bucket totals.
-__global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t* __restrict__ cnt,
+static __global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t* __restrict__ cnt,
                                                         uint32_t* __restrict__ part) {
     __shared__ uint32_t lds[kBlock / 64];
     const uint4* c4 = reinterpret_cast<const uint4*>(cnt + static_cast<size_t>(blockIdx.x) * kScanTile);
@@ -295,7 +295,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t* __restri
 }
 
 // Exclusive scan of the tile totals (one workgroup); start[m] = total.
-__global__ __launch_bounds__(kPartialsThreads) void k_scan_partials(uint32_t* __restrict__ part,
+static __global__ __launch_bounds__(kPartialsThreads) void k_scan_partials(uint32_t* __restrict__ part,
                                                                     uint32_t n_part,
                                                                     uint32_t* __restrict__ start_end) {
     __shared__ uint32_t lds[kPartialsThreads / 64];
@@ -322,7 +322,7 @@ __global__ __launch_bounds__(kPartialsThreads) void k_scan_partials(uint32_t* __
 }
 
 // Bucket starts for one tile; clears the counts for the next tick.
-__global__ __launch_bounds__(kBlock) void k_scan_down(uint32_t* __restrict__ cnt,
+static __global__ __launch_bounds__(kBlock) void k_scan_down(uint32_t* __restrict__ cnt,
                                                       const uint32_t* __restrict__ part,
                                                       uint32_t* __restrict__ start) {
     __shared__ uint32_t lds[kBlock / 64];
@@ -354,7 +354,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(uint32_t* __restrict__ cnt
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_scatter(AgentsDev a, GridDev g) {
+static __global__ __launch_bounds__(kBlock) void k_scatter(AgentsDev a, GridDev g) {
     const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (i >= a.n) {
         return;
@@ -378,7 +378,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter(AgentsDev a, GridDev g) {
 // finish folds the dispatch's pair total into the counters and history.
 // UNIF enables the wave-uniform path below (heavy cells).
 template <bool SLOW, bool UNIF>
-__global__ __launch_bounds__(kBlock) void k_collide(AgentsDev a, GridDev g, const float2* __restrict__ bpos,
+static __global__ __launch_bounds__(kBlock) void k_collide(AgentsDev a, GridDev g, const float2* __restrict__ bpos,
                                                     unsigned long long* __restrict__ part_pairs) {
     __shared__ uint32_t lds[kBlock / 64];
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
@@ -508,7 +508,7 @@ __global__ __launch_bounds__(kBlock) void k_collide(AgentsDev a, GridDev g, cons
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_set_init(uint32_t* init, uint32_t n) {
+static __global__ __launch_bounds__(kBlock) void k_set_init(uint32_t* init, uint32_t n) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i < n) {
         init[i] = 1u;
@@ -535,7 +535,7 @@ __device__ __forceinline__ unsigned long long take_slots(unsigned long long* par
 
 // Folds the pending advance statistics (insertions, arrivals) into the
 // counters; launched with one wave.
-__global__ __launch_bounds__(64) void k_fold_stats(unsigned long long* __restrict__ part_stats,
+static __global__ __launch_bounds__(64) void k_fold_stats(unsigned long long* __restrict__ part_stats,
                                                    Counters* __restrict__ ctr) {
     const unsigned long long ins = take_slots(part_stats, kWordInserts);
     const unsigned long long arr = take_slots(part_stats, kWordArrivals);
@@ -549,7 +549,7 @@ __global__ __launch_bounds__(64) void k_fold_stats(unsigned long long* __restric
 // from both ends in the fast path, once in the SLOW path) and, when
 // fold_stats is set, the pending advance statistics; launched with one wave.
 template <bool SLOW>
-__global__ __launch_bounds__(64) void k_collide_finish(unsigned long long* __restrict__ part_pairs,
+static __global__ __launch_bounds__(64) void k_collide_finish(unsigned long long* __restrict__ part_pairs,
                                                        Counters* __restrict__ ctr, uint32_t* __restrict__ hist,
                                                        uint32_t hist_mask,
                                                        unsigned long long* __restrict__ part_stats,
@@ -574,7 +574,7 @@ __global__ __launch_bounds__(64) void k_collide_finish(unsigned long long* __res
 
 // Gathers every per-agent array into bucket order so that agents that are
 // close in space are close in memory; afterwards bid is the identity.
-__global__ __launch_bounds__(kBlock) void k_reorder(AgentsDev src, AgentsDev dst, GridDev g) {
+static __global__ __launch_bounds__(kBlock) void k_reorder(AgentsDev src, AgentsDev dst, GridDev g) {
     const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
     if (s >= src.n) {
         return;
@@ -595,7 +595,7 @@ __global__ __launch_bounds__(kBlock) void k_reorder(AgentsDev src, AgentsDev dst
 }
 
 // Per-agent cell export in caller order.
-__global__ __launch_bounds__(kBlock) void k_export_cells(AgentsDev a, float inv_cell, int2* out) {
+static __global__ __launch_bounds__(kBlock) void k_export_cells(AgentsDev a, float inv_cell, int2* out) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= a.n) {
         return;

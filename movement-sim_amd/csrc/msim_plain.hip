@@ -91,7 +91,14 @@ struct msim_sim {
     uint32_t reorder_interval = 0;
     uint32_t since_reorder = 0;
     uint32_t graph_ticks = 32;
-    hipGraphExec_t gexec[2][2][2] = {};  // [cur][bcur][qpar]
+    // captured segments keyed by the start buffers [cur][bcur][qpar]; each
+    // holds whole reorder periods and records the buffers it ends on
+    struct Segment {
+        hipGraphExec_t exec = nullptr;
+        int cur = 0, bcur = 0, qpar = 0;
+    };
+    Segment gexec[2][2][2] = {};
+    uint32_t seg_ticks = 32;
     uint32_t watchdog_ms = 600000;
     uint64_t bytes = 0;
     std::vector<void*> allocs;
@@ -410,9 +417,9 @@ void drop_graphs(msim_sim* s) {
     for (auto& a : s->gexec) {
         for (auto& b : a) {
             for (auto& g : b) {
-                if (g) {
-                    (void)hipGraphExecDestroy(g);
-                    g = nullptr;
+                if (g.exec) {
+                    (void)hipGraphExecDestroy(g.exec);
+                    g.exec = nullptr;
                 }
             }
         }
@@ -452,22 +459,33 @@ int wait_stream(msim_sim* s) {
     }
 }
 
-// Captures graph_ticks fused ticks starting from the current (cur, bcur)
-// buffers; graph_ticks is even, so a replay returns to the same bcur.
+int after_move(msim_sim* s);
+
+// Captures seg_ticks fused ticks starting from the current (cur, bcur, qpar)
+// buffers, right after a reorder: the segment is whole reorder periods, each
+// ending with its reorder, so every replay starts on freshly ordered agents.
 int build_graph(msim_sim* s) {
     hipGraph_t graph = nullptr;
     const int cur0 = s->cur, bcur0 = s->bcur, qpar0 = s->qpar;
+    const uint32_t since0 = s->since_reorder;
+    const uint64_t ticks0 = s->ticks_simulated;
     MSIM_TRY_HIP(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
     int rc = MSIM_OK;
-    for (uint32_t t = 0; t < s->graph_ticks && rc == MSIM_OK; t++) {
+    for (uint32_t t = 0; t < s->seg_ticks && rc == MSIM_OK; t++) {
         rc = enqueue_move(s, kMoveAll, false);
-        if (rc == MSIM_OK) {
-            rc = enqueue_collide(s, false);
-        }
+        if (rc == MSIM_OK) rc = enqueue_collide(s, false);
+        if (rc == MSIM_OK) rc = after_move(s);
     }
     hipError_t e = hipStreamEndCapture(s->stream, &graph);
-    s->bcur = bcur0;  // capture only recorded the ticks; nothing ran
+    msim_sim::Segment& g = s->gexec[cur0][bcur0][qpar0];
+    g.cur = s->cur;
+    g.bcur = s->bcur;
+    g.qpar = s->qpar;
+    s->cur = cur0;  // capture only recorded the ticks; nothing ran
+    s->bcur = bcur0;
     s->qpar = qpar0;
+    s->since_reorder = since0;
+    s->ticks_simulated = ticks0;
     if (rc != MSIM_OK) {
         if (graph) (void)hipGraphDestroy(graph);
         return rc;
@@ -475,10 +493,10 @@ int build_graph(msim_sim* s) {
     if (e != hipSuccess) {
         return fail(s, MSIM_ERR_HIP, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
     }
-    e = hipGraphInstantiate(&s->gexec[cur0][bcur0][qpar0], graph, nullptr, nullptr, 0);
+    e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (e != hipSuccess) {
-        s->gexec[cur0][bcur0][qpar0] = nullptr;
+        g.exec = nullptr;
         return fail(s, MSIM_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
     }
     return MSIM_OK;
@@ -617,7 +635,8 @@ extern "C" int msim_create(const msim_road* roads, uint32_t n_roads, const uint3
     s->n_conns = n_connections;
     s->pc = *pc;
     s->reorder_interval = o.reorder_interval;
-    s->graph_ticks = o.graph_ticks ? (o.graph_ticks + 1u) & ~1u : 32u;  // even: replays keep bcur
+    s->graph_ticks = o.graph_ticks ? o.graph_ticks : 32u;
+    s->seg_ticks = segment_ticks(s->graph_ticks, s->reorder_interval);
     s->watchdog_ms = o.watchdog_ms ? o.watchdog_ms : 600000;
     s->hist_cap = next_pow2_u32(o.history_capacity ? o.history_capacity : 4096);
     s->uniform = (speed == nullptr);
@@ -796,21 +815,21 @@ extern "C" int msim_run_ticks(msim_sim* s, uint32_t n_ticks) {
             continue;
         }
         const uint32_t left = n_ticks - done;
-        const uint32_t until_reorder =
-            s->reorder_interval ? s->reorder_interval - s->since_reorder : UINT32_MAX;
-        if (s->graph_ticks > 0 && left - 1 >= s->graph_ticks && until_reorder >= s->graph_ticks) {
-            if (!s->gexec[s->cur][s->bcur][s->qpar]) {
+        if (left - 1 >= s->seg_ticks && (s->reorder_interval == 0 || s->since_reorder == 0)) {
+            msim_sim::Segment& g = s->gexec[s->cur][s->bcur][s->qpar];
+            if (!g.exec) {
                 rc = build_graph(s);
                 if (rc != MSIM_OK) break;
             }
-            hipError_t e = hipGraphLaunch(s->gexec[s->cur][s->bcur][s->qpar], s->stream);
+            hipError_t e = hipGraphLaunch(g.exec, s->stream);
             if (e != hipSuccess) return fail(s, MSIM_ERR_HIP, "hipGraphLaunch: %s", hipGetErrorString(e));
-            s->pc.tick += 2 * s->graph_ticks;
-            s->ticks_simulated += s->graph_ticks;
-            s->since_reorder += s->graph_ticks;
+            s->cur = g.cur;
+            s->bcur = g.bcur;
+            s->qpar = g.qpar;
+            s->pc.tick += 2 * s->seg_ticks;
+            s->ticks_simulated += s->seg_ticks;
             s->any_collide = true;
-            done += s->graph_ticks;
-            if (s->reorder_interval && s->since_reorder >= s->reorder_interval) rc = reorder(s);
+            done += s->seg_ticks;
         } else {
             // Direction is output-only state: write it on the final tick.
             rc = enqueue_move(s, kMoveAll, left == 1);

@@ -83,6 +83,15 @@ SIGNATURES = {
     "msim_last_error": (ctypes.c_char_p, [_VP]),
     "msim_get_last_error": (ctypes.c_char_p, []),
     "msim_version": (ctypes.c_char_p, []),
+    "msim_batch_create": (ctypes.c_int, [_U32, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP,
+                                         ctypes.POINTER(_VP)]),
+    "msim_batch_run_ticks": (ctypes.c_int, [_VP, _U32]),
+    "msim_batch_sync": (ctypes.c_int, [_VP]),
+    "msim_batch_read_state": (ctypes.c_int, [_VP, _U32, _VP, _VP, _VP, _VP, _VP, _VP]),
+    "msim_batch_read_pairs": (ctypes.c_int, [_VP, _VP, _VP]),
+    "msim_batch_time_ticks": (ctypes.c_int, [_VP, _U32, _VP]),
+    "msim_batch_destroy": (ctypes.c_int, [_VP]),
+    "msim_batch_last_error": (ctypes.c_char_p, [_VP]),
 }
 
 _lib = None
@@ -252,3 +261,99 @@ class Simulation:
         s = ctypes.c_void_p()
         self._check(self.lib.msim_get_stream(self.h, ctypes.byref(s)))
         return s.value
+
+
+class Batch:
+    """Many independent scenarios advanced together (msim_batch_*).
+
+    scenarios: sequence of objects with .roads, .conns, .entities, .speed
+    (float32 array or None) and one shared .pc (collisionRadius and tick must
+    agree).  Either every scenario has per-agent speeds or none has.
+    """
+
+    def __init__(self, scenarios, pc=None, **options):
+        self.lib = load_library()
+        scs = list(scenarios)
+        if not scs:
+            raise ValueError("no scenarios")
+        pcv = np.array(scs[0].pc if pc is None else pc, PUSH_DTYPE).reshape(())
+        speeds = [sc.speed for sc in scs]
+        if any(v is None for v in speeds) and not all(v is None for v in speeds):
+            raise ValueError("either every scenario has per-agent speeds or none has")
+        roads = [np.ascontiguousarray(sc.roads, ROAD_DTYPE) for sc in scs]
+        conns = [np.ascontiguousarray(sc.conns, np.uint32) for sc in scs]
+        ents = [np.ascontiguousarray(sc.entities, ENTITY_DTYPE) for sc in scs]
+
+        def offsets(parts):
+            return np.concatenate([[0], np.cumsum([p.shape[0] for p in parts])]).astype(np.uint32)
+
+        self.road_off, self.conn_off, self.agent_off = offsets(roads), offsets(conns), offsets(ents)
+        r = np.concatenate(roads)
+        c = np.concatenate(conns)
+        e = np.concatenate(ents)
+        spd = None if speeds[0] is None else np.ascontiguousarray(np.concatenate(speeds), np.float32)
+        opts = make_options(**options) if options else None
+        h = ctypes.c_void_p()
+        rc = self.lib.msim_batch_create(len(scs), _ptr(self.road_off), _ptr(r), _ptr(self.conn_off), _ptr(c),
+                                        _ptr(self.agent_off), _ptr(e), _ptr(spd), _ptr(pcv),
+                                        None if opts is None else ctypes.byref(opts), ctypes.byref(h))
+        if rc != MSIM_OK:
+            raise MsimError(rc, self.lib.msim_batch_last_error(None).decode())
+        self.h = h
+        self.n_scen = len(scs)
+        self.n = int(self.agent_off[-1])
+
+    def _check(self, rc):
+        if rc != MSIM_OK:
+            raise MsimError(rc, self.lib.msim_batch_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.msim_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def run_ticks(self, n):
+        self._check(self.lib.msim_batch_run_ticks(self.h, n))
+
+    def sync(self):
+        self._check(self.lib.msim_batch_sync(self.h))
+
+    def state(self, s):
+        """Scenario s's state in its own agent order and road numbering."""
+        n = int(self.agent_off[s + 1] - self.agent_off[s])
+        st = {
+            "pos": np.zeros((n, 2), np.float32),
+            "tgt": np.zeros((n, 2), np.float32),
+            "dir": np.zeros((n, 2), np.float32),
+            "road": np.zeros(n, np.uint32),
+            "rng": np.zeros((n, 4), np.uint32),
+            "ncount": np.zeros(n, np.uint32),
+        }
+        self._check(self.lib.msim_batch_read_state(self.h, s, _ptr(st["pos"]), _ptr(st["tgt"]), _ptr(st["dir"]),
+                                                   _ptr(st["road"]), _ptr(st["rng"]), _ptr(st["ncount"])))
+        return st
+
+    def pairs(self):
+        """(cumulative pairs per scenario, pairs of the last tick per scenario)."""
+        tot = np.zeros(self.n_scen, np.uint64)
+        last = np.zeros(self.n_scen, np.uint32)
+        self._check(self.lib.msim_batch_read_pairs(self.h, _ptr(tot), _ptr(last)))
+        return tot, last
+
+    def time_ticks(self, n):
+        """Milliseconds for n ticks of every scenario (HIP events)."""
+        ms = np.zeros(1, np.float32)
+        self._check(self.lib.msim_batch_time_ticks(self.h, n, _ptr(ms)))
+        return float(ms[0])

@@ -352,8 +352,32 @@ def config(k, scale=1.0, seed=None, layout="own_first"):
                         push_consts(size, size, 1.0), 2_000, (size, size),
                         {"nodes": nodes.shape[0], "config": 3,
                          "max_cell_load": max_cell_load(ents["pos"])})
-    raise ValueError(f"no plain configuration {k}")
+    raise ValueError(f"no plain configuration {k} (the batched workload is batch_config())")
 
+
+
+def batch_config(n_scen=512, scale=1.0, seed=None, layout="own_first"):
+    """The batched workload: n_scen independent lattice scenarios (the plain
+    small construction at 32.3 m node spacing), node counts log-uniform in
+    [1e3, 1e5] and agent counts log-uniform in [1e3, 5e5] (about 80k agents
+    per scenario on average), each with its own seed.  scale < 1 shrinks both
+    counts (parity tests)."""
+    seed = MASTER_SEED + 4 if seed is None else seed
+    rng = np.random.default_rng(seed)
+    nodes = np.exp(rng.uniform(np.log(1e3), np.log(1e5), n_scen))
+    agents = np.exp(rng.uniform(np.log(1e3), np.log(5e5), n_scen))
+    out = []
+    for s in range(n_scen):
+        ss = int(splitmix64(np.uint64(seed) * np.uint64(1000003) + np.uint64(s)) & np.uint64(0x7FFFFFFF))
+        side = max(4, int(round(np.sqrt(nodes[s] * scale))))
+        size = 1000.0 * (side - 1) / 31.0
+        nd, edges = lattice_graph(side=side, size=size, seed=ss)
+        roads, conns = build_roads(nd, edges, layout)
+        n = max(1, int(round(agents[s] * scale)))
+        ents, speed = place_agents(roads, n, ss + 1)
+        out.append(Scenario(f"batch-{s}", roads, conns, ents, speed, push_consts(size, size, 1.0), 5_000,
+                            (size, size), {"nodes": nd.shape[0], "config": 4, "index": s}))
+    return out
 
 # Zipf-Mandelbrot offset for the dense configuration: with q = 0 the heaviest
 # of 1.3M roads would carry ~2M agents (~5e4 per 1 m cell, ~1e11 pair tests

@@ -86,3 +86,51 @@ def test_simulation_raises_on_invalid():
     roads, conns, w, h = S.star_map()
     with pytest.raises(msim.MsimError):
         msim.Simulation(roads, conns[:-1], S.reference_agents(roads, 4, 2), msim.push_consts(w, h, 10.0))
+
+
+def _batch_create(scs, pc):
+    lib = msim.load_library()
+    offs = [np.concatenate([[0], np.cumsum([getattr(s, f).shape[0] for s in scs])]).astype(np.uint32)
+            for f in ("roads", "conns", "entities")]
+    r = np.concatenate([s.roads for s in scs])
+    c = np.concatenate([s.conns for s in scs]).astype(np.uint32)
+    e = np.concatenate([s.entities for s in scs])
+    h = ctypes.c_void_p()
+    rc = lib.msim_batch_create(len(scs), _lib._ptr(offs[0]), _lib._ptr(r), _lib._ptr(offs[1]), _lib._ptr(c),
+                               _lib._ptr(offs[2]), _lib._ptr(e), None, _lib._ptr(pc), None, ctypes.byref(h))
+    return rc, lib.msim_batch_last_error(None).decode()
+
+
+def _star_scenarios(k):
+    roads, conns, w, h = S.star_map()
+    return [S.Scenario(f"s{i}", roads, conns.copy(), S.reference_agents(roads, 8 + i, i), None,
+                       msim.push_consts(w, h, 10.0), 0, (w, h)) for i in range(k)], msim.push_consts(w, h, 10.0)
+
+
+def test_batch_rejects_malformed_scenarios():
+    scs, pc = _star_scenarios(4)
+    scs[2].conns = scs[2].conns[:-1].copy()  # no guard entry in scenario 2 only
+    rc, msg = _batch_create(scs, pc)
+    assert rc == 1 and "scenario 2" in msg and "guard" in msg
+    scs, pc = _star_scenarios(3)
+    scs[1].entities["roadIndex"][0] = 4  # local index past scenario 1's roads
+    rc, msg = _batch_create(scs, pc)
+    assert rc == 1 and "scenario 1" in msg
+
+
+def test_batch_rejects_mixed_init_and_even_tick():
+    scs, pc = _star_scenarios(3)
+    scs[2].entities["initialized"][3] = 1
+    rc, msg = _batch_create(scs, pc)
+    assert rc == 1 and "initialized" in msg
+    scs, pc = _star_scenarios(2)
+    pc["tick"] = 2
+    rc, msg = _batch_create(scs, pc)
+    assert rc == 1 and "tick" in msg
+
+
+def test_batch_binding_rejects_mixed_speed_modes():
+    sa = S.batch_config(2, scale=0.01)
+    sa[1].speed = None
+    with pytest.raises(ValueError):
+        msim.Batch(sa)
