@@ -1,11 +1,13 @@
-"""Benchmark: fused ticks of the plain pipeline on one MI355X.
+"""Benchmark on one MI355X: agent-ticks/s of the batched form against the
+plain per-scenario loop at bit-exact state, and of the plain drop-in.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2] [--cpu-seconds S]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 4] [--cpu-seconds S]
 
 A step is one tick (Simulator::sim_tick: move dispatch + collision dispatch)
-over every agent of the workload.  Inputs are uploaded to HBM before timing;
-K ticks run as hipGraph replays between two stream synchronisations.  Prints
-one JSON line.
+over every agent of the workload.  --config 4 (default) is the batched
+workload (512 scenarios); 1-3 are plain single-map workloads.  Inputs are
+uploaded to HBM before timing; K ticks run as hipGraph segment replays
+between two stream synchronisations.  Prints one JSON line.
 """
 import argparse
 import json
@@ -19,6 +21,9 @@ sys.path.insert(0, os.path.join(ROOT, "movement-sim_amd"))
 METRIC = "agent-ticks/s, 1 GPU: batched vs plain at bit-exact state; HBM GB/s % of peak"
 
 WORKLOADS = {
+    4: "batched: 512 independent jittered-lattice scenarios (32.3 m node spacing, 15% of edges dropped), "
+       "nodes log-uniform in [1e3, 1e5], agents log-uniform in [1e3, 5e5] (~41M agents), speed "
+       "U[0.8,1.6) m/tick, R 1 m, 1 m cells",
     1: "plain small: 32x32 jittered lattice (1,024 nodes, 1,686 roads), 10,000 agents, R 1 m, 1 m cells",
     2: "city-scale plain drop-in: 150k-node Gaussian-clustered Delaunay city graph (12 km square, "
        "mean degree 2.6), 1,000,000 agents, speed U[0.8,1.6) m/tick, R 1 m, 1 m cells",
@@ -27,57 +32,46 @@ WORKLOADS = {
 }
 
 
-def cpu_baseline(sc, seconds, threads):
+def cpu_baseline(scs, seconds, threads):
     """The CPU oracle (restatement; the reference itself needs a Vulkan ICD)
-    on the same initial state: one untimed init tick, then full ticks until
-    `seconds` elapse."""
+    on a bounded sample of the workload: scenarios in order, each from its own
+    initial state (one untimed insert-only tick), ticked until its share of
+    `seconds` elapses."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import orc
 
-    o = orc.OracleSim(sc.roads, sc.conns, sc.entities, sc.pc, speed=sc.speed, threads=threads)
-    o.sim_tick()  # insert-only move dispatch + first collision dispatch
-    ticks = 0
-    t0 = time.perf_counter()
-    while True:
-        o.sim_tick()
-        ticks += 1
-        el = time.perf_counter() - t0
-        if el >= seconds:
-            break
-    n = sc.entities.shape[0]
-    return {"value": n * ticks / el, "unit": "agent-ticks/s", "cores": threads, "kind": "port",
-            "sample": f"{n} agents x {ticks} ticks of the same workload ({el:.1f} s), oracle/msim_oracle.c "
-                      f"with {threads} POSIX threads"}
+    k = min(len(scs), 8)
+    share = seconds / k
+    agent_ticks = 0
+    el_total = 0.0
+    parts = []
+    for sc in scs[:k]:
+        o = orc.OracleSim(sc.roads, sc.conns, sc.entities, sc.pc, speed=sc.speed, threads=threads)
+        o.sim_tick()  # insert-only move dispatch + first collision dispatch
+        ticks = 0
+        t0 = time.perf_counter()
+        while True:
+            o.sim_tick()
+            ticks += 1
+            el = time.perf_counter() - t0
+            if el >= share:
+                break
+        n = sc.entities.shape[0]
+        agent_ticks += n * ticks
+        el_total += el
+        parts.append(f"{n}x{ticks}")
+    return {"value": agent_ticks / el_total, "unit": "agent-ticks/s", "cores": threads, "kind": "port",
+            "sample": f"first {k} scenario(s) of the same workload, agents x ticks = {', '.join(parts)} "
+                      f"({el_total:.1f} s), oracle/msim_oracle.c with {threads} POSIX threads"}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10000)
-    ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--config", type=int, default=2, choices=(1, 2, 3))
-    ap.add_argument("--profile-ticks", type=int, default=240)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--untuned", action="store_true", help="library defaults instead of msim/tuned.json")
-    args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build measures one GPU only; multi-GPU runs are not part of it "
-              "(use --gpus 1)", file=sys.stderr)
-        sys.exit(2)
-
+def plain_line(args, sc):
+    """Plain drop-in on one workload: fused ticks through the graph path."""
     import msim
-    from msim import roofline as RL
-    from msim import scenarios as S
-
-    t = time.perf_counter()
-    sc = S.config(args.config)
-    gen_s = time.perf_counter() - t
-    n = sc.entities.shape[0]
-    print(f"# workload config {args.config}: {n} agents, {sc.roads.shape[0]} roads, "
-          f"{sc.conns.shape[0]} connection entries (generated in {gen_s:.1f} s)", file=sys.stderr, flush=True)
     from msim import autotune
+    from msim import roofline as RL
 
+    n = sc.entities.shape[0]
     opts = dict(autotune.DEFAULT) if args.untuned else autotune.tuned_options(sc)
     sim = msim.Simulation(sc.roads, sc.conns, sc.entities, sc.pc, speed=sc.speed, **opts)
     info = sim.info()
@@ -88,32 +82,184 @@ def main():
     sim.run_ticks(args.steps)
     sim.sync()
     el = time.perf_counter() - t0
-    ms_per_step = 1e3 * el / args.steps
-    value = n * args.steps / el
     print(f"# timed {args.steps} ticks in {el:.3f} s", file=sys.stderr, flush=True)
 
     # per-kernel durations: HIP events right before and after one launch per
     # tick, on the handle's stream, each kernel probed in turn
     _, kms = sim.time_ticks(args.profile_ticks, kernels=True)
     per_launch_ms = {k: v / args.profile_ticks for k, v in kms.items()}
+    roof = roofline(per_launch_ms, n, bool(info["uniform_speed"]), args.config)
+    config = {
+        "workload": WORKLOADS[args.config],
+        "agents": n,
+        "roads": int(sc.roads.shape[0]),
+        "nodes": int(sc.meta.get("nodes", 0)),
+        "collision_radius_m": float(sc.pc["collisionRadius"]),
+        "cell_m": info["cell_size"],
+        "bucket_table": [info["table_width"], info["table_height"]],
+        "options": opts,
+    }
+    extra = {"fused_tick": fused(el / args.steps, n, per_launch_ms)}
+    return n * args.steps / el, 1e3 * el / args.steps, config, roof, extra
+
+
+def roofline(per_launch_ms, n, uniform, cfg):
+    """Dominant kernel's algorithmic bytes per launch / its average launch time."""
+    from msim import roofline as RL
+
     dom = max(per_launch_ms, key=per_launch_ms.get)
-    uniform = bool(info["uniform_speed"])
     dom_bytes = RL.kernel_bytes(dom, uniform) * n
     achieved = dom_bytes / (per_launch_ms[dom] * 1e-3) / 1e9
     traffic = None
-    tf = os.path.join(ROOT, "profiles", f"traffic_config{args.config}.json")
+    tf = os.path.join(ROOT, "profiles", f"traffic_config{cfg}.json")
     if os.path.exists(tf):
         with open(tf) as f:
-            tj = json.load(f)
-        traffic = tj.get("kernels", {}).get(dom, {}).get("bytes_per_launch")
-    tick_kernel_ms = sum(per_launch_ms.values())
-    fused_gbs = RL.B_FUSED * n / (ms_per_step * 1e-3) / 1e9
+            traffic = json.load(f).get("kernels", {}).get(dom.split("<")[0], {}).get("bytes_per_launch")
+    peak = RL.HBM_PEAK_BYTES_PER_S / 1e9
+    return {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": peak, "unit": "GB/s",
+            "frac": achieved / peak, "traffic": traffic, "algorithmic_bytes_per_launch": dom_bytes,
+            "avg_launch_us": per_launch_ms[dom] * 1e3,
+            "kernels_us_per_tick": {k: v * 1e3 for k, v in per_launch_ms.items()}}
+
+
+def fused(s_per_tick, n, per_launch_ms):
+    from msim import roofline as RL
+
+    gbs = RL.B_FUSED * n / s_per_tick / 1e9
+    return {"bytes_per_agent": RL.B_FUSED, "achieved_GBps": gbs, "frac": gbs / (RL.HBM_PEAK_BYTES_PER_S / 1e9),
+            "kernel_sum_us": sum(per_launch_ms.values()) * 1e3}
+
+
+def compare_plain(args, scs, b, n):
+    """The same ticks from the same initial state, batched and through the
+    plain per-scenario loop; every scenario's state compared bit for bit."""
+    import numpy as np
+
+    import msim
+    from msim import autotune
+
+    t0 = time.perf_counter()
+    b.run_ticks(args.compare_ticks)
+    b.sync()
+    t_batch_cmp = time.perf_counter() - t0
+    t_plain = 0.0
+    exact = True
+    first_bad = None
+    batch_pairs = b.pairs()[0]
+    for s, sc in enumerate(scs):
+        g = msim.Simulation(sc.roads, sc.conns, sc.entities, sc.pc, speed=sc.speed, **autotune.tuned_options(sc))
+        g.sync()
+        t0 = time.perf_counter()
+        g.run_ticks(args.compare_ticks)
+        g.sync()
+        t_plain += time.perf_counter() - t0
+        bs, ps = b.state(s), g.state()
+        same = all(np.array_equal(np.ascontiguousarray(bs[k]).view(np.uint32),
+                                  np.ascontiguousarray(ps[k]).view(np.uint32)) for k in bs)
+        same = same and int(batch_pairs[s]) == g.pairs()[0]
+        if not same and first_bad is None:
+            first_bad = s
+        exact = exact and same
+        g.close()
+    print(f"# comparison: {args.compare_ticks} ticks, batched {t_batch_cmp:.3f} s, plain loop {t_plain:.3f} s, "
+          f"bit-exact {exact}", file=sys.stderr, flush=True)
+    plain_rate = n * args.compare_ticks / t_plain
+    batch_cmp_rate = n * args.compare_ticks / t_batch_cmp
+    return {
+        "ticks": args.compare_ticks,
+        "batched_agent_ticks_per_s": batch_cmp_rate,
+        "plain_loop_agent_ticks_per_s": plain_rate,
+        "speedup": batch_cmp_rate / plain_rate,
+        "bit_exact_scenarios": exact,
+        "first_mismatch": first_bad,
+        "plain_loop": "one plain handle per scenario (tuned options), created and uploaded before its "
+                      "timed run_ticks; times summed over the scenarios",
+    }
+
+
+def batched_line(args, scs):
+    """Batched workload: every scenario advanced by one set of launches per
+    tick (graph segments), against the plain per-scenario loop on the same
+    scenarios and ticks, with every scenario's final state compared bit for
+    bit."""
+    import msim
+
+    n = sum(int(sc.entities.shape[0]) for sc in scs)
+    b = msim.Batch(scs, reorder_interval=args.batch_reorder, graph_ticks=args.batch_graph_ticks)
+    extra = {}
+    if args.compare_ticks > 0:
+        extra["batched_vs_plain"] = compare_plain(args, scs, b, n)
+    b.run_ticks(max(1, args.warmup))
+    b.sync()
+    t0 = time.perf_counter()
+    b.run_ticks(args.steps)
+    b.sync()
+    el = time.perf_counter() - t0
+    print(f"# timed {args.steps} batched ticks in {el:.3f} s", file=sys.stderr, flush=True)
+    _, kms = b.time_ticks(args.profile_ticks, kernels=True)
+    per_launch_ms = {k: v / args.profile_ticks for k, v in kms.items()}
+    roof = roofline(per_launch_ms, n, scs[0].speed is None, 4)
+    agents = [int(sc.entities.shape[0]) for sc in scs]
+    config = {
+        "workload": WORKLOADS[4],
+        "scenarios": len(scs),
+        "agents": n,
+        "agents_min_max": [min(agents), max(agents)],
+        "roads": int(sum(sc.roads.shape[0] for sc in scs)),
+        "collision_radius_m": float(scs[0].pc["collisionRadius"]),
+        "graph_segment_ticks": args.batch_graph_ticks,
+        "reorder_interval": args.batch_reorder,
+    }
+    extra["fused_tick"] = fused(el / args.steps, n, per_launch_ms)
+    return n * args.steps / el, 1e3 * el / args.steps, config, roof, extra
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None, help="timed ticks (default: 1000 batched, 10000 plain)")
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--config", type=int, default=4, choices=(1, 2, 3, 4))
+    ap.add_argument("--scenarios", type=int, default=512, help="batched workload: number of scenarios")
+    ap.add_argument("--compare-ticks", type=int, default=200, help="batched workload: ticks of the plain-loop "
+                                                                    "comparison")
+    ap.add_argument("--batch-reorder", type=int, default=20)
+    ap.add_argument("--batch-graph-ticks", type=int, default=100)
+    ap.add_argument("--profile-ticks", type=int, default=240)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--untuned", action="store_true", help="library defaults instead of msim/tuned.json")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build measures one GPU only; multi-GPU runs are not part of it "
+              "(use --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+    if args.steps is None:
+        args.steps = {1: 10000, 2: 10000, 3: 500, 4: 1000}[args.config]
+
+    from msim import scenarios as S
+
+    t = time.perf_counter()
+    if args.config == 4:
+        scs = S.batch_config(args.scenarios)
+        n = sum(int(sc.entities.shape[0]) for sc in scs)
+        print(f"# batched workload: {len(scs)} scenarios, {n} agents (generated in "
+              f"{time.perf_counter() - t:.1f} s)", file=sys.stderr, flush=True)
+        value, ms_per_step, config, roof, extra = batched_line(args, scs)
+        sample_sc = scs
+    else:
+        sc = S.config(args.config)
+        print(f"# workload config {args.config}: {sc.entities.shape[0]} agents, {sc.roads.shape[0]} roads "
+              f"(generated in {time.perf_counter() - t:.1f} s)", file=sys.stderr, flush=True)
+        value, ms_per_step, config, roof, extra = plain_line(args, sc)
+        sample_sc = [sc]
 
     cpu = None
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(sc, args.cpu_seconds, threads)
+        cpu = cpu_baseline(sample_sc, args.cpu_seconds, threads)
 
+    kus = roof.pop("kernels_us_per_tick")
     out = {
         "metric": METRIC,
         "value": value,
@@ -127,32 +273,10 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded generator, msim/scenarios.py)",
-        "config": {
-            "workload": WORKLOADS[args.config],
-            "agents": n,
-            "roads": int(sc.roads.shape[0]),
-            "nodes": int(sc.meta.get("nodes", 0)),
-            "collision_radius_m": float(sc.pc["collisionRadius"]),
-            "cell_m": info["cell_size"],
-            "bucket_table": [info["table_width"], info["table_height"]],
-            "options": opts,
-            "graph_ticks": 32,
-        },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": dom,
-            "achieved": achieved,
-            "peak": RL.HBM_PEAK_BYTES_PER_S / 1e9,
-            "unit": "GB/s",
-            "frac": achieved / (RL.HBM_PEAK_BYTES_PER_S / 1e9),
-            "traffic": traffic,
-            "algorithmic_bytes_per_launch": dom_bytes,
-            "avg_launch_us": per_launch_ms[dom] * 1e3,
-        },
-        "kernels_us_per_tick": {k: v * 1e3 for k, v in per_launch_ms.items()},
-        "fused_tick": {"bytes_per_agent": RL.B_FUSED, "achieved_GBps": fused_gbs,
-                       "frac": fused_gbs / (RL.HBM_PEAK_BYTES_PER_S / 1e9),
-                       "kernel_sum_us": tick_kernel_ms * 1e3},
+        "config": config,
+        "roofline": roof,
+        "kernels_us_per_tick": kus,
+        **extra,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

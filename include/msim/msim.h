@@ -55,9 +55,8 @@ typedef struct msim_options {
                                   per-segment offsets); 1: global-atomic counting */
     uint32_t route_queue;      /* 0: arrivals routed inside the advance kernel;
                                   1: queued and routed by a separate pass       */
-    uint32_t collide_wave_uniform; /* 0: every lane walks its own candidates;
-                                  1: waves inside one bucket range share one
-                                  candidate walk (readlane broadcast)           */
+    uint32_t collide_mode;     /* 0: each wave stages its candidates in LDS with
+                                  16-byte loads; 1: every lane loads its own     */
 } msim_options;
 
 typedef struct msim_info {
@@ -207,8 +206,15 @@ int msim_batch_read_state(msim_batch* b, uint32_t scen, float* pos, float* targe
  * last collision dispatch (n_scen entries); either may be NULL. */
 int msim_batch_read_pairs(msim_batch* b, uint64_t* totals, uint32_t* last);
 
-/* Elapsed milliseconds of n_ticks batched ticks (HIP events). */
-int msim_batch_time_ticks(msim_batch* b, uint32_t n_ticks, float* ms_total);
+/* Elapsed milliseconds of n_ticks batched ticks (HIP events).  kernel_ms (may
+ * be NULL) receives each kernel's summed duration over a second, separate
+ * pass of n_ticks ticks in which every launch is bracketed by events, in
+ * msim_batch_kernel_name order.  Both passes advance the simulation. */
+int msim_batch_time_ticks(msim_batch* b, uint32_t n_ticks, float* ms_total, float* kernel_ms,
+                          uint32_t n_kernel_ms);
+
+/* Name of the k-th kernel of a batched tick, or NULL past the last one. */
+const char* msim_batch_kernel_name(uint32_t k);
 
 int msim_batch_destroy(msim_batch* b);
 const char* msim_batch_last_error(const msim_batch* b);

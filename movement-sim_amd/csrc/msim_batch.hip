@@ -93,24 +93,16 @@ __global__ __launch_bounds__(kBinThreads) void k_bbin_place(const float2* __rest
                       b.new_start + sc.start_base, b.new_bpos, b.new_bid);
 }
 
-// Largest s with off[s] <= x (off has n_scen + 1 entries, off[0] = 0).
-__device__ __forceinline__ uint32_t find_scen(const uint32_t* __restrict__ off, uint32_t n_scen, uint32_t x) {
-    uint32_t lo = 0, hi = n_scen;  // invariant: off[lo] <= x < off[hi]
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (off[mid] <= x) {
-            lo = mid;
-        } else {
-            hi = mid;
-        }
-    }
-    return lo;
-}
+// Table parameters of the scenario holding a block's first slot, and the
+// slot where that scenario ends (read with scalar loads: block-uniform).
+struct BlockScen {
+    uint32_t sid, end, start_base, lw, wmask, hmask, pad0, pad1;
+};
 
 struct BatchCollide {
+    const BlockScen* blk;
     const ScenDev* scen;
     const uint32_t* agent_off;
-    uint32_t n_scen;
     uint32_t n;
     const uint32_t* start;
     const float2* bpos;
@@ -121,49 +113,48 @@ struct BatchCollide {
     unsigned long long* part;  // [n_scen][kScenSlots * kScenSlotStride]
 };
 
-// Neighbour pass over every binned slot of every scenario (the plain
-// k_collide's per-lane walk on the slot's scenario table).
+// Neighbour pass over every binned slot of every scenario, each slot on its
+// own scenario's bucket table (count_in_cells, as in the plain pass).
+template <bool STAGED>
 __global__ __launch_bounds__(kBlock) void k_bcollide(BatchCollide c) {
-    const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    __shared__ float2 stage_all[STAGED ? kBlock / 64 * kStage : 1];
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t s = blk * kBlock + threadIdx.x;
     const bool valid = s < c.n;
+    const BlockScen bs = c.blk[blk];
     uint32_t cnt = 0;
-    uint32_t sid = 0;
-    if (valid) {
-        sid = find_scen(c.agent_off, c.n_scen, s);
-        const ScenDev sc = c.scen[sid];
-        const uint32_t* start = c.start + sc.start_base;
-        const float2 me = c.bpos[s];
-        const int32_t cx = cell_coord(me.x, c.inv_cell);
-        const int32_t cy = cell_coord(me.y, c.inv_cell);
-        const float T = c.d2_limit;
-#pragma unroll 1
-        for (int dy = -1; dy <= 1; dy++) {
-            const uint32_t rowb = (static_cast<uint32_t>(cy + dy) & sc.hmask) << sc.lw;
-            const uint32_t c0 = static_cast<uint32_t>(cx - 1) & sc.wmask;
-            const uint32_t c2 = static_cast<uint32_t>(cx + 1) & sc.wmask;
-            uint32_t r0 = start[rowb + c0], r1, r2 = 0, r3 = 0;
-            if (c0 < c2) {
-                r1 = start[rowb + c2 + 1];
-            } else {
-                r1 = start[rowb + sc.wmask + 1];
-                r2 = start[rowb];
-                r3 = start[rowb + c2 + 1];
+    uint32_t sid = bs.sid;
+    const float2 me = valid ? c.bpos[s] : make_float2(0.0f, 0.0f);
+    const int32_t cx = cell_coord(me.x, c.inv_cell);
+    const int32_t cy = cell_coord(me.y, c.inv_cell);
+    if (valid && s >= bs.end) {  // a later scenario starts inside this block
+        sid++;
+        while (s >= c.agent_off[sid + 1]) sid++;
+    }
+    const uint32_t w0 = s - __lane_id();
+    bool done = false;
+    if (STAGED && w0 < c.n) {
+        const uint32_t sid0 = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(sid)));
+        if (__ballot(valid && sid != sid0) == 0ull) {  // the wave lies in one scenario
+            uint32_t base = bs.start_base, lw = bs.lw, wmask = bs.wmask, hmask = bs.hmask;
+            if (sid0 != bs.sid) {
+                const ScenDev& sc = c.scen[sid0];
+                base = sc.start_base;
+                lw = sc.lw;
+                wmask = sc.wmask;
+                hmask = sc.hmask;
             }
-#pragma unroll 1
-            for (int seg = 0; seg < 2; seg++) {
-                const uint32_t lo = seg ? r2 : r0;
-                const uint32_t hi = seg ? r3 : r1;
-                uint32_t k = lo;
-                for (; k + 4u <= hi; k += 4u) {
-                    const float2 q0 = c.bpos[k], q1 = c.bpos[k + 1], q2 = c.bpos[k + 2], q3 = c.bpos[k + 3];
-                    cnt += (in_range(q0, me, T) ? 1u : 0u) + (in_range(q1, me, T) ? 1u : 0u) +
-                           (in_range(q2, me, T) ? 1u : 0u) + (in_range(q3, me, T) ? 1u : 0u);
-                }
-                for (; k < hi; k++) {
-                    cnt += in_range(c.bpos[k], me, T) ? 1u : 0u;
-                }
-            }
+            cnt = wave_count_in_cells(me, valid, min(63u, c.n - 1u - w0), cx, cy, c.start + base, lw, wmask,
+                                      hmask, c.bpos, c.d2_limit, stage_all + (threadIdx.x >> 6) * kStage);
+            done = true;
         }
+    }
+    if (valid && !done) {
+        const ScenDev& sc = c.scen[sid];
+        cnt = count_in_cells(me, s, cx, cy, c.start + sc.start_base, sc.lw, sc.wmask, sc.hmask, c.bpos,
+                             c.d2_limit);
+    }
+    if (valid) {
         cnt -= 1u;  // own slot
         c.ncount[c.bid[s]] = cnt;
     }
@@ -173,7 +164,7 @@ __global__ __launch_bounds__(kBlock) void k_bcollide(BatchCollide c) {
     const int first = act ? __ffsll(static_cast<long long>(act)) - 1 : 0;
     const uint32_t s0 = __shfl(sid, first, 64);
     const bool one = __ballot(valid && sid != s0) == 0ull;
-    const uint32_t wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
+    const uint32_t wave = (blk * kBlock + threadIdx.x) >> 6;
     const uint32_t slot = wave % kScenSlots;
     if (one) {
         unsigned long long v = cnt;
@@ -241,6 +232,7 @@ struct msim_batch {
     float inv_cell = 1.0f, d2_limit = 1.0f;
     ScenDev* scen = nullptr;
     uint32_t* agent_off = nullptr;
+    BlockScen* blk_scen = nullptr;
     uint32_t* seg_scen = nullptr;
     uint32_t* seg_total = nullptr;
     uint32_t* bstart[2] = {nullptr, nullptr};
@@ -254,6 +246,7 @@ struct msim_batch {
     unsigned int* faults = nullptr;
     uint32_t* qdummy = nullptr;
     bool first_tick_insert_only = false;
+    bool collide_per_lane = false;
     uint32_t tick = 1;
     uint32_t graph_ticks = 100, seg_ticks = 100;
     uint32_t reorder_interval = 20, since_reorder = 0;
@@ -263,6 +256,10 @@ struct msim_batch {
     };
     Segment gexec[2][2] = {};  // [cur][bcur] at the segment's start
     uint32_t watchdog_ms = 600000;
+    // msim_batch_time_ticks probe: launch number probe_kernel of a tick is
+    // bracketed by probe_ev[0] / probe_ev[1]; -1 = no probe
+    int probe_kernel = -1, launch_no = 0;
+    hipEvent_t probe_ev[2] = {nullptr, nullptr};
     std::vector<void*> allocs;
     std::string err;
 
@@ -312,6 +309,19 @@ int bcheck(msim_batch* b, const char* what) {
     return MSIM_OK;
 }
 
+// Launch order of a batched tick, for msim_batch_time_ticks.
+const char* const kBatchKernels[] = {"k_advance", "k_bbin_count", "k_bbin_place", "k_bcollide", "k_bfinish"};
+constexpr int kBatchKernelCount = 5;
+
+void bprobe_pre(msim_batch* b) {
+    if (b->probe_kernel == b->launch_no) (void)hipEventRecord(b->probe_ev[0], b->stream);
+}
+
+void bprobe_post(msim_batch* b) {
+    if (b->probe_kernel == b->launch_no) (void)hipEventRecord(b->probe_ev[1], b->stream);
+    b->launch_no++;
+}
+
 int bwait(msim_batch* b) {
     const auto t0 = std::chrono::steady_clock::now();
     for (;;) {
@@ -337,9 +347,9 @@ GridDev bgrid(const msim_batch* b) {
 
 int benqueue_collide(msim_batch* b) {
     BatchCollide c{};
+    c.blk = b->blk_scen;
     c.scen = b->scen;
     c.agent_off = b->agent_off;
-    c.n_scen = b->n_scen;
     c.n = b->n;
     c.start = b->bstart[b->bcur];
     c.bpos = b->bbpos[b->bcur];
@@ -348,8 +358,16 @@ int benqueue_collide(msim_batch* b) {
     c.inv_cell = b->inv_cell;
     c.d2_limit = b->d2_limit;
     c.part = b->part;
-    hipLaunchKernelGGL(k_bcollide, dim3(bblocks(b->n)), dim3(kBlock), 0, b->stream, c);
+    bprobe_pre(b);
+    if (b->collide_per_lane) {
+        hipLaunchKernelGGL(k_bcollide<false>, dim3(bblocks(b->n)), dim3(kBlock), 0, b->stream, c);
+    } else {
+        hipLaunchKernelGGL(k_bcollide<true>, dim3(bblocks(b->n)), dim3(kBlock), 0, b->stream, c);
+    }
+    bprobe_post(b);
+    bprobe_pre(b);
     hipLaunchKernelGGL(k_bfinish, dim3(b->n_scen), dim3(64), 0, b->stream, b->part, b->totals, b->last);
+    bprobe_post(b);
     return bcheck(b, "batched collision dispatch");
 }
 
@@ -361,6 +379,8 @@ void badvance(msim_batch* b) {
 }
 
 int benqueue_move(msim_batch* b, bool write_dir) {
+    b->launch_no = 0;
+    bprobe_pre(b);
     if (b->uniform) {
         if (write_dir) badvance<true, true>(b);
         else badvance<true, false>(b);
@@ -368,6 +388,7 @@ int benqueue_move(msim_batch* b, bool write_dir) {
         if (write_dir) badvance<false, true>(b);
         else badvance<false, false>(b);
     }
+    bprobe_post(b);
     const int nb = 1 - b->bcur;
     BatchBin bb{};
     bb.scen = b->scen;
@@ -382,8 +403,12 @@ int benqueue_move(msim_batch* b, bool write_dir) {
     bb.inv_cell = b->inv_cell;
     const size_t lds = sizeof(uint32_t) << b->max_seg_log;
     const float2* pos = b->agents(b->cur).pos;
+    bprobe_pre(b);
     hipLaunchKernelGGL(k_bbin_count, dim3(b->n_seg), dim3(kBinThreads), lds, b->stream, pos, bb);
+    bprobe_post(b);
+    bprobe_pre(b);
     hipLaunchKernelGGL(k_bbin_place, dim3(b->n_seg), dim3(kBinThreads), lds, b->stream, pos, bb);
+    bprobe_post(b);
     b->bcur = nb;
     return bcheck(b, "batched move dispatch");
 }
@@ -550,6 +575,7 @@ extern "C" int msim_batch_create(uint32_t n_scen, const uint32_t* road_off, cons
     b->graph_ticks = o.graph_ticks ? o.graph_ticks : 100u;
     b->reorder_interval = opt ? o.reorder_interval : 20u;
     b->seg_ticks = segment_ticks(b->graph_ticks, b->reorder_interval);
+    b->collide_per_lane = o.collide_mode != 0;
     b->watchdog_ms = o.watchdog_ms ? o.watchdog_ms : 600000;
     b->agent_off_h.assign(agent_off, agent_off + n_scen + 1);
     b->road_off_h.assign(road_off, road_off + n_scen + 1);
@@ -618,11 +644,12 @@ extern "C" int msim_batch_create(uint32_t n_scen, const uint32_t* road_off, cons
         BALLOC(bb.ncount, n);
         BALLOC(bb.perm, n);
         BALLOC(b->bstart[k], start_len);
-        BALLOC(b->bbpos[k], n);
+        BALLOC(b->bbpos[k], n + 2);  // 16-byte staged reads may touch one slot past n
         BALLOC(b->bbid[k], n);
     }
     BALLOC(b->scen, n_scen);
     BALLOC(b->agent_off, n_scen + 1);
+    BALLOC(b->blk_scen, bblocks(n));
     BALLOC(b->seg_scen, b->n_seg);
     BALLOC(b->seg_total, b->n_seg);
     BALLOC(b->part, static_cast<size_t>(n_scen) * kScenSlots * kScenSlotStride);
@@ -686,6 +713,14 @@ extern "C" int msim_batch_create(uint32_t n_scen, const uint32_t* road_off, cons
     up(b0.perm, perm.data(), sizeof(uint32_t) * n);
     up(b->scen, sc.data(), sizeof(ScenDev) * n_scen);
     up(b->agent_off, agent_off, sizeof(uint32_t) * (n_scen + 1));
+    std::vector<BlockScen> blk(bblocks(n));
+    for (uint32_t k = 0, sid = 0; k < blk.size(); k++) {
+        const uint32_t s0 = k * kBlock;
+        while (sid + 1 < n_scen && agent_off[sid + 1] <= s0) sid++;  // skips empty scenarios
+        const ScenDev& d = sc[sid];
+        blk[k] = BlockScen{sid, agent_off[sid + 1], d.start_base, d.lw, d.wmask, d.hmask, 0u, 0u};
+    }
+    up(b->blk_scen, blk.data(), sizeof(BlockScen) * blk.size());
     up(b->seg_scen, seg_scen.data(), sizeof(uint32_t) * b->n_seg);
     up(b->bstart[0], start_h.data(), sizeof(uint32_t) * start_len);
     up(b->bbpos[0], bpos_h.data(), sizeof(float2) * n);
@@ -700,6 +735,12 @@ extern "C" int msim_batch_create(uint32_t n_scen, const uint32_t* road_off, cons
         if (e == hipSuccess) e = hipMemset(b->qdummy, 0, sizeof(uint32_t) * kQueues * kQueueStride);
         if (e == hipSuccess) e = hipMemset(b->bstart[1], 0, sizeof(uint32_t) * start_len);
         if (e != hipSuccess) rc = bfail(b, MSIM_ERR_HIP, "hipMemset failed: %s", hipGetErrorString(e));
+    }
+    // the uploads and fills above run on the null stream, which the handle's
+    // non-blocking stream does not wait for
+    if (rc == MSIM_OK) {
+        hipError_t e = hipStreamSynchronize(nullptr);
+        if (e != hipSuccess) rc = bfail(b, MSIM_ERR_HIP, "upload failed: %s", hipGetErrorString(e));
     }
     if (rc == MSIM_OK) rc = breorder(b);
     if (rc == MSIM_OK) rc = bwait(b);
@@ -824,20 +865,56 @@ extern "C" int msim_batch_read_pairs(msim_batch* b, uint64_t* totals, uint32_t* 
     return MSIM_OK;
 }
 
-extern "C" int msim_batch_time_ticks(msim_batch* b, uint32_t n_ticks, float* ms_total) {
-    if (!b || !ms_total) return bfail(b, MSIM_ERR_INVALID, "NULL argument");
-    hipEvent_t e0, e1;
-    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+extern "C" int msim_batch_time_ticks(msim_batch* b, uint32_t n_ticks, float* ms_total, float* kernel_ms,
+                                     uint32_t n_kernel_ms) {
+    if (!b) return bfail(b, MSIM_ERR_INVALID, "NULL handle");
+    hipEvent_t ev[2];
+    if (hipEventCreate(&ev[0]) != hipSuccess) return bfail(b, MSIM_ERR_HIP, "hipEventCreate failed");
+    if (hipEventCreate(&ev[1]) != hipSuccess) {
+        (void)hipEventDestroy(ev[0]);
         return bfail(b, MSIM_ERR_HIP, "hipEventCreate failed");
     }
-    (void)hipEventRecord(e0, b->stream);
-    int rc = msim_batch_run_ticks(b, n_ticks);
-    (void)hipEventRecord(e1, b->stream);
-    if (rc == MSIM_OK) rc = msim_batch_sync(b);
-    if (rc == MSIM_OK) (void)hipEventElapsedTime(ms_total, e0, e1);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    int rc = MSIM_OK;
+    if (ms_total) {
+        (void)hipEventRecord(ev[0], b->stream);
+        rc = msim_batch_run_ticks(b, n_ticks);
+        (void)hipEventRecord(ev[1], b->stream);
+        if (rc == MSIM_OK) rc = msim_batch_sync(b);
+        if (rc == MSIM_OK) (void)hipEventElapsedTime(ms_total, ev[0], ev[1]);
+    }
+    if (rc == MSIM_OK && kernel_ms && n_kernel_ms) {
+        uint32_t t0 = 0;
+        if (n_ticks && b->first_tick_insert_only) {  // no move kernels in that tick
+            rc = msim_batch_run_ticks(b, 1);
+            t0 = 1;
+        }
+        // kernel k of a tick timed alone, the tick's other launches untimed
+        double acc[kBatchKernelCount] = {0, 0, 0, 0, 0};
+        for (uint32_t t = t0; t < n_ticks && rc == MSIM_OK; t++) {
+            const int k = static_cast<int>((t - t0) % kBatchKernelCount);
+            b->probe_kernel = k;
+            b->probe_ev[0] = ev[0];
+            b->probe_ev[1] = ev[1];
+            rc = btick(b, t + 1 == n_ticks);
+            b->probe_kernel = -1;
+            if (rc == MSIM_OK) rc = bwait(b);
+            float ms = 0.f;
+            if (rc == MSIM_OK && hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) {
+                acc[k] += static_cast<double>(ms) * kBatchKernelCount;
+            }
+        }
+        for (uint32_t k = 0; k < n_kernel_ms && k < static_cast<uint32_t>(kBatchKernelCount); k++) {
+            kernel_ms[k] = static_cast<float>(acc[k]);
+        }
+    }
+    (void)hipGetLastError();
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
     return rc;
+}
+
+extern "C" const char* msim_batch_kernel_name(uint32_t k) {
+    return k < static_cast<uint32_t>(kBatchKernelCount) ? kBatchKernels[k] : nullptr;
 }
 
 extern "C" int msim_batch_destroy(msim_batch* b) {

@@ -376,11 +376,160 @@ static __global__ __launch_bounds__(kBlock) void k_scatter(AgentsDev a, GridDev 
 // when agent i searches (Entity.initialized was 1 at dispatch start), which is
 // the reference's "j < i" rule at random_move.comp:574.  The last workgroup to
 // finish folds the dispatch's pair total into the counters and history.
-// UNIF enables the wave-uniform path below (heavy cells).
-template <bool SLOW, bool UNIF>
+// Slots within the d2 limit of `me` over the 3x3 cells around (cx, cy) of a
+// bucket table (the agent's own slot `self` included).  Latency first: the 12
+// row bounds are loaded together, then the first kNbBatch candidates of the
+// three main row ranges together; longer ranges and the ranges of rows that
+// wrap around the table's edge column continue in batches.  Loads past a
+// range's end read the own slot instead (always valid) and are not counted,
+// so every load is unconditional.
+constexpr uint32_t kNbBatch = 4;
+
+__device__ __forceinline__ uint32_t count_in_cells(float2 me, uint32_t self, int32_t cx, int32_t cy,
+                                                   const uint32_t* __restrict__ start, uint32_t lw,
+                                                   uint32_t wmask, uint32_t hmask,
+                                                   const float2* __restrict__ bpos, float T) {
+    uint32_t lo[3], hi[3], wlo[3], whi[3];
+    const uint32_t c0 = static_cast<uint32_t>(cx - 1) & wmask;
+    const uint32_t c2 = static_cast<uint32_t>(cx + 1) & wmask;
+    const bool wrap = c0 > c2;  // the 3 columns cross the table's edge
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        const uint32_t rowb = (static_cast<uint32_t>(cy + r - 1) & hmask) << lw;
+        lo[r] = start[rowb + c0];
+        hi[r] = start[rowb + (wrap ? wmask + 1 : c2 + 1)];
+        wlo[r] = start[rowb];
+        whi[r] = start[rowb + (wrap ? c2 + 1 : 0)];
+    }
+    uint32_t cnt = 0;
+    float2 q[3][kNbBatch];
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+#pragma unroll
+        for (uint32_t u = 0; u < kNbBatch; u++) {
+            const uint32_t k = lo[r] + u;
+            q[r][u] = bpos[k < hi[r] ? k : self];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+#pragma unroll
+        for (uint32_t u = 0; u < kNbBatch; u++) {
+            cnt += (lo[r] + u < hi[r] && in_range(q[r][u], me, T)) ? 1u : 0u;
+        }
+    }
+#pragma unroll 1
+    for (int g = 0; g < 6; g++) {
+        const int r = g >> 1;
+        uint32_t k = (g & 1) ? wlo[r] : lo[r] + kNbBatch;
+        const uint32_t h = (g & 1) ? whi[r] : hi[r];
+        for (; k < h; k += kNbBatch) {
+            float2 v[kNbBatch];
+#pragma unroll
+            for (uint32_t u = 0; u < kNbBatch; u++) {
+                v[u] = bpos[k + u < h ? k + u : self];
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kNbBatch; u++) {
+                cnt += (k + u < h && in_range(v[u], me, T)) ? 1u : 0u;
+            }
+        }
+    }
+    return cnt;
+}
+
+// Orders this wave's LDS writes before its later LDS reads (other lanes'
+// data), and its reads before the next writes; no other wave takes part.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr uint32_t kStage = 128;  // binned positions staged per wave round (16 B per lane)
+
+// count_in_cells for a whole wave whose valid lanes (a prefix, lane 0 valid,
+// the last one last_lane) hold consecutive slots of one bucket table, hence
+// nondecreasing buckets.  Each lane's main range in a row, its columns
+// clipped to the row ([col-1, col+1] within [0, wmask]), then starts and ends
+// no earlier than the previous lane's, so lane 0's start and the last lane's
+// end bound the wave's candidates in that row: they are loaded once per wave,
+// 128 slots per 16-byte-per-lane load, into LDS, and every lane scans its own
+// range there.  Rows where a lane's neighbour row wraps around the table's
+// top or bottom edge, and the wrapped column of lanes in the first or last
+// table column, are read per lane from global memory.  The agent's own slot
+// is included in the count.
+__device__ __forceinline__ uint32_t wave_count_in_cells(float2 me, bool valid, uint32_t last_lane, int32_t cx,
+                                                        int32_t cy, const uint32_t* __restrict__ start,
+                                                        uint32_t lw, uint32_t wmask, uint32_t hmask,
+                                                        const float2* __restrict__ bpos, float T,
+                                                        float2* stage) {
+    const uint32_t lane = __lane_id();
+    const uint32_t col = static_cast<uint32_t>(cx) & wmask;
+    const uint32_t row = static_cast<uint32_t>(cy) & hmask;
+    const uint32_t cl = col ? col - 1u : 0u;
+    const uint32_t ch = col < wmask ? col + 1u : wmask;
+    uint32_t lo[3], hi[3];
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        const uint32_t rowb = ((row + static_cast<uint32_t>(r - 1)) & hmask) << lw;
+        lo[r] = start[rowb + cl];
+        hi[r] = start[rowb + ch + 1u];
+    }
+    uint32_t cnt = 0;
+#pragma unroll 1
+    for (int r = 0; r < 3; r++) {
+        const bool edge = (r == 0 && row == 0u) || (r == 2 && row == hmask);
+        if (__ballot(valid && edge) != 0ull) {  // the wave's rows are not monotone here
+            if (valid) {
+                for (uint32_t k = lo[r]; k < hi[r]; k++) {
+                    cnt += in_range(bpos[k], me, T) ? 1u : 0u;
+                }
+            }
+            continue;
+        }
+        const uint32_t u_lo = __builtin_amdgcn_readfirstlane(lo[r]);
+        const uint32_t u_hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(hi[r]),
+                                                                              static_cast<int>(last_lane)));
+        for (uint32_t base = u_lo & ~1u; base < u_hi; base += kStage) {
+            const uint32_t i = base + 2u * lane;
+            if (i < u_hi) {
+                const float4 v = *reinterpret_cast<const float4*>(bpos + i);
+                stage[2u * lane] = make_float2(v.x, v.y);
+                stage[2u * lane + 1u] = make_float2(v.z, v.w);
+            }
+            wave_lds_sync();
+            if (valid) {
+                const uint32_t k1 = min(hi[r], base + kStage);
+                for (uint32_t k = max(lo[r], base); k < k1; k++) {
+                    cnt += in_range(stage[k - base], me, T) ? 1u : 0u;
+                }
+            }
+            wave_lds_sync();
+        }
+    }
+    if (valid && (col == 0u || col == wmask) && wmask != 0u) {  // the column across the table's edge
+        const uint32_t wc = col == 0u ? wmask : 0u;
+#pragma unroll 1
+        for (int r = 0; r < 3; r++) {
+            const uint32_t rowb = ((row + static_cast<uint32_t>(r - 1)) & hmask) << lw;
+            const uint32_t k1 = start[rowb + wc + 1u];
+            for (uint32_t k = start[rowb + wc]; k < k1; k++) {
+                cnt += in_range(bpos[k], me, T) ? 1u : 0u;
+            }
+        }
+    }
+    return cnt;
+}
+
+// Neighbour pass over the binned slots.  STAGED: wave_count_in_cells;
+// otherwise count_in_cells per lane.  SLOW: the initialisation dispatch, where
+// agents with Entity.initialized == 0 do not search (random_move.comp:863-867).
+template <bool SLOW, bool STAGED>
 static __global__ __launch_bounds__(kBlock) void k_collide(AgentsDev a, GridDev g, const float2* __restrict__ bpos,
                                                     unsigned long long* __restrict__ part_pairs) {
     __shared__ uint32_t lds[kBlock / 64];
+    __shared__ float2 stage_all[STAGED ? kBlock / 64 * kStage : 1];
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t s = blk * kBlock + threadIdx.x;
     const bool valid = s < a.n;
@@ -390,64 +539,24 @@ static __global__ __launch_bounds__(kBlock) void k_collide(AgentsDev a, GridDev 
     const float2 me = valid ? bpos[s] : make_float2(0.0f, 0.0f);
     const int32_t cx = cell_coord(me.x, g.inv_cell);
     const int32_t cy = cell_coord(me.y, g.inv_cell);
-    const uint32_t brow = static_cast<uint32_t>(cy) & g.hmask;
-    const uint32_t bcol = static_cast<uint32_t>(cx) & g.wmask;
-
-    // Wave-uniform path: every valid lane sits in one table row, in columns
-    // [c_lo, c_lo + 2] away from the row ends.  Then the union of the lanes'
-    // 3x3 neighbourhoods is 3 contiguous bucket ranges shared by the wave;
-    // each candidate is fetched once per wave (scalar load) and tested by all
-    // lanes.  A lane's own slot is among the candidates and always passes
-    // (distance 0), so it is subtracted afterwards.
-    bool uniform = false;
-    if (!SLOW && UNIF) {
-        const unsigned long long act = __ballot(valid);
-        const int first = act ? __ffsll(static_cast<long long>(act)) - 1 : 0;
-        const uint32_t row0 = __shfl(brow, first, 64);
-        uint32_t lo = valid ? bcol : 0xFFFFFFFFu;
-        uint32_t hi = valid ? bcol : 0u;
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            lo = min(lo, static_cast<uint32_t>(__shfl_xor(static_cast<int>(lo), o, 64)));
-            hi = max(hi, static_cast<uint32_t>(__shfl_xor(static_cast<int>(hi), o, 64)));
-        }
-        const bool same_row = __ballot(valid && brow != row0) == 0ull;
-        uniform = act != 0ull && same_row && lo >= 1u && hi + 2u <= g.wmask + 1u && hi - lo <= 2u;
-        if (uniform) {
-            const uint32_t r0 = __builtin_amdgcn_readfirstlane(row0);
-            const uint32_t c_lo = __builtin_amdgcn_readfirstlane(lo) - 1u;
-            const uint32_t c_hi = __builtin_amdgcn_readfirstlane(hi) + 2u;  // exclusive
-            const uint32_t lane = __lane_id();
-#pragma unroll 1
-            for (int dy = -1; dy <= 1; dy++) {
-                const uint32_t rowb = ((r0 + static_cast<uint32_t>(dy)) & g.hmask) << g.lw;
-                const uint32_t k0 = __builtin_amdgcn_readfirstlane(g.start[rowb + c_lo]);
-                const uint32_t k1 = __builtin_amdgcn_readfirstlane(g.start[rowb + c_hi]);
-                // 64 candidates per coalesced load, broadcast lane by lane
-                for (uint32_t base = k0; base < k1; base += 64u) {
-                    const uint32_t kk = base + lane;
-                    const float2 qv = kk < k1 ? bpos[kk] : make_float2(0.0f, 0.0f);
-                    const uint32_t m = min(64u, k1 - base);
-                    const int qx = __float_as_int(qv.x);
-                    const int qy = __float_as_int(qv.y);
-                    for (uint32_t j = 0; j < m; j++) {
-                        const float2 q = make_float2(__int_as_float(__builtin_amdgcn_readlane(qx, j)),
-                                                     __int_as_float(__builtin_amdgcn_readlane(qy, j)));
-                        cnt += in_range(q, me, T) ? 1u : 0u;
-                    }
-                }
-            }
-            cnt = valid ? cnt - 1u : 0u;  // the lane's own slot
+    if (!SLOW && STAGED) {
+        const uint32_t w0 = s - __lane_id();  // the wave's first slot
+        if (w0 < a.n) {
+            const uint32_t last_lane = min(63u, a.n - 1u - w0);
+            cnt = wave_count_in_cells(me, valid, last_lane, cx, cy, g.start, g.lw, g.wmask, g.hmask, bpos, T,
+                                      stage_all + (threadIdx.x >> 6) * kStage);
         }
     }
-    if (!uniform && valid) {
+    if (!SLOW && !STAGED && valid) {
+        cnt = count_in_cells(me, s, cx, cy, g.start, g.lw, g.wmask, g.hmask, bpos, T);
+    }
+    if (SLOW && valid) {
+        // initialisation dispatch: a pair counts once, found from the end
+        // with the larger caller index if that agent searches, else from the
+        // other end if it searches
         const uint32_t me_id = g.bid[s];
-        uint32_t me_ext = 0;
-        bool me_search = true;
-        if (SLOW) {
-            me_ext = a.perm[me_id];
-            me_search = a.init[me_id] != 0u;
-        }
+        const uint32_t me_ext = a.perm[me_id];
+        const bool me_search = a.init[me_id] != 0u;
 #pragma unroll 1
         for (int dy = -1; dy <= 1; dy++) {
             const uint32_t rowb = (static_cast<uint32_t>(cy + dy) & g.hmask) << g.lw;
@@ -468,19 +577,6 @@ static __global__ __launch_bounds__(kBlock) void k_collide(AgentsDev a, GridDev 
             for (int seg = 0; seg < 2; seg++) {
                 const uint32_t lo = seg ? r2 : r0;
                 const uint32_t hi = seg ? r3 : r1;
-                if (!SLOW) {
-                    // four independent candidate loads in flight per step
-                    uint32_t k = lo;
-                    for (; k + 4u <= hi; k += 4u) {
-                        const float2 q0 = bpos[k], q1 = bpos[k + 1], q2 = bpos[k + 2], q3 = bpos[k + 3];
-                        cnt += (in_range(q0, me, T) ? 1u : 0u) + (in_range(q1, me, T) ? 1u : 0u) +
-                               (in_range(q2, me, T) ? 1u : 0u) + (in_range(q3, me, T) ? 1u : 0u);
-                    }
-                    for (; k < hi; k++) {
-                        cnt += in_range(bpos[k], me, T) ? 1u : 0u;
-                    }
-                    continue;
-                }
                 for (uint32_t k = lo; k < hi; k++) {
                     const float2 q = bpos[k];
                     if (in_range(q, me, T) && k != s) {
@@ -496,7 +592,7 @@ static __global__ __launch_bounds__(kBlock) void k_collide(AgentsDev a, GridDev 
             }
         }
     }
-    if (!SLOW && !uniform && valid) {
+    if (!SLOW && valid) {
         cnt -= 1u;  // the agent's own slot passed the predicate
     }
     if (valid) {
@@ -575,7 +671,7 @@ static __global__ __launch_bounds__(64) void k_collide_finish(unsigned long long
 // Gathers every per-agent array into bucket order so that agents that are
 // close in space are close in memory; afterwards bid is the identity.
 static __global__ __launch_bounds__(kBlock) void k_reorder(AgentsDev src, AgentsDev dst, GridDev g) {
-    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (s >= src.n) {
         return;
     }

@@ -63,7 +63,7 @@ struct msim_sim {
     bool stats_pending = false;
     int binning = 0;  // 0: segment-owned (k_bin_count + k_bin_place), 1: atomic counting
     bool route_inline = false;  // arrivals routed inside k_advance instead of k_route
-    bool collide_uniform = false;  // wave-uniform neighbour path for waves in one bucket range
+    bool collide_per_lane = false;  // neighbour pass without wave staging
     // msim_time_ticks probe: launch number probe_kernel of the tick is
     // bracketed by probe_ev[0] / probe_ev[1]; -1 = no probe.
     int probe_kernel = -1;
@@ -395,13 +395,13 @@ int enqueue_collide(msim_sim* s, bool slow, hipEvent_t* ev = nullptr) {
                            s->hist, s->hist_cap - 1, s->part_stats, fold);
         hipLaunchKernelGGL(k_set_init, g, dim3(kBlock), 0, s->stream, a.init, s->n);
     } else {
-        if (s->collide_uniform) {
+        if (s->collide_per_lane) {
             probe_pre(s);
-            hipLaunchKernelGGL((k_collide<false, true>), g, dim3(kBlock), 0, s->stream, a, s->grid_cur(),
+            hipLaunchKernelGGL((k_collide<false, false>), g, dim3(kBlock), 0, s->stream, a, s->grid_cur(),
                                static_cast<const float2*>(s->bbpos[s->bcur]), s->part_pairs);
         } else {
             probe_pre(s);
-            hipLaunchKernelGGL((k_collide<false, false>), g, dim3(kBlock), 0, s->stream, a, s->grid_cur(),
+            hipLaunchKernelGGL((k_collide<false, true>), g, dim3(kBlock), 0, s->stream, a, s->grid_cur(),
                                static_cast<const float2*>(s->bbpos[s->bcur]), s->part_pairs);
         }
         probe_post(s);
@@ -680,7 +680,7 @@ extern "C" int msim_create(const msim_road* roads, uint32_t n_roads, const uint3
     g.hmask = (1u << (tlog - g.lw)) - 1u;
     s->binning = static_cast<int>(o.binning);
     s->route_inline = o.route_queue == 0;
-    s->collide_uniform = o.collide_wave_uniform != 0;
+    s->collide_per_lane = o.collide_mode != 0;
     s->seg_log = seg_use;
     s->n_seg = g.m >> seg_use;
     // An agent moves at most max_speed per axis per tick (the heading has
@@ -715,7 +715,7 @@ extern "C" int msim_create(const msim_road* roads, uint32_t n_roads, const uint3
     ALLOC(g.rank, n);
     for (int k = 0; k < 2; k++) {
         ALLOC(s->bstart[k], g.m + 1);
-        ALLOC(s->bbpos[k], n);
+        ALLOC(s->bbpos[k], n + 2);  // 16-byte staged reads may touch one slot past n
         ALLOC(s->bbid[k], n);
     }
     ALLOC(s->seg_total, s->n_seg);
@@ -780,6 +780,13 @@ extern "C" int msim_create(const msim_road* roads, uint32_t n_roads, const uint3
         for (int k = 0; k < 2 && e == hipSuccess; k++) e = hipMemset(s->bstart[k], 0, sizeof(uint32_t) * (g.m + 1));
         if (e == hipSuccess) e = hipMemset(s->hist, 0, sizeof(uint32_t) * s->hist_cap);
         if (e != hipSuccess) rc = fail(s, MSIM_ERR_HIP, "hipMemset failed: %s", hipGetErrorString(e));
+    }
+    // The uploads and fills above run on the null stream, which the handle's
+    // non-blocking stream does not wait for: drain the device before the
+    // first launch reads them.
+    if (rc == MSIM_OK) {
+        hipError_t e = hipStreamSynchronize(nullptr);
+        if (e != hipSuccess) rc = fail(s, MSIM_ERR_HIP, "upload failed: %s", hipGetErrorString(e));
     }
     // Initial binning, then renumber agents into bucket order for locality.
     if (rc == MSIM_OK) rc = enqueue_move(s, kBinOnly, false);

@@ -39,7 +39,7 @@ class Options(ctypes.Structure):
         ("segment_log2", ctypes.c_uint32),
         ("binning", ctypes.c_uint32),
         ("route_queue", ctypes.c_uint32),
-        ("collide_wave_uniform", ctypes.c_uint32),
+        ("collide_mode", ctypes.c_uint32),
     ]
 
 
@@ -89,7 +89,8 @@ SIGNATURES = {
     "msim_batch_sync": (ctypes.c_int, [_VP]),
     "msim_batch_read_state": (ctypes.c_int, [_VP, _U32, _VP, _VP, _VP, _VP, _VP, _VP]),
     "msim_batch_read_pairs": (ctypes.c_int, [_VP, _VP, _VP]),
-    "msim_batch_time_ticks": (ctypes.c_int, [_VP, _U32, _VP]),
+    "msim_batch_time_ticks": (ctypes.c_int, [_VP, _U32, _VP, _VP, _U32]),
+    "msim_batch_kernel_name": (ctypes.c_char_p, [_U32]),
     "msim_batch_destroy": (ctypes.c_int, [_VP]),
     "msim_batch_last_error": (ctypes.c_char_p, [_VP]),
 }
@@ -352,8 +353,19 @@ class Batch:
         self._check(self.lib.msim_batch_read_pairs(self.h, _ptr(tot), _ptr(last)))
         return tot, last
 
-    def time_ticks(self, n):
-        """Milliseconds for n ticks of every scenario (HIP events)."""
+    def kernel_names(self):
+        out = []
+        while True:
+            nm = self.lib.msim_batch_kernel_name(len(out))
+            if not nm:
+                return out
+            out.append(nm.decode())
+
+    def time_ticks(self, n, kernels=False):
+        """(ms for n ticks of every scenario, {kernel: summed ms over a second
+        pass of n ticks with events around every launch} or {})."""
         ms = np.zeros(1, np.float32)
-        self._check(self.lib.msim_batch_time_ticks(self.h, n, _ptr(ms)))
-        return float(ms[0])
+        names = self.kernel_names() if kernels else []
+        k = np.zeros(max(1, len(names)), np.float32)
+        self._check(self.lib.msim_batch_time_ticks(self.h, n, _ptr(ms), _ptr(k) if kernels else None, len(names)))
+        return float(ms[0]), dict(zip(names, (float(v) for v in k)))

@@ -28,10 +28,10 @@ SPACE = {
     "binning": [0, 1],
     "route_queue": [0, 1],
     "reorder_interval": [8, 16, 32, 64],
-    "collide_wave_uniform": [0, 1],
+    "collide_mode": [0, 1],
 }
 DEFAULT = {"table_width_log2": 0, "segment_log2": 0, "binning": 0, "route_queue": 0,
-           "reorder_interval": 16, "collide_wave_uniform": 0}
+           "reorder_interval": 16, "collide_mode": 0}
 
 
 def workload_key(sc):

@@ -66,8 +66,9 @@ def _spanning_tree(n, edges, weights):
     w = np.asarray(weights, np.float64) + 1e-9  # zero weights would vanish from the sparse matrix
     m = coo_matrix((w, (edges[:, 0], edges[:, 1])), shape=(n, n)).tocsr()
     t = minimum_spanning_tree(m).tocoo()
-    key = {(int(a), int(b)) for a, b in zip(t.row, t.col)}
-    return np.array([(int(a), int(b)) in key or (int(b), int(a)) in key for a, b in edges], bool)
+    r, c = t.row.astype(np.int64), t.col.astype(np.int64)
+    key = np.concatenate([r * n + c, c * n + r])
+    return np.isin(edges[:, 0].astype(np.int64) * n + edges[:, 1].astype(np.int64), key)
 
 
 def lattice_graph(side=32, size=1000.0, jitter=0.3, drop=0.15, seed=0):

@@ -112,8 +112,10 @@ def test_batch_repeated_runs_and_time_ticks():
     b = msim.Batch(scs)
     for k in (1, 5, 100, 2, 133):
         a.run_ticks(k)
-    ms = b.time_ticks(241)
+    ms, _ = b.time_ticks(141)
     assert ms > 0.0
+    ms, kms = b.time_ticks(50, kernels=True)
+    assert ms > 0.0 and set(kms) == set(b.kernel_names()) and all(v > 0.0 for v in kms.values())
     for s in range(len(scs)):
         assert_state_equal(a.state(s), b.state(s), f"scenario {s}")
     np.testing.assert_array_equal(a.pairs()[0], b.pairs()[0])

@@ -95,8 +95,8 @@ def test_dispatch_loop_equals_fused_graph_path():
                                   dict(binning=1), dict(binning=1, reorder_interval=5),
                                   dict(segment_log2=9, table_width_log2=3),
                                   dict(table_log2=16, segment_log2=13, table_width_log2=8),
-                                  dict(route_queue=1), dict(collide_wave_uniform=1),
-                                  dict(route_queue=1, binning=1, collide_wave_uniform=1)])
+                                  dict(route_queue=1), dict(collide_mode=1),
+                                  dict(route_queue=1, binning=1, collide_mode=1)])
 def test_options_do_not_change_results(opts):
     sc = S.config(1, scale=0.5)
     g, o = pair(sc, **opts)
