@@ -185,7 +185,8 @@ def batched_line(args, scs):
     import msim
 
     n = sum(int(sc.entities.shape[0]) for sc in scs)
-    b = msim.Batch(scs, reorder_interval=args.batch_reorder, graph_ticks=args.batch_graph_ticks)
+    b = msim.Batch(scs, reorder_interval=args.batch_reorder, graph_ticks=args.batch_graph_ticks,
+                   segment_log2=args.batch_segment_log2)
     extra = {}
     if args.compare_ticks > 0:
         extra["batched_vs_plain"] = compare_plain(args, scs, b, n)
@@ -209,6 +210,7 @@ def batched_line(args, scs):
         "collision_radius_m": float(scs[0].pc["collisionRadius"]),
         "graph_segment_ticks": args.batch_graph_ticks,
         "reorder_interval": args.batch_reorder,
+        "segment_log2": args.batch_segment_log2,
     }
     extra["fused_tick"] = fused(el / args.steps, n, per_launch_ms)
     return n * args.steps / el, 1e3 * el / args.steps, config, roof, extra
@@ -225,6 +227,7 @@ def main():
                                                                     "comparison")
     ap.add_argument("--batch-reorder", type=int, default=20)
     ap.add_argument("--batch-graph-ticks", type=int, default=100)
+    ap.add_argument("--batch-segment-log2", type=int, default=0, help="0: per-scenario default")
     ap.add_argument("--profile-ticks", type=int, default=240)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")

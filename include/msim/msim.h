@@ -50,7 +50,8 @@ typedef struct msim_options {
     int32_t device;            /* -1 (or 0 with struct_size 0): current device */
     uint32_t watchdog_ms;      /* 0: 600000 ms host-side wait limit            */
     uint32_t table_width_log2; /* 0: auto; buckets per table row = 2^this      */
-    uint32_t segment_log2;     /* 0: auto; buckets per rebinning workgroup     */
+    uint32_t segment_log2;     /* 0: auto; buckets per rebinning workgroup
+                                  (batched: for every scenario, 9..its table) */
     uint32_t binning;          /* 0: segment-owned rebinning (LDS histograms,
                                   per-segment offsets); 1: global-atomic counting */
     uint32_t route_queue;      /* 0: arrivals routed inside the advance kernel;

@@ -373,9 +373,9 @@ int benqueue_collide(msim_batch* b) {
 
 template <bool U, bool W>
 void badvance(msim_batch* b) {
-    const QueueDev q{b->qdummy, b->qdummy, b->qdummy, 0};
-    hipLaunchKernelGGL((k_advance<U, W, false, true>), dim3(bblocks(b->n)), dim3(kBlock), 0, b->stream,
-                       b->agents(b->cur), bgrid(b), q, b->part_stats, b->roads, b->conns);
+    const dim3 nb(std::max(1u, (b->n + kAdv * kBlock - 1) / (kAdv * kBlock)));
+    hipLaunchKernelGGL((k_advance4<U, W>), nb, dim3(kBlock), 0, b->stream, b->agents(b->cur), b->inv_cell,
+                       b->part_stats, b->roads, b->conns);
 }
 
 int benqueue_move(msim_batch* b, bool write_dir) {
@@ -611,6 +611,7 @@ extern "C" int msim_batch_create(uint32_t n_scen, const uint32_t* road_off, cons
                                                    entities + agent_off[s], d.n, cell);
         d.lw = auto_width_log2(tlog, rows_log);
         uint32_t seg_log = auto_segment_log2(tlog, d.lw);
+        if (o.segment_log2) seg_log = std::min(std::max(o.segment_log2, 9u), tlog);
         while (seg_log > 9 && (static_cast<size_t>(4) << seg_log) + 1024 > static_cast<size_t>(lds_max)) seg_log--;
         d.seg_log = seg_log;
         d.m = 1u << tlog;
@@ -628,16 +629,17 @@ extern "C" int msim_batch_create(uint32_t n_scen, const uint32_t* road_off, cons
     b->start_len = start_len;
 
     int rc = MSIM_OK;
+    const size_t n_adv = (static_cast<size_t>(n) + kAdv - 1) / kAdv * kAdv;
 #define BALLOC(p, cnt) \
     if (rc == MSIM_OK) rc = balloc_n(b, &(p), (cnt));
     BALLOC(b->roads, n_roads);
     BALLOC(b->conns, n_conns);
     for (int k = 0; k < 2; k++) {
         BBuffers& bb = b->buf[k];
-        BALLOC(bb.pos, n);
-        BALLOC(bb.tgt, n);
-        BALLOC(bb.dir, n);
-        if (!b->uniform) BALLOC(bb.spd, n);
+        BALLOC(bb.pos, n_adv);  // k_advance4 moves kAdv agents per thread
+        BALLOC(bb.tgt, n_adv);
+        BALLOC(bb.dir, n_adv);
+        if (!b->uniform) BALLOC(bb.spd, n_adv);
         BALLOC(bb.road, n);
         BALLOC(bb.rng, n);
         BALLOC(bb.init, n);

@@ -128,6 +128,130 @@ static __global__ __launch_bounds__(kBlock) void k_advance(AgentsDev a, GridDev 
     }
 }
 
+// Streaming advance with kAdv agents per thread: 16-byte loads and stores of
+// the SoA arrays (the kernels are bound by vector-memory instruction issue,
+// not bytes), arrivals compacted per wave in LDS and routed with the wave's
+// lanes in parallel.  Per agent the arithmetic is k_advance's exactly.  The
+// per-agent arrays are padded to a multiple of kAdv agents; the padding is
+// computed on but never routed, counted or read back.
+constexpr uint32_t kAdv = 4;
+
+template <bool UNIFORM, bool WRITE_DIR>
+static __global__ __launch_bounds__(kBlock) void k_advance4(AgentsDev a, float inv_cell,
+                                                     unsigned long long* __restrict__ part_stats,
+                                                     const msim_road* __restrict__ roads,
+                                                     const uint32_t* __restrict__ conns) {
+    __shared__ uint32_t arr_id[kBlock / 64][64 * kAdv];
+    __shared__ float2 arr_dir[WRITE_DIR ? kBlock / 64 : 1][WRITE_DIR ? 64 * kAdv : 1];
+    __shared__ uint32_t lds[2 * (kBlock / 64)];
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t t = blk * kBlock + threadIdx.x;
+    const uint32_t w = threadIdx.x >> 6;
+    const uint32_t i0 = t * kAdv;
+    const bool any = i0 < a.n;
+    float2 np[kAdv], d[kAdv];
+    bool arr[kAdv];
+    uint32_t n_ins = 0;
+#pragma unroll
+    for (uint32_t u = 0; u < kAdv; u++) {
+        arr[u] = false;
+        d[u] = make_float2(0.0f, 0.0f);
+        np[u] = d[u];
+    }
+    if (any) {
+        const float4* P = reinterpret_cast<const float4*>(a.pos) + 2 * t;
+        const float4* G = reinterpret_cast<const float4*>(a.tgt) + 2 * t;
+        const float4 p01 = P[0], p23 = P[1], t01 = G[0], t23 = G[1];
+        const float4 sv = UNIFORM ? make_float4(a.spd_const, a.spd_const, a.spd_const, a.spd_const)
+                                  : reinterpret_cast<const float4*>(a.spd)[t];
+        const float2 p[kAdv] = {make_float2(p01.x, p01.y), make_float2(p01.z, p01.w), make_float2(p23.x, p23.y),
+                                make_float2(p23.z, p23.w)};
+        const float2 tg[kAdv] = {make_float2(t01.x, t01.y), make_float2(t01.z, t01.w),
+                                 make_float2(t23.x, t23.y), make_float2(t23.z, t23.w)};
+        const float sp[kAdv] = {sv.x, sv.y, sv.z, sv.w};
+#pragma unroll
+        for (uint32_t u = 0; u < kAdv; u++) {
+            const bool valid = i0 + u < a.n;
+            const float dx = tg[u].x - p[u].x;
+            const float dy = tg[u].y - p[u].y;
+            const float len = vlen(dx, dy);
+            if (len > sp[u]) {
+                d[u] = make_float2((dx / len) * sp[u], (dy / len) * sp[u]);
+                np[u] = make_float2(p[u].x + d[u].x, p[u].y + d[u].y);
+            } else {
+                np[u] = tg[u];
+                arr[u] = valid;
+            }
+            n_ins += (valid && (cell_coord(np[u].x, inv_cell) != cell_coord(p[u].x, inv_cell) ||
+                                cell_coord(np[u].y, inv_cell) != cell_coord(p[u].y, inv_cell)))
+                         ? 1u
+                         : 0u;
+        }
+    }
+    // the wave's arrivals, in (u, lane) order
+    const unsigned long long lt = (1ull << __lane_id()) - 1ull;
+    uint32_t n_arr = 0;
+    uint32_t slot[kAdv];
+#pragma unroll
+    for (uint32_t u = 0; u < kAdv; u++) {
+        const unsigned long long m = __ballot(arr[u]);
+        slot[u] = n_arr + static_cast<uint32_t>(__popcll(m & lt));
+        n_arr += static_cast<uint32_t>(__popcll(m));
+        if (arr[u]) {
+            arr_id[w][slot[u]] = i0 + u;
+        }
+    }
+    wave_lds_sync();
+    for (uint32_t j = __lane_id(); j < n_arr; j += 64) {
+        const uint32_t i = arr_id[w][j];
+        float2 nt = a.tgt[i];  // the old target, where the agent now stands
+        const float2 at = nt;
+        uint32_t r = a.road[i];
+        uint4 st = a.rng[i];
+        choose_route(nt, r, st, roads, conns);
+        a.tgt[i] = nt;
+        a.road[i] = r;
+        a.rng[i] = st;
+        if (WRITE_DIR) {
+            arr_dir[w][j] = heading(nt, at, UNIFORM ? a.spd_const : a.spd[i]);
+        }
+    }
+    if (any) {
+        if (WRITE_DIR) {
+            wave_lds_sync();
+#pragma unroll
+            for (uint32_t u = 0; u < kAdv; u++) {
+                if (arr[u]) {
+                    d[u] = arr_dir[w][slot[u]];
+                }
+            }
+            float4* D = reinterpret_cast<float4*>(a.dir) + 2 * t;
+            D[0] = make_float4(d[0].x, d[0].y, d[1].x, d[1].y);
+            D[1] = make_float4(d[2].x, d[2].y, d[3].x, d[3].y);
+        }
+        float4* P = reinterpret_cast<float4*>(a.pos) + 2 * t;
+        P[0] = make_float4(np[0].x, np[0].y, np[1].x, np[1].y);
+        P[1] = make_float4(np[2].x, np[2].y, np[3].x, np[3].y);
+    }
+    // block totals of insertions and arrivals, one slot update each
+    const uint32_t wi = wave_sum(n_ins);
+    if (__lane_id() == 0) {
+        lds[w] = wi;
+        lds[kBlock / 64 + w] = n_arr;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t si = 0, sa = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < kBlock / 64; k++) {
+            si += lds[k];
+            sa += lds[kBlock / 64 + k];
+        }
+        slot_add(part_stats, blk, kWordInserts, si);
+        slot_add(part_stats, blk, kWordArrivals, sa);
+    }
+}
+
 // Route choice for the queued arrivals; grid = kQueues * blocks_per_queue.
 template <bool UNIFORM, bool WRITE_DIR>
 static __global__ __launch_bounds__(kBlock) void k_route(AgentsDev a, QueueDev q, const msim_road* __restrict__ roads,

@@ -249,7 +249,11 @@ void launch_advance_ri(msim_sim* s, const QueueDev& q) {
 
 template <bool U, bool W>
 void launch_advance_route(msim_sim* s, const QueueDev& q) {
-    if (s->route_inline) {
+    if (s->route_inline && !s->binning) {
+        const dim3 nb(std::max(1u, (s->n + kAdv * kBlock - 1) / (kAdv * kBlock)));
+        hipLaunchKernelGGL((k_advance4<U, W>), nb, dim3(kBlock), 0, s->stream, s->agents(s->cur),
+                           s->grid.inv_cell, s->part_stats, s->roads, s->conns);
+    } else if (s->route_inline) {
         launch_advance_ri<U, W, true>(s, q);
     } else {
         launch_advance_ri<U, W, false>(s, q);
@@ -694,16 +698,17 @@ extern "C" int msim_create(const msim_road* roads, uint32_t n_roads, const uint3
     const uint32_t D = static_cast<uint32_t>(std::floor(smax * (1.0f + 1.0f / 1048576.0f) / cell)) + 1u;
     s->margin = (D + 1u) * (g.wmask + 1u) + D;
 
+    const size_t n_adv = (static_cast<size_t>(n) + kAdv - 1) / kAdv * kAdv;
 #define ALLOC(p, cnt)                                   \
     if (rc == MSIM_OK) rc = dalloc_n(s, &(p), (cnt));
     ALLOC(s->roads, n_roads);
     ALLOC(s->conns, n_connections);
     for (int k = 0; k < 2; k++) {
         Buffers& b = s->buf[k];
-        ALLOC(b.pos, n);
-        ALLOC(b.tgt, n);
-        ALLOC(b.dir, n);
-        if (!s->uniform) ALLOC(b.spd, n);
+        ALLOC(b.pos, n_adv);  // k_advance4 moves kAdv agents per thread
+        ALLOC(b.tgt, n_adv);
+        ALLOC(b.dir, n_adv);
+        if (!s->uniform) ALLOC(b.spd, n_adv);
         ALLOC(b.road, n);
         ALLOC(b.rng, n);
         ALLOC(b.init, n);
