@@ -220,6 +220,39 @@ const char* msim_batch_kernel_name(uint32_t k);
 int msim_batch_destroy(msim_batch* b);
 const char* msim_batch_last_error(const msim_batch* b);
 
+/* ---- differentiable tick on a batched handle ------------------------------
+ * No reference counterpart.  Per tick: speed v_i * theta[road_i], the plain
+ * move (arrivals snap and take the plain route choice), then a soft-avoidance
+ * push p += alpha * sum_j (1 - |x_ij|^2/R^2)^3 x_ij over the neighbours of the
+ * collision dispatch (x_ij = p_i - p_j).  theta == 1, alpha == 0 is the plain
+ * tick.  Every run starts from the handle's state at setup; after setup the
+ * handle runs differentiable ticks only (msim_batch_run_ticks refuses), and
+ * msim_batch_read_state shows the state after the last simulated tick.
+ * Pointers marked (device) are device memory on the handle's device; calls
+ * block until their work is done. */
+
+/* Allocates checkpoints (every checkpoint_every ticks) and a segment tape for
+ * runs of n_ticks ticks; keep_schedule != 0 also keeps every tick's decisions
+ * (targets, roads, arrival flags) for replays and readback. */
+int msim_batch_diff_setup(msim_batch* b, uint32_t n_ticks, uint32_t checkpoint_every, int keep_schedule);
+
+/* n_ticks differentiable ticks with theta (device, one multiplier per packed
+ * road) and alpha.  replay != 0 reuses the decisions recorded by the last
+ * forward run instead of making them (frozen route choices; finite
+ * differences of the adjoint's function).  final_pos (device, nullable)
+ * receives the end positions, 2 floats per agent in packed caller order. */
+int msim_batch_diff_forward(msim_batch* b, const float* theta, float alpha, int replay, float* final_pos);
+
+/* Adjoint of the last (non-replay) forward run: grad_pos (device, 2 floats
+ * per agent, caller order) is dL/d(final positions); grad_theta (device, one
+ * double per packed road) and grad_alpha (device, 1 double) receive dL/dtheta
+ * and dL/dalpha with the run's decisions and neighbour cells frozen. */
+int msim_batch_diff_backward(msim_batch* b, const float* grad_pos, double* grad_theta, double* grad_alpha);
+
+/* One tick's recorded decisions (host, caller order, packed road indices):
+ * target (2 floats per agent), road, flag (bit 0 arrived, bit 1 no move). */
+int msim_batch_diff_read_schedule(msim_batch* b, uint32_t tick, float* target, uint32_t* road, uint8_t* flag);
+
 const char* msim_last_error(const msim_sim* sim);
 const char* msim_get_last_error(void);
 

@@ -10,366 +10,18 @@
 // the segment (rebinning) or by binary search over agent_off (neighbours).
 // A tick is 5 launches for all scenarios; ticks run as captured hipGraph
 // segments.  Per-scenario results are bit-identical to plain runs.
-#include <hip/hip_runtime.h>
+#include "msim_batch.hpp"
 
-#include <algorithm>
-#include <chrono>
 #include <cmath>
-#include <cstdarg>
-#include <cstdio>
 #include <cstring>
-#include <string>
-#include <thread>
-#include <vector>
-
-#include "msim/msim.h"
-#include "msim_bin.hpp"
-#include "msim_host.hpp"
-#include "msim_kernels.hpp"
 
 using namespace msim;
 
 namespace msim {
-
-struct ScenDev {
-    uint32_t agent_off, n;
-    uint32_t start_base;  // offset of this scenario's start[m + 1] in the start arrays
-    uint32_t lw, wmask, hmask, m;
-    uint32_t seg_log, seg_first, n_seg, margin;
-    uint32_t pad;
-};
-
-constexpr uint32_t kScenSlots = 16;     // pair partial slots per scenario
-constexpr uint32_t kScenSlotStride = 8;  // 64-bit words between slots (64 B)
-
-struct BatchBin {
-    const ScenDev* scen;
-    const uint32_t* seg_scen;  // global segment -> scenario
-    const uint32_t* old_start;
-    const uint32_t* old_bid;
-    uint32_t* new_start;
-    float2* new_bpos;
-    uint32_t* new_bid;
-    uint32_t* seg_total;
-    unsigned int* faults;
-    float inv_cell;
-};
-
-__device__ __forceinline__ TableView scen_view(const ScenDev& sc, const BatchBin& b) {
-    return TableView{b.old_start + sc.start_base, b.old_bid, b.inv_cell, sc.lw, sc.wmask, sc.hmask, sc.m,
-                     sc.seg_log, sc.margin};
-}
-
-__global__ __launch_bounds__(kBinThreads) void k_bbin_count(const float2* __restrict__ pos, BatchBin b) {
-    extern __shared__ uint32_t hist[];
-    __shared__ uint32_t lds[kBinThreads / 64];
-    const uint32_t g = blockIdx.x;
-    const ScenDev sc = b.scen[b.seg_scen[g]];
-    const uint32_t s0 = (g - sc.seg_first) << sc.seg_log;
-    const uint32_t total = bin_count_segment(pos, scen_view(sc, b), s0, hist, lds, b.new_start + sc.start_base);
-    if (threadIdx.x == 0) {
-        b.seg_total[g] = total;
-    }
-}
-
-__global__ __launch_bounds__(kBinThreads) void k_bbin_place(const float2* __restrict__ pos, BatchBin b) {
-    extern __shared__ uint32_t cursor[];
-    __shared__ uint32_t lds[kBinThreads / 64];
-    const uint32_t g = blockIdx.x;
-    const ScenDev sc = b.scen[b.seg_scen[g]];
-    uint32_t before = 0;
-    for (uint32_t k = sc.seg_first + threadIdx.x; k < g; k += kBinThreads) {
-        before += b.seg_total[k];
-    }
-    const uint32_t base = sc.agent_off + block_sum<kBinThreads>(before, lds);
-    if (g == sc.seg_first + sc.n_seg - 1 && threadIdx.x == 0) {
-        const uint32_t end = base + b.seg_total[g];
-        b.new_start[sc.start_base + sc.m] = end;
-        if (end != sc.agent_off + sc.n) {
-            atomicOr(b.faults, kFaultBinCount);
-        }
-    }
-    bin_place_segment(pos, scen_view(sc, b), (g - sc.seg_first) << sc.seg_log, base, cursor,
-                      b.new_start + sc.start_base, b.new_bpos, b.new_bid);
-}
-
-// Table parameters of the scenario holding a block's first slot, and the
-// slot where that scenario ends (read with scalar loads: block-uniform).
-struct BlockScen {
-    uint32_t sid, end, start_base, lw, wmask, hmask, pad0, pad1;
-};
-
-struct BatchCollide {
-    const BlockScen* blk;
-    const ScenDev* scen;
-    const uint32_t* agent_off;
-    uint32_t n;
-    const uint32_t* start;
-    const float2* bpos;
-    const uint32_t* bid;
-    uint32_t* ncount;
-    float inv_cell;
-    float d2_limit;
-    unsigned long long* part;  // [n_scen][kScenSlots * kScenSlotStride]
-};
-
-// Neighbour pass over every binned slot of every scenario, each slot on its
-// own scenario's bucket table (count_in_cells, as in the plain pass).
-template <bool STAGED>
-__global__ __launch_bounds__(kBlock) void k_bcollide(BatchCollide c) {
-    __shared__ float2 stage_all[STAGED ? kBlock / 64 * kStage : 1];
-    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
-    const uint32_t s = blk * kBlock + threadIdx.x;
-    const bool valid = s < c.n;
-    const BlockScen bs = c.blk[blk];
-    uint32_t cnt = 0;
-    uint32_t sid = bs.sid;
-    const float2 me = valid ? c.bpos[s] : make_float2(0.0f, 0.0f);
-    const int32_t cx = cell_coord(me.x, c.inv_cell);
-    const int32_t cy = cell_coord(me.y, c.inv_cell);
-    if (valid && s >= bs.end) {  // a later scenario starts inside this block
-        sid++;
-        while (s >= c.agent_off[sid + 1]) sid++;
-    }
-    const uint32_t w0 = s - __lane_id();
-    bool done = false;
-    if (STAGED && w0 < c.n) {
-        const uint32_t sid0 = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(sid)));
-        if (__ballot(valid && sid != sid0) == 0ull) {  // the wave lies in one scenario
-            uint32_t base = bs.start_base, lw = bs.lw, wmask = bs.wmask, hmask = bs.hmask;
-            if (sid0 != bs.sid) {
-                const ScenDev& sc = c.scen[sid0];
-                base = sc.start_base;
-                lw = sc.lw;
-                wmask = sc.wmask;
-                hmask = sc.hmask;
-            }
-            cnt = wave_count_in_cells(me, valid, min(63u, c.n - 1u - w0), cx, cy, c.start + base, lw, wmask,
-                                      hmask, c.bpos, c.d2_limit, stage_all + (threadIdx.x >> 6) * kStage);
-            done = true;
-        }
-    }
-    if (valid && !done) {
-        const ScenDev& sc = c.scen[sid];
-        cnt = count_in_cells(me, s, cx, cy, c.start + sc.start_base, sc.lw, sc.wmask, sc.hmask, c.bpos,
-                             c.d2_limit);
-    }
-    if (valid) {
-        cnt -= 1u;  // own slot
-        c.ncount[c.bid[s]] = cnt;
-    }
-    // per-scenario partial sums: one atomic per wave when the wave is one
-    // scenario, per lane at scenario boundaries
-    const unsigned long long act = __ballot(valid);
-    const int first = act ? __ffsll(static_cast<long long>(act)) - 1 : 0;
-    const uint32_t s0 = __shfl(sid, first, 64);
-    const bool one = __ballot(valid && sid != s0) == 0ull;
-    const uint32_t wave = (blk * kBlock + threadIdx.x) >> 6;
-    const uint32_t slot = wave % kScenSlots;
-    if (one) {
-        unsigned long long v = cnt;
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            v += __shfl_xor(v, o, 64);
-        }
-        if (act && __lane_id() == 0 && v) {
-            atomicAdd(&c.part[(static_cast<size_t>(s0) * kScenSlots + slot) * kScenSlotStride], v);
-        }
-    } else if (valid && cnt) {
-        atomicAdd(&c.part[(static_cast<size_t>(sid) * kScenSlots + slot) * kScenSlotStride],
-                  static_cast<unsigned long long>(cnt));
-    }
-}
-
-// One wave per scenario: fold the slots into the scenario's pair totals.
-__global__ __launch_bounds__(64) void k_bfinish(unsigned long long* __restrict__ part,
-                                                unsigned long long* __restrict__ totals,
-                                                uint32_t* __restrict__ last) {
-    const uint32_t sidx = blockIdx.x;
-    unsigned long long v = 0;
-    if (threadIdx.x < kScenSlots) {
-        unsigned long long* p = &part[(static_cast<size_t>(sidx) * kScenSlots + threadIdx.x) * kScenSlotStride];
-        v = *p;
-        *p = 0ull;
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        v += __shfl_xor(v, o, 64);
-    }
-    if (threadIdx.x == 0) {
-        const unsigned long long pairs = v / 2;
-        totals[sidx] += pairs;
-        last[sidx] = static_cast<uint32_t>(pairs);
-    }
-}
-
+thread_local std::string g_batch_error;
 }  // namespace msim
 
-// ---------------------------------------------------------------------------
-
 namespace {
-
-thread_local std::string g_batch_error;
-
-struct BBuffers {
-    float2 *pos = nullptr, *tgt = nullptr, *dir = nullptr;
-    float* spd = nullptr;
-    uint32_t *road = nullptr, *init = nullptr, *ncount = nullptr, *perm = nullptr;
-    uint4* rng = nullptr;
-};
-
-}  // namespace
-
-struct msim_batch {
-    hipStream_t stream = nullptr;
-    uint32_t n_scen = 0, n = 0, n_roads = 0, n_conns = 0, n_seg = 0, start_len = 0, max_seg_log = 9;
-    std::vector<uint32_t> agent_off_h, road_off_h;
-    msim_road* roads = nullptr;
-    uint32_t* conns = nullptr;
-    BBuffers buf[2];
-    int cur = 0;
-    bool uniform = true;
-    float inv_cell = 1.0f, d2_limit = 1.0f;
-    ScenDev* scen = nullptr;
-    uint32_t* agent_off = nullptr;
-    BlockScen* blk_scen = nullptr;
-    uint32_t* seg_scen = nullptr;
-    uint32_t* seg_total = nullptr;
-    uint32_t* bstart[2] = {nullptr, nullptr};
-    float2* bbpos[2] = {nullptr, nullptr};
-    uint32_t* bbid[2] = {nullptr, nullptr};
-    int bcur = 0;
-    unsigned long long* part = nullptr;
-    unsigned long long* part_stats = nullptr;
-    unsigned long long* totals = nullptr;
-    uint32_t* last = nullptr;
-    unsigned int* faults = nullptr;
-    uint32_t* qdummy = nullptr;
-    bool first_tick_insert_only = false;
-    bool collide_per_lane = false;
-    uint32_t tick = 1;
-    uint32_t graph_ticks = 100, seg_ticks = 100;
-    uint32_t reorder_interval = 20, since_reorder = 0;
-    struct Segment {
-        hipGraphExec_t exec = nullptr;
-        int cur = 0, bcur = 0;
-    };
-    Segment gexec[2][2] = {};  // [cur][bcur] at the segment's start
-    uint32_t watchdog_ms = 600000;
-    // msim_batch_time_ticks probe: launch number probe_kernel of a tick is
-    // bracketed by probe_ev[0] / probe_ev[1]; -1 = no probe
-    int probe_kernel = -1, launch_no = 0;
-    hipEvent_t probe_ev[2] = {nullptr, nullptr};
-    std::vector<void*> allocs;
-    std::string err;
-
-    AgentsDev agents(int k) const {
-        const BBuffers& b = buf[k];
-        return AgentsDev{n, b.pos, b.tgt, b.dir, uniform ? nullptr : b.spd, kShaderSpeed,
-                         b.road, b.rng, b.init, b.ncount, b.perm};
-    }
-};
-
-namespace {
-
-int bfail(msim_batch* b, int code, const char* fmt, ...) {
-    char buf[512];
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(buf, sizeof(buf), fmt, ap);
-    va_end(ap);
-    if (b) b->err = buf;
-    g_batch_error = buf;
-    return code;
-}
-
-int balloc(msim_batch* b, void** p, size_t bytes) {
-    *p = nullptr;
-    if (bytes == 0) bytes = 16;
-    hipError_t e = hipMalloc(p, bytes);
-    if (e != hipSuccess) {
-        *p = nullptr;
-        (void)hipGetLastError();
-        return bfail(b, MSIM_ERR_OOM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
-    }
-    b->allocs.push_back(*p);
-    return MSIM_OK;
-}
-
-template <class T>
-int balloc_n(msim_batch* b, T** p, size_t count) {
-    return balloc(b, reinterpret_cast<void**>(p), count * sizeof(T));
-}
-
-uint32_t bblocks(uint32_t n) { return std::max(1u, (n + kBlock - 1) / kBlock); }
-
-int bcheck(msim_batch* b, const char* what) {
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return bfail(b, MSIM_ERR_HIP, "launch of %s failed: %s", what, hipGetErrorString(e));
-    return MSIM_OK;
-}
-
-// Launch order of a batched tick, for msim_batch_time_ticks.
-const char* const kBatchKernels[] = {"k_advance", "k_bbin_count", "k_bbin_place", "k_bcollide", "k_bfinish"};
-constexpr int kBatchKernelCount = 5;
-
-void bprobe_pre(msim_batch* b) {
-    if (b->probe_kernel == b->launch_no) (void)hipEventRecord(b->probe_ev[0], b->stream);
-}
-
-void bprobe_post(msim_batch* b) {
-    if (b->probe_kernel == b->launch_no) (void)hipEventRecord(b->probe_ev[1], b->stream);
-    b->launch_no++;
-}
-
-int bwait(msim_batch* b) {
-    const auto t0 = std::chrono::steady_clock::now();
-    for (;;) {
-        hipError_t e = hipStreamQuery(b->stream);
-        if (e == hipSuccess) return MSIM_OK;
-        if (e != hipErrorNotReady) return bfail(b, MSIM_ERR_HIP, "stream failed: %s", hipGetErrorString(e));
-        const auto ms =
-            std::chrono::duration_cast<std::chrono::milliseconds>(std::chrono::steady_clock::now() - t0).count();
-        if (ms > static_cast<long long>(b->watchdog_ms)) {
-            return bfail(b, MSIM_ERR_TIMEOUT, "stream did not drain within %u ms", b->watchdog_ms);
-        }
-        std::this_thread::sleep_for(std::chrono::microseconds(ms < 10 ? 20 : 500));
-    }
-}
-
-GridDev bgrid(const msim_batch* b) {
-    GridDev g{};
-    g.inv_cell = b->inv_cell;
-    g.d2_limit = b->d2_limit;
-    g.bid = b->bbid[b->bcur];
-    return g;
-}
-
-int benqueue_collide(msim_batch* b) {
-    BatchCollide c{};
-    c.blk = b->blk_scen;
-    c.scen = b->scen;
-    c.agent_off = b->agent_off;
-    c.n = b->n;
-    c.start = b->bstart[b->bcur];
-    c.bpos = b->bbpos[b->bcur];
-    c.bid = b->bbid[b->bcur];
-    c.ncount = b->buf[b->cur].ncount;
-    c.inv_cell = b->inv_cell;
-    c.d2_limit = b->d2_limit;
-    c.part = b->part;
-    bprobe_pre(b);
-    if (b->collide_per_lane) {
-        hipLaunchKernelGGL(k_bcollide<false>, dim3(bblocks(b->n)), dim3(kBlock), 0, b->stream, c);
-    } else {
-        hipLaunchKernelGGL(k_bcollide<true>, dim3(bblocks(b->n)), dim3(kBlock), 0, b->stream, c);
-    }
-    bprobe_post(b);
-    bprobe_pre(b);
-    hipLaunchKernelGGL(k_bfinish, dim3(b->n_scen), dim3(64), 0, b->stream, b->part, b->totals, b->last);
-    bprobe_post(b);
-    return bcheck(b, "batched collision dispatch");
-}
 
 template <bool U, bool W>
 void badvance(msim_batch* b) {
@@ -570,6 +222,7 @@ extern "C" int msim_batch_create(uint32_t n_scen, const uint32_t* road_off, cons
     b->uniform = speed == nullptr;
     b->inv_cell = 1.0f / cell;
     b->d2_limit = lim;
+    b->radius = radius;
     b->first_tick_insert_only = (init0 == 0);
     b->tick = pc->tick;
     b->graph_ticks = o.graph_ticks ? o.graph_ticks : 100u;
@@ -757,6 +410,10 @@ extern "C" int msim_batch_create(uint32_t n_scen, const uint32_t* road_off, cons
 
 extern "C" int msim_batch_run_ticks(msim_batch* b, uint32_t n_ticks) {
     if (!b) return bfail(b, MSIM_ERR_INVALID, "NULL handle");
+    if (b->diff) {
+        return bfail(b, MSIM_ERR_STATE, "this handle runs differentiable ticks (msim_batch_diff_setup); plain ticks "
+                                        "need a handle of their own");
+    }
     int rc = MSIM_OK;
     uint32_t done = 0;
     if (n_ticks && b->first_tick_insert_only) {
@@ -870,6 +527,7 @@ extern "C" int msim_batch_read_pairs(msim_batch* b, uint64_t* totals, uint32_t* 
 extern "C" int msim_batch_time_ticks(msim_batch* b, uint32_t n_ticks, float* ms_total, float* kernel_ms,
                                      uint32_t n_kernel_ms) {
     if (!b) return bfail(b, MSIM_ERR_INVALID, "NULL handle");
+    if (b->diff) return bfail(b, MSIM_ERR_STATE, "this handle runs differentiable ticks");
     hipEvent_t ev[2];
     if (hipEventCreate(&ev[0]) != hipSuccess) return bfail(b, MSIM_ERR_HIP, "hipEventCreate failed");
     if (hipEventCreate(&ev[1]) != hipSuccess) {
@@ -922,6 +580,7 @@ extern "C" const char* msim_batch_kernel_name(uint32_t k) {
 extern "C" int msim_batch_destroy(msim_batch* b) {
     if (!b) return MSIM_OK;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->diff) diff_release(b);
     for (auto& row : b->gexec)
         for (auto& g : row)
             if (g.exec) (void)hipGraphExecDestroy(g.exec);

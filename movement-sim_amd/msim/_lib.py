@@ -93,6 +93,10 @@ SIGNATURES = {
     "msim_batch_kernel_name": (ctypes.c_char_p, [_U32]),
     "msim_batch_destroy": (ctypes.c_int, [_VP]),
     "msim_batch_last_error": (ctypes.c_char_p, [_VP]),
+    "msim_batch_diff_setup": (ctypes.c_int, [_VP, _U32, _U32, ctypes.c_int]),
+    "msim_batch_diff_forward": (ctypes.c_int, [_VP, _VP, ctypes.c_float, ctypes.c_int, _VP]),
+    "msim_batch_diff_backward": (ctypes.c_int, [_VP, _VP, _VP, _VP]),
+    "msim_batch_diff_read_schedule": (ctypes.c_int, [_VP, _U32, _VP, _VP, _VP]),
 }
 
 _lib = None
@@ -303,6 +307,8 @@ class Batch:
         self.h = h
         self.n_scen = len(scs)
         self.n = int(self.agent_off[-1])
+        self.n_roads = int(self.road_off[-1])
+        self.diff_ticks = 0
 
     def _check(self, rc):
         if rc != MSIM_OK:
@@ -369,3 +375,25 @@ class Batch:
         k = np.zeros(max(1, len(names)), np.float32)
         self._check(self.lib.msim_batch_time_ticks(self.h, n, _ptr(ms), _ptr(k) if kernels else None, len(names)))
         return float(ms[0]), dict(zip(names, (float(v) for v in k)))
+
+    # -- differentiable tick (device pointers: ints, e.g. torch data_ptr()) --
+    def diff_setup(self, ticks, checkpoint_every=50, keep_schedule=False):
+        self._check(self.lib.msim_batch_diff_setup(self.h, ticks, checkpoint_every, int(keep_schedule)))
+        self.diff_ticks = ticks
+
+    def diff_forward(self, theta_ptr, alpha, final_pos_ptr, replay=False):
+        self._check(self.lib.msim_batch_diff_forward(self.h, ctypes.c_void_p(theta_ptr), float(alpha),
+                                                     int(replay), ctypes.c_void_p(final_pos_ptr)))
+
+    def diff_backward(self, grad_pos_ptr, grad_theta_ptr, grad_alpha_ptr):
+        self._check(self.lib.msim_batch_diff_backward(self.h, ctypes.c_void_p(grad_pos_ptr),
+                                                      ctypes.c_void_p(grad_theta_ptr),
+                                                      ctypes.c_void_p(grad_alpha_ptr)))
+
+    def diff_schedule(self, tick):
+        """(targets [n, 2], packed roads [n], flags [n]) of one recorded tick."""
+        t = np.zeros((self.n, 2), np.float32)
+        r = np.zeros(self.n, np.uint32)
+        f = np.zeros(self.n, np.uint8)
+        self._check(self.lib.msim_batch_diff_read_schedule(self.h, tick, _ptr(t), _ptr(r), _ptr(f)))
+        return t, r, f

@@ -380,6 +380,49 @@ def batch_config(n_scen=512, scale=1.0, seed=None, layout="own_first"):
                             (size, size), {"nodes": nd.shape[0], "config": 4, "index": s}))
     return out
 
+
+def pack_scenarios(scs):
+    """Scenarios packed the way the batched handle packs them: roads,
+    connections and agents concatenated, road indices (in connections and
+    Entity.roadIndex) and connection indices rebased to the packed arrays.
+    Returns (roads, conns, entities, speed or None, road_off, conn_off,
+    agent_off)."""
+    road_off = np.concatenate([[0], np.cumsum([sc.roads.shape[0] for sc in scs])]).astype(np.uint32)
+    conn_off = np.concatenate([[0], np.cumsum([sc.conns.shape[0] for sc in scs])]).astype(np.uint32)
+    agent_off = np.concatenate([[0], np.cumsum([sc.entities.shape[0] for sc in scs])]).astype(np.uint32)
+    roads, conns, ents = [], [], []
+    for k, sc in enumerate(scs):
+        r = sc.roads.copy()
+        r["start_index"] += conn_off[k]
+        r["end_index"] += conn_off[k]
+        roads.append(r)
+        conns.append(np.asarray(sc.conns, np.uint32) + road_off[k])
+        e = sc.entities.copy()
+        e["roadIndex"] += road_off[k]
+        ents.append(e)
+    speed = None if scs[0].speed is None else np.concatenate([sc.speed for sc in scs]).astype(np.float32)
+    return (np.concatenate(roads), np.concatenate(conns).astype(np.uint32), np.concatenate(ents), speed,
+            road_off, conn_off, agent_off)
+
+
+def adjoint_config(n_scen=64, agents=100_000, nodes=20_000, scale=1.0, seed=None, layout="own_first"):
+    """The calibration workload: n_scen scenarios on one jittered-lattice map
+    of about `nodes` nodes (the plain small construction at 32.3 m spacing),
+    `agents` agents each with their own seed.  scale < 1 shrinks nodes and
+    agents (tests)."""
+    seed = MASTER_SEED + 5 if seed is None else seed
+    side = max(4, int(round(np.sqrt(nodes * scale))))
+    size = 1000.0 * (side - 1) / 31.0
+    nd, edges = lattice_graph(side=side, size=size, seed=seed)
+    roads, conns = build_roads(nd, edges, layout)
+    n = max(1, int(round(agents * scale)))
+    out = []
+    for s in range(n_scen):
+        ents, speed = place_agents(roads, n, seed + 1 + s)
+        out.append(Scenario(f"adjoint-{s}", roads, conns, ents, speed, push_consts(size, size, 1.0), 1000,
+                            (size, size), {"nodes": nd.shape[0], "config": 5, "index": s}))
+    return out
+
 # Zipf-Mandelbrot offset for the dense configuration: with q = 0 the heaviest
 # of 1.3M roads would carry ~2M agents (~5e4 per 1 m cell, ~1e11 pair tests
 # per tick); q = 1000 keeps the top road at a few thousand agents so cell

@@ -1,0 +1,363 @@
+/*
+ * msim_diff_oracle.c — CPU restatement of the differentiable tick and its
+ * adjoint.  TEST INFRASTRUCTURE ONLY (see msim_oracle.h).
+ *
+ * The differentiable tick extends the reference tick (random_move.comp:841-852
+ * move, :778-828 route choice, :551-562 neighbour predicate) with a per-road
+ * speed multiplier theta and a soft-avoidance push:
+ *   s_i = v_i * theta[road_i];  arrived (|T - p| <= s): p~ = T, route choice;
+ *   else p~ = p + ((T - p) / |T - p|) * s;
+ *   p' = p~ + alpha * sum_{j in scenario, j != i, in range} w(|x|^2) x,
+ *   x = p~_i - p~_j, w(q) = (1 - q / R^2)^3 (C^2 across the range boundary).
+ * orc_diff_record runs it in binary32 (the move exactly as orc_move_agent;
+ * the push summed over neighbours in ascending index order) and records the
+ * decisions.  orc_diff_replay64 re-runs it in binary64 under recorded
+ * decisions; orc_diff_adjoint64 is the reverse mode of that replay.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "msim_oracle.h"
+
+#define ORC_SPEED 1.4f
+
+typedef struct {
+    int64_t key;
+    uint32_t id;
+} dbin;
+
+static int dbin_cmp(const void* a, const void* b) {
+    const dbin* x = (const dbin*)a;
+    const dbin* y = (const dbin*)b;
+    if (x->key != y->key) return x->key < y->key ? -1 : 1;
+    return x->id < y->id ? -1 : (x->id > y->id ? 1 : 0);
+}
+
+static int64_t dkey(int64_t cx, int64_t cy) { return (cy << 32) + cx; }
+
+/* Neighbour lists of n points (cell >= R): for each i, the j != i with
+ * pred(i, j), in ascending j.  Returns CSR (off[n+1], idx) via malloc. */
+typedef int (*pred_fn)(const void* pts, uint32_t i, uint32_t j, double r);
+
+static int pred_f32(const void* pts, uint32_t i, uint32_t j, double r) {
+    const float* p = (const float*)pts;
+    return orc_in_range(&p[2 * j], &p[2 * i], (float)r);
+}
+
+static int pred_f64(const void* pts, uint32_t i, uint32_t j, double r) {
+    const double* p = (const double*)pts;
+    double dx = p[2 * i] - p[2 * j], dy = p[2 * i + 1] - p[2 * j + 1];
+    return dx * dx + dy * dy < r * r;
+}
+
+static int neighbour_lists(uint32_t n, const void* pts, int f64, double r, double cell, uint32_t** off_out,
+                           uint32_t** idx_out) {
+    dbin* b = (dbin*)malloc(sizeof(dbin) * (n ? n : 1));
+    uint32_t* off = (uint32_t*)calloc((size_t)n + 1, sizeof(uint32_t));
+    if (!b || !off) {
+        free(b);
+        free(off);
+        return -1;
+    }
+    const float* pf = (const float*)pts;
+    const double* pd = (const double*)pts;
+    for (uint32_t i = 0; i < n; i++) {
+        double x = f64 ? pd[2 * i] : (double)pf[2 * i];
+        double y = f64 ? pd[2 * i + 1] : (double)pf[2 * i + 1];
+        b[i].key = dkey((int64_t)floor(x / cell), (int64_t)floor(y / cell));
+        b[i].id = i;
+    }
+    qsort(b, n, sizeof(dbin), dbin_cmp);
+    pred_fn pred = f64 ? pred_f64 : pred_f32;
+    size_t cap = 16, cnt = 0;
+    uint32_t* idx = (uint32_t*)malloc(sizeof(uint32_t) * cap);
+    uint32_t* tmp = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
+    if (!idx || !tmp) {
+        free(b);
+        free(off);
+        free(idx);
+        free(tmp);
+        return -1;
+    }
+    for (uint32_t i = 0; i < n; i++) {
+        double x = f64 ? pd[2 * i] : (double)pf[2 * i];
+        double y = f64 ? pd[2 * i + 1] : (double)pf[2 * i + 1];
+        int64_t cx = (int64_t)floor(x / cell), cy = (int64_t)floor(y / cell);
+        uint32_t m = 0;
+        for (int64_t dy = -1; dy <= 1; dy++) {
+            for (int64_t dx = -1; dx <= 1; dx++) {
+                int64_t key = dkey(cx + dx, cy + dy);
+                uint32_t lo = 0, hi = n;
+                while (lo < hi) {
+                    uint32_t mid = lo + (hi - lo) / 2;
+                    if (b[mid].key < key) lo = mid + 1; else hi = mid;
+                }
+                for (uint32_t k = lo; k < n && b[k].key == key; k++) {
+                    uint32_t j = b[k].id;
+                    if (j != i && pred(pts, i, j, r)) tmp[m++] = j;
+                }
+            }
+        }
+        /* ascending j */
+        for (uint32_t a = 1; a < m; a++) {
+            uint32_t v = tmp[a], c = a;
+            while (c > 0 && tmp[c - 1] > v) {
+                tmp[c] = tmp[c - 1];
+                c--;
+            }
+            tmp[c] = v;
+        }
+        if (cnt + m > cap) {
+            while (cnt + m > cap) cap *= 2;
+            uint32_t* grown = (uint32_t*)realloc(idx, sizeof(uint32_t) * cap);
+            if (!grown) {
+                free(b);
+                free(off);
+                free(idx);
+                free(tmp);
+                return -1;
+            }
+            idx = grown;
+        }
+        memcpy(idx + cnt, tmp, sizeof(uint32_t) * m);
+        cnt += m;
+        off[i + 1] = (uint32_t)cnt;
+    }
+    free(b);
+    free(tmp);
+    *off_out = off;
+    *idx_out = idx;
+    return 0;
+}
+
+int orc_diff_record(uint32_t n, uint32_t n_scen, const uint32_t* agent_off, float* pos, float* tgt,
+                    uint32_t* road, uint32_t* rng, const float* speed, const orc_road* roads, uint32_t n_roads,
+                    const uint32_t* conns, uint32_t n_conns, const float* theta, float alpha, float radius,
+                    uint32_t n_ticks, int no_move_first, float* sched_tgt, uint32_t* sched_road,
+                    uint8_t* sched_flag) {
+    const double cell = (double)orc_cell_size(radius);
+    const float inv_r2 = 1.0f / (radius * radius);
+    float* ptil = (float*)malloc(sizeof(float) * 2 * (n ? n : 1));
+    float dir[2];
+    if (!ptil) return -1;
+    for (uint32_t t = 0; t < n_ticks; t++) {
+        for (uint32_t i = 0; i < n; i++) {
+            uint32_t r = road[i];
+            if (r >= n_roads) {
+                free(ptil);
+                return ORC_ERR_ROAD;
+            }
+            uint8_t flag = 0;
+            sched_tgt[((size_t)t * n + i) * 2] = tgt[2 * i];
+            sched_tgt[((size_t)t * n + i) * 2 + 1] = tgt[2 * i + 1];
+            sched_road[(size_t)t * n + i] = r;
+            if (t == 0 && no_move_first) {
+                flag = 2;
+            } else {
+                float s = (speed ? speed[i] : ORC_SPEED) * theta[r];
+                int arrived = 0;
+                int rc = orc_move_agent(&pos[2 * i], &tgt[2 * i], dir, &road[i], &rng[4 * i], s, roads, n_roads,
+                                        conns, n_conns, &arrived);
+                if (rc != ORC_OK) {
+                    free(ptil);
+                    return rc;
+                }
+                flag = arrived ? 1 : 0;
+            }
+            sched_flag[(size_t)t * n + i] = flag;
+            ptil[2 * i] = pos[2 * i];
+            ptil[2 * i + 1] = pos[2 * i + 1];
+        }
+        if (alpha != 0.0f) {
+            for (uint32_t sc = 0; sc < n_scen; sc++) {
+                uint32_t o = agent_off[sc], m = agent_off[sc + 1] - o;
+                uint32_t *off = NULL, *idx = NULL;
+                if (neighbour_lists(m, ptil + 2 * (size_t)o, 0, radius, cell, &off, &idx) != 0) {
+                    free(ptil);
+                    return -1;
+                }
+                for (uint32_t i = 0; i < m; i++) {
+                    const float* me = &ptil[2 * (size_t)(o + i)];
+                    float ax = 0.0f, ay = 0.0f;
+                    for (uint32_t k = off[i]; k < off[i + 1]; k++) {
+                        const float* q = &ptil[2 * (size_t)(o + idx[k])];
+                        float x = me[0] - q[0], y = me[1] - q[1];
+                        float h = 1.0f - (x * x + y * y) * inv_r2;
+                        float w = h * h * h;
+                        ax += w * x;
+                        ay += w * y;
+                    }
+                    pos[2 * (size_t)(o + i)] = me[0] + alpha * ax;
+                    pos[2 * (size_t)(o + i) + 1] = me[1] + alpha * ay;
+                }
+                free(off);
+                free(idx);
+            }
+        }
+    }
+    free(ptil);
+    return ORC_OK;
+}
+
+/* binary64 forward under a recorded schedule; traj (nullable) receives the
+ * pre-move and post-move positions of every tick: [T][2][n][2]. */
+static int replay64(uint32_t n, uint32_t n_scen, const uint32_t* agent_off, const float* pos0, const float* speed,
+                    const double* theta, double alpha, double radius, uint32_t n_ticks, const float* sched_tgt,
+                    const uint32_t* sched_road, const uint8_t* sched_flag, double* pos_out, double* traj) {
+    const double cell = (double)orc_cell_size((float)radius);
+    double* p = (double*)malloc(sizeof(double) * 2 * (n ? n : 1));
+    double* pt = (double*)malloc(sizeof(double) * 2 * (n ? n : 1));
+    if (!p || !pt) {
+        free(p);
+        free(pt);
+        return -1;
+    }
+    for (uint32_t i = 0; i < 2 * n; i++) p[i] = (double)pos0[i];
+    for (uint32_t t = 0; t < n_ticks; t++) {
+        if (traj) memcpy(traj + (size_t)t * 4 * n, p, sizeof(double) * 2 * n);
+        for (uint32_t i = 0; i < n; i++) {
+            uint8_t f = sched_flag[(size_t)t * n + i];
+            double Tx = sched_tgt[((size_t)t * n + i) * 2], Ty = sched_tgt[((size_t)t * n + i) * 2 + 1];
+            if (f & 2) {
+                pt[2 * i] = p[2 * i];
+                pt[2 * i + 1] = p[2 * i + 1];
+            } else if (f & 1) {
+                pt[2 * i] = Tx;
+                pt[2 * i + 1] = Ty;
+            } else {
+                double s = (double)(speed ? speed[i] : ORC_SPEED) * theta[sched_road[(size_t)t * n + i]];
+                double dx = Tx - p[2 * i], dy = Ty - p[2 * i + 1];
+                double len = sqrt(dx * dx + dy * dy);
+                pt[2 * i] = p[2 * i] + dx / len * s;
+                pt[2 * i + 1] = p[2 * i + 1] + dy / len * s;
+            }
+        }
+        if (traj) memcpy(traj + (size_t)t * 4 * n + 2 * (size_t)n, pt, sizeof(double) * 2 * n);
+        memcpy(p, pt, sizeof(double) * 2 * n);
+        if (alpha != 0.0) {
+            for (uint32_t sc = 0; sc < n_scen; sc++) {
+                uint32_t o = agent_off[sc], m = agent_off[sc + 1] - o;
+                uint32_t *off = NULL, *idx = NULL;
+                if (neighbour_lists(m, pt + 2 * (size_t)o, 1, radius, cell, &off, &idx) != 0) {
+                    free(p);
+                    free(pt);
+                    return -1;
+                }
+                for (uint32_t i = 0; i < m; i++) {
+                    const double* me = &pt[2 * (size_t)(o + i)];
+                    double ax = 0.0, ay = 0.0;
+                    for (uint32_t k = off[i]; k < off[i + 1]; k++) {
+                        const double* q = &pt[2 * (size_t)(o + idx[k])];
+                        double x = me[0] - q[0], y = me[1] - q[1];
+                        double h = 1.0 - (x * x + y * y) / (radius * radius);
+                        ax += h * h * h * x;
+                        ay += h * h * h * y;
+                    }
+                    p[2 * (size_t)(o + i)] = me[0] + alpha * ax;
+                    p[2 * (size_t)(o + i) + 1] = me[1] + alpha * ay;
+                }
+                free(off);
+                free(idx);
+            }
+        }
+    }
+    memcpy(pos_out, p, sizeof(double) * 2 * n);
+    free(p);
+    free(pt);
+    return 0;
+}
+
+int orc_diff_replay64(uint32_t n, uint32_t n_scen, const uint32_t* agent_off, const float* pos0,
+                      const float* speed, const double* theta, double alpha, double radius, uint32_t n_ticks,
+                      const float* sched_tgt, const uint32_t* sched_road, const uint8_t* sched_flag,
+                      double* pos_out) {
+    return replay64(n, n_scen, agent_off, pos0, speed, theta, alpha, radius, n_ticks, sched_tgt, sched_road,
+                    sched_flag, pos_out, NULL);
+}
+
+int orc_diff_adjoint64(uint32_t n, uint32_t n_scen, const uint32_t* agent_off, const float* pos0,
+                       const float* speed, const double* theta, uint32_t n_roads, double alpha, double radius,
+                       uint32_t n_ticks, const float* sched_tgt, const uint32_t* sched_road,
+                       const uint8_t* sched_flag, const double* grad_pos, double* grad_theta, double* grad_alpha) {
+    const double cell = (double)orc_cell_size((float)radius);
+    const double inv_r2 = 1.0 / (radius * radius);
+    double* traj = (double*)malloc(sizeof(double) * 4 * (size_t)n * (n_ticks ? n_ticks : 1));
+    double* fin = (double*)malloc(sizeof(double) * 2 * (n ? n : 1));
+    double* lam = (double*)malloc(sizeof(double) * 2 * (n ? n : 1));
+    double* lt = (double*)malloc(sizeof(double) * 2 * (n ? n : 1));
+    int rc = -1;
+    if (!traj || !fin || !lam || !lt) goto done;
+    if (replay64(n, n_scen, agent_off, pos0, speed, theta, alpha, radius, n_ticks, sched_tgt, sched_road,
+                 sched_flag, fin, traj) != 0) {
+        goto done;
+    }
+    memset(grad_theta, 0, sizeof(double) * n_roads);
+    *grad_alpha = 0.0;
+    memcpy(lam, grad_pos, sizeof(double) * 2 * n);
+    for (uint32_t t = n_ticks; t-- > 0;) {
+        const double* pre = traj + (size_t)t * 4 * n;
+        const double* pt = pre + 2 * (size_t)n;
+        /* avoidance: lt = lam + alpha sum J (lam_i - lam_j); dL/dalpha += lam . A */
+        memcpy(lt, lam, sizeof(double) * 2 * n);
+        for (uint32_t sc = 0; sc < n_scen; sc++) {
+            uint32_t o = agent_off[sc], m = agent_off[sc + 1] - o;
+            uint32_t *off = NULL, *idx = NULL;
+            if (neighbour_lists(m, pt + 2 * (size_t)o, 1, radius, cell, &off, &idx) != 0) goto done;
+            for (uint32_t i = 0; i < m; i++) {
+                size_t gi = o + i;
+                double gx = 0.0, gy = 0.0, ax = 0.0, ay = 0.0;
+                for (uint32_t k = off[i]; k < off[i + 1]; k++) {
+                    size_t gj = o + idx[k];
+                    double x = pt[2 * gi] - pt[2 * gj], y = pt[2 * gi + 1] - pt[2 * gj + 1];
+                    double h = 1.0 - (x * x + y * y) * inv_r2;
+                    double w = h * h * h, wp = -3.0 * h * h * inv_r2;
+                    double dlx = lam[2 * gi] - lam[2 * gj], dly = lam[2 * gi + 1] - lam[2 * gj + 1];
+                    double xd = 2.0 * wp * (x * dlx + y * dly);
+                    gx += w * dlx + xd * x;
+                    gy += w * dly + xd * y;
+                    ax += w * x;
+                    ay += w * y;
+                }
+                lt[2 * gi] = lam[2 * gi] + alpha * gx;
+                lt[2 * gi + 1] = lam[2 * gi + 1] + alpha * gy;
+                *grad_alpha += lam[2 * gi] * ax + lam[2 * gi + 1] * ay;
+            }
+            free(off);
+            free(idx);
+        }
+        /* move */
+        for (uint32_t i = 0; i < n; i++) {
+            uint8_t f = sched_flag[(size_t)t * n + i];
+            if (f & 1) {
+                lam[2 * i] = 0.0;
+                lam[2 * i + 1] = 0.0;
+                continue;
+            }
+            if (f & 2) {
+                lam[2 * i] = lt[2 * i];
+                lam[2 * i + 1] = lt[2 * i + 1];
+                continue;
+            }
+            uint32_t r = sched_road[(size_t)t * n + i];
+            double v = (double)(speed ? speed[i] : ORC_SPEED);
+            double s = v * theta[r];
+            double dx = (double)sched_tgt[((size_t)t * n + i) * 2] - pre[2 * i];
+            double dy = (double)sched_tgt[((size_t)t * n + i) * 2 + 1] - pre[2 * i + 1];
+            double len = sqrt(dx * dx + dy * dy);
+            double ux = dx / len, uy = dy / len;
+            double ul = ux * lt[2 * i] + uy * lt[2 * i + 1];
+            double fct = s / len;
+            lam[2 * i] = lt[2 * i] - fct * (lt[2 * i] - ux * ul);
+            lam[2 * i + 1] = lt[2 * i + 1] - fct * (lt[2 * i + 1] - uy * ul);
+            grad_theta[r] += v * ul;
+        }
+    }
+    rc = 0;
+done:
+    free(traj);
+    free(fin);
+    free(lam);
+    free(lt);
+    return rc;
+}

@@ -94,6 +94,30 @@ int orc_move_phase_mt(uint32_t n, float* pos, float* tgt, float* dir, uint32_t* 
 void orc_advance_f64(uint32_t n, const float* pos, const float* tgt,
                      const float* speed, double* pos_out, int32_t* arrive_out);
 
+/* ---- differentiable tick (msim_diff_oracle.c) ----------------------------
+ * Soft-avoidance weight w(q) = (1 - q/R^2)^3 over in-range neighbours.
+ * Agents of scenario s are [agent_off[s], agent_off[s+1]); neighbours never
+ * cross scenarios.  theta has one multiplier per road.  The schedule arrays
+ * are [n_ticks][n] (targets: 2 floats per agent): the target and road each
+ * move used and a flag (bit 0 arrived, bit 1 no move; the first tick does not
+ * move when no_move_first). */
+int orc_diff_record(uint32_t n, uint32_t n_scen, const uint32_t* agent_off, float* pos, float* tgt,
+                    uint32_t* road, uint32_t* rng, const float* speed, const orc_road* roads, uint32_t n_roads,
+                    const uint32_t* conns, uint32_t n_conns, const float* theta, float alpha, float radius,
+                    uint32_t n_ticks, int no_move_first, float* sched_tgt, uint32_t* sched_road,
+                    uint8_t* sched_flag);
+/* binary64 replay under a schedule: final positions (2 doubles per agent). */
+int orc_diff_replay64(uint32_t n, uint32_t n_scen, const uint32_t* agent_off, const float* pos0,
+                      const float* speed, const double* theta, double alpha, double radius, uint32_t n_ticks,
+                      const float* sched_tgt, const uint32_t* sched_road, const uint8_t* sched_flag,
+                      double* pos_out);
+/* Reverse mode of orc_diff_replay64: dL/dtheta (n_roads) and dL/dalpha from
+ * dL/d(final positions). */
+int orc_diff_adjoint64(uint32_t n, uint32_t n_scen, const uint32_t* agent_off, const float* pos0,
+                       const float* speed, const double* theta, uint32_t n_roads, double alpha, double radius,
+                       uint32_t n_ticks, const float* sched_tgt, const uint32_t* sched_road,
+                       const uint8_t* sched_flag, const double* grad_pos, double* grad_theta, double* grad_alpha);
+
 #ifdef __cplusplus
 }
 #endif

@@ -43,6 +43,12 @@ def _load():
         "orc_collide_brute": (ctypes.c_int64, [_u32, _vp, ctypes.c_float, _vp, _vp]),
         "orc_collide_grid_mt": (ctypes.c_int64, [_u32, _vp, ctypes.c_float, _vp, ctypes.c_int]),
         "orc_advance_f64": (None, [_u32, _vp, _vp, _vp, _vp, _vp]),
+        "orc_diff_record": (ctypes.c_int, [_u32, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _vp, _u32, _vp,
+                                           ctypes.c_float, ctypes.c_float, _u32, ctypes.c_int, _vp, _vp, _vp]),
+        "orc_diff_replay64": (ctypes.c_int, [_u32, _u32, _vp, _vp, _vp, _vp, ctypes.c_double, ctypes.c_double,
+                                             _u32, _vp, _vp, _vp, _vp]),
+        "orc_diff_adjoint64": (ctypes.c_int, [_u32, _u32, _vp, _vp, _vp, _vp, _u32, ctypes.c_double,
+                                              ctypes.c_double, _u32, _vp, _vp, _vp, _vp, _vp, _vp]),
     }
     for k, (r, a) in sig.items():
         f = getattr(lib, k)
@@ -214,3 +220,69 @@ def advance_f64(pos, tgt, speed=None):
     arr = np.zeros(p.shape[0], np.int32)
     L.orc_advance_f64(p.shape[0], _p(p), _p(t), _p(s), _p(out), _p(arr))
     return out, arr
+
+
+# ---- differentiable tick ---------------------------------------------------
+
+class DiffSchedule:
+    """Decisions of a recorded differentiable run: per tick the target and
+    road each move used and a flag (1 arrived, 2 no move), caller order."""
+
+    def __init__(self, tgt, road, flag):
+        self.tgt = np.ascontiguousarray(tgt, np.float32)    # [T, n, 2]
+        self.road = np.ascontiguousarray(road, np.uint32)   # [T, n]
+        self.flag = np.ascontiguousarray(flag, np.uint8)    # [T, n]
+        self.ticks = self.flag.shape[0]
+
+
+def diff_record(roads, conns, entities, speed, agent_off, theta, alpha, radius, ticks):
+    """binary32 differentiable run from the entities' state (the first tick
+    does not move when the agents are uninitialised): (final pos, schedule)."""
+    st = State(entities, speed)
+    n = st.n
+    off = np.ascontiguousarray(agent_off, np.uint32)
+    th = np.ascontiguousarray(theta, np.float32)
+    rb = roads_buffer(roads)
+    cn = np.ascontiguousarray(conns, np.uint32)
+    tg = np.zeros((ticks, n, 2), np.float32)
+    rd = np.zeros((ticks, n), np.uint32)
+    fl = np.zeros((ticks, n), np.uint8)
+    no_move = int((st.init == 0).all()) if n else 0
+    rc = L.orc_diff_record(n, off.shape[0] - 1, _p(off), _p(st.pos), _p(st.tgt), _p(st.road), _p(st.rng),
+                           _p(st.speed), _p(rb), roads.shape[0], _p(cn), cn.shape[0], _p(th), alpha, radius, ticks,
+                           no_move, _p(tg), _p(rd), _p(fl))
+    if rc != 0:
+        raise RuntimeError(f"orc_diff_record failed: {rc}")
+    return st.pos, DiffSchedule(tg, rd, fl)
+
+
+def diff_replay64(pos0, speed, agent_off, theta, alpha, radius, sched):
+    p0 = np.ascontiguousarray(pos0, np.float32)
+    n = p0.shape[0]
+    off = np.ascontiguousarray(agent_off, np.uint32)
+    th = np.ascontiguousarray(theta, np.float64)
+    sp = None if speed is None else np.ascontiguousarray(speed, np.float32)
+    out = np.zeros((n, 2), np.float64)
+    rc = L.orc_diff_replay64(n, off.shape[0] - 1, _p(off), _p(p0), _p(sp), _p(th), float(alpha), float(radius),
+                             sched.ticks, _p(sched.tgt), _p(sched.road), _p(sched.flag), _p(out))
+    if rc != 0:
+        raise RuntimeError("orc_diff_replay64 failed")
+    return out
+
+
+def diff_adjoint64(pos0, speed, agent_off, theta, alpha, radius, sched, grad_pos):
+    """(dL/dtheta, dL/dalpha) in binary64 for dL/d(final pos) = grad_pos."""
+    p0 = np.ascontiguousarray(pos0, np.float32)
+    n = p0.shape[0]
+    off = np.ascontiguousarray(agent_off, np.uint32)
+    th = np.ascontiguousarray(theta, np.float64)
+    sp = None if speed is None else np.ascontiguousarray(speed, np.float32)
+    g = np.ascontiguousarray(grad_pos, np.float64)
+    gt = np.zeros(th.shape[0], np.float64)
+    ga = np.zeros(1, np.float64)
+    rc = L.orc_diff_adjoint64(n, off.shape[0] - 1, _p(off), _p(p0), _p(sp), _p(th), th.shape[0], float(alpha),
+                              float(radius), sched.ticks, _p(sched.tgt), _p(sched.road), _p(sched.flag), _p(g),
+                              _p(gt), _p(ga))
+    if rc != 0:
+        raise RuntimeError("orc_diff_adjoint64 failed")
+    return gt, float(ga[0])

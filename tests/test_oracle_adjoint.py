@@ -1,0 +1,75 @@
+"""The oracle's differentiable tick and its binary64 adjoint, pinned on the
+CPU: with theta = 1 and alpha = 0 the recording run is the oracle's plain tick;
+the binary64 adjoint agrees with central finite differences of the binary64
+replay, and the finite differences converge to it at the rate of a central
+difference (error ~ eps^2)."""
+import numpy as np
+import pytest
+
+import orc
+from msim import scenarios as S
+
+
+def packed(n_scen=3, scale=0.01, seed=5):
+    scs = S.batch_config(n_scen, scale=scale, seed=seed)
+    return scs, S.pack_scenarios(scs)
+
+
+def test_neutral_record_is_the_plain_tick():
+    scs, (roads, conns, ents, speed, _, _, off) = packed()
+    theta = np.ones(roads.shape[0], np.float32)
+    fin, sched = orc.diff_record(roads, conns, ents, speed, off, theta, 0.0, 1.0, 90)
+    for s, sc in enumerate(scs):
+        o = orc.OracleSim(sc.roads, sc.conns, sc.entities, sc.pc, speed=sc.speed)
+        o.run_ticks(90)
+        a, b = off[s], off[s + 1]
+        np.testing.assert_array_equal(fin[a:b].view(np.uint32), o.st.pos.view(np.uint32))
+    assert sched.flag[0].min() == 2 and sched.flag[1:].max() <= 1  # insert-only first tick
+
+
+def _problem(alpha=0.05, ticks=80, seed=1):
+    scs, (roads, conns, ents, speed, _, _, off) = packed()
+    rng = np.random.default_rng(seed)
+    theta = np.exp(rng.normal(0.0, 0.2, roads.shape[0])).astype(np.float32)
+    fin, sched = orc.diff_record(roads, conns, ents, speed, off, theta, alpha, 1.0, ticks)
+    p0 = ents["pos"].copy()
+    th = theta.astype(np.float64)
+    p = orc.diff_replay64(p0, speed, off, th, alpha, 1.0, sched)
+    target = p + rng.normal(0.0, 1.0, p.shape)
+
+    def loss(t, a):
+        q = orc.diff_replay64(p0, speed, off, t, a, 1.0, sched)
+        return float(((q - target) ** 2).sum())
+
+    gt, ga = orc.diff_adjoint64(p0, speed, off, th, alpha, 1.0, sched, 2.0 * (p - target))
+    return th, alpha, loss, gt, ga, rng
+
+
+def test_fp64_adjoint_matches_central_differences():
+    th, alpha, loss, gt, ga, rng = _problem()
+    v = rng.normal(0.0, 1.0, th.shape)
+    eps = 1e-3
+    fd = (loss(th + eps * v, alpha) - loss(th - eps * v, alpha)) / (2 * eps)
+    assert abs(gt @ v - fd) <= 1e-3 * abs(fd)
+    fda = (loss(th, alpha + eps) - loss(th, alpha - eps)) / (2 * eps)
+    assert abs(ga - fda) <= 1e-3 * abs(fda)
+
+
+def test_fp64_adjoint_is_the_limit_of_central_differences():
+    th, alpha, loss, gt, _, _ = _problem()
+    for r in np.argsort(-np.abs(gt))[:3]:
+        e = np.zeros_like(th)
+        e[r] = 1.0
+        errs = []
+        for eps in (1e-3, 1e-4):
+            fd = (loss(th + eps * e, alpha) - loss(th - eps * e, alpha)) / (2 * eps)
+            errs.append(abs(fd - gt[r]) / abs(gt[r]))
+        assert errs[1] < 1e-5 and (errs[0] < 1e-9 or errs[1] < 0.05 * errs[0]), (r, errs)
+
+
+def test_alpha_zero_gradient_is_the_pushes_first_order_effect():
+    # at alpha = 0 the alpha derivative is sum(lam . A) over the run
+    th, _, loss, gt, ga, _ = _problem(alpha=0.0)
+    eps = 1e-4
+    fda = (loss(th, eps) - loss(th, -eps)) / (2 * eps)
+    assert abs(ga - fda) <= 1e-4 * max(1.0, abs(fda))
