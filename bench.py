@@ -21,6 +21,10 @@ sys.path.insert(0, os.path.join(ROOT, "movement-sim_amd"))
 METRIC = "agent-ticks/s, 1 GPU: batched vs plain at bit-exact state; HBM GB/s % of peak"
 
 WORKLOADS = {
+    5: "adjoint calibration: 64 scenarios of 100,000 agents on one 20,000-node jittered lattice (141x141, "
+       "32.3 m spacing), 1,000 ticks, per-road speed multipliers (true: log-normal sigma 0.2) and a soft-"
+       "avoidance strength (true 0.02), loss = mean squared final-position error, Adam, checkpoints every "
+       "50 ticks",
     4: "batched: 512 independent jittered-lattice scenarios (32.3 m node spacing, 15% of edges dropped), "
        "nodes log-uniform in [1e3, 1e5], agents log-uniform in [1e3, 5e5] (~41M agents), speed "
        "U[0.8,1.6) m/tick, R 1 m, 1 m cells",
@@ -183,10 +187,15 @@ def batched_line(args, scs):
     scenarios and ticks, with every scenario's final state compared bit for
     bit."""
     import msim
+    from msim import autotune
 
     n = sum(int(sc.entities.shape[0]) for sc in scs)
-    b = msim.Batch(scs, reorder_interval=args.batch_reorder, graph_ticks=args.batch_graph_ticks,
-                   segment_log2=args.batch_segment_log2)
+    opts = dict(autotune.BATCH_DEFAULT) if args.untuned else autotune.tuned_batch_options(scs)
+    for k in ("segment_log2", "reorder_interval", "graph_ticks"):
+        v = getattr(args, "batch_" + k)
+        if v is not None:
+            opts[k] = v
+    b = msim.Batch(scs, **opts)
     extra = {}
     if args.compare_ticks > 0:
         extra["batched_vs_plain"] = compare_plain(args, scs, b, n)
@@ -208,12 +217,53 @@ def batched_line(args, scs):
         "agents_min_max": [min(agents), max(agents)],
         "roads": int(sum(sc.roads.shape[0] for sc in scs)),
         "collision_radius_m": float(scs[0].pc["collisionRadius"]),
-        "graph_segment_ticks": args.batch_graph_ticks,
-        "reorder_interval": args.batch_reorder,
-        "segment_log2": args.batch_segment_log2,
+        "options": opts,
     }
     extra["fused_tick"] = fused(el / args.steps, n, per_launch_ms)
     return n * args.steps / el, 1e3 * el / args.steps, config, roof, extra
+
+
+def adjoint_line(args):
+    """Calibration through the adjoint: one step = one forward + adjoint
+    gradient of the whole run (all scenarios, all ticks) and an Adam update."""
+    from msim import calibrate
+    from msim import scenarios as S
+
+    scs = S.adjoint_config()
+    res = calibrate.calibrate(scs, ticks=1000, steps=args.steps,
+                              log=lambda m: print("# " + m, file=sys.stderr, flush=True))
+    n = res["agents"]
+    sec = res["seconds_per_gradient"]
+    h = res["history"]
+    out = {
+        "metric": METRIC,
+        "value": n * res["ticks"] / sec,
+        "unit": "agent-ticks/s (differentiable forward + adjoint)",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": 0,
+        "ms_per_step": 1e3 * sec,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (adjoint f64)",
+        "data": "synthetic (seeded generator, msim/scenarios.py)",
+        "config": {"workload": WORKLOADS[5], "agents": n, "roads_per_map": res["roads_per_map"],
+                   "ticks": res["ticks"], "checkpoint_every": 50},
+        "adjoint": {
+            "seconds_per_gradient": sec,
+            "hidden_run_seconds": res["hidden_run_seconds"],
+            "loss_m2_first_last": [h[0]["loss_m2"], h[-1]["loss_m2"]],
+            "loss_m2_best": [res["best_step"], h[res["best_step"]]["loss_m2"]],
+            "alpha_first_last": [h[0]["alpha"], h[-1]["alpha"]],
+            "alpha_true": res["alpha_true"],
+            "theta_mean_rel_err_first_last": [h[0]["theta_mean_rel_err"], h[-1]["theta_mean_rel_err"]],
+            "loss_m2_every_10": [x["loss_m2"] for x in h[::10]],
+        },
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
 
 
 def main():
@@ -221,13 +271,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed ticks (default: 1000 batched, 10000 plain)")
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--config", type=int, default=4, choices=(1, 2, 3, 4))
+    ap.add_argument("--config", type=int, default=4, choices=(1, 2, 3, 4, 5))
     ap.add_argument("--scenarios", type=int, default=512, help="batched workload: number of scenarios")
     ap.add_argument("--compare-ticks", type=int, default=200, help="batched workload: ticks of the plain-loop "
                                                                     "comparison")
-    ap.add_argument("--batch-reorder", type=int, default=20)
-    ap.add_argument("--batch-graph-ticks", type=int, default=100)
-    ap.add_argument("--batch-segment-log2", type=int, default=0, help="0: per-scenario default")
+    ap.add_argument("--batch-reorder-interval", type=int, default=None, help="override the tuned value")
+    ap.add_argument("--batch-graph-ticks", type=int, default=None, help="override the tuned value")
+    ap.add_argument("--batch-segment-log2", type=int, default=None, help="override the tuned value")
     ap.add_argument("--profile-ticks", type=int, default=240)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -238,11 +288,14 @@ def main():
               "(use --gpus 1)", file=sys.stderr)
         sys.exit(2)
     if args.steps is None:
-        args.steps = {1: 10000, 2: 10000, 3: 500, 4: 1000}[args.config]
+        args.steps = {1: 10000, 2: 10000, 3: 500, 4: 1000, 5: 50}[args.config]
 
     from msim import scenarios as S
 
     t = time.perf_counter()
+    if args.config == 5:
+        adjoint_line(args)
+        return
     if args.config == 4:
         scs = S.batch_config(args.scenarios)
         n = sum(int(sc.entities.shape[0]) for sc in scs)

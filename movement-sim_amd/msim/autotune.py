@@ -1,4 +1,4 @@
-"""Autotuner for the plain pipeline's launch and layout parameters.
+"""Autotuner for the plain and batched pipelines' launch and layout parameters.
 
 Coordinate descent over the msim_options knobs that change speed but never
 results (every value is covered by the parity tests): bucket-table row width,
@@ -8,6 +8,10 @@ workload with HIP events around graph-replayed ticks; the winner is cached in
 a JSON file keyed by the workload's shape.
 
     python -m msim.autotune --config 2          (from movement-sim_amd/)
+    python -m msim.autotune --batch             (the 512-scenario batched workload)
+
+The batched handle's knobs: rebinning segment size (for every scenario),
+reorder interval, ticks per captured graph segment and the neighbour walk.
 """
 import argparse
 import json
@@ -16,7 +20,7 @@ import time
 
 import numpy as np
 
-from ._lib import Simulation, MsimError
+from ._lib import Batch, Simulation, MsimError
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CACHE = os.path.join(HERE, "tuned.json")
@@ -32,6 +36,48 @@ SPACE = {
 }
 DEFAULT = {"table_width_log2": 0, "segment_log2": 0, "binning": 0, "route_queue": 0,
            "reorder_interval": 16, "collide_mode": 0}
+
+
+BATCH_SPACE = {
+    "segment_log2": [0, 11, 12, 13],
+    "reorder_interval": [10, 20, 25, 50],
+    "graph_ticks": [50, 100, 200],
+    "collide_mode": [0, 1],
+}
+BATCH_DEFAULT = {"segment_log2": 0, "reorder_interval": 20, "graph_ticks": 100, "collide_mode": 0}
+
+
+def batch_key(scs):
+    n = int(sum(sc.entities.shape[0] for sc in scs))
+    r = int(sum(sc.roads.shape[0] for sc in scs))
+    uni = "u" if scs[0].speed is None else "s"
+    return f"batch{len(scs)}-n{n}-r{r}-R{float(scs[0].pc['collisionRadius']):g}-{uni}"
+
+
+def time_batch_options(scs, opts, warmup, ticks):
+    """Microseconds per batched tick (HIP events around graph segments)."""
+    try:
+        b = Batch(scs, **opts)
+    except MsimError:
+        return float("inf")
+    try:
+        b.run_ticks(warmup)
+        b.sync()
+        ms, _ = b.time_ticks(ticks)
+        return 1e3 * ms / ticks
+    except MsimError:
+        return float("inf")
+    finally:
+        b.close()
+
+
+def tune_batch(scs, warmup=101, ticks=200, log=print):
+    return _descend(lambda o: time_batch_options(scs, o, warmup, ticks), BATCH_DEFAULT, BATCH_SPACE, log)
+
+
+def tuned_batch_options(scs, path=CACHE):
+    ent = load_cache(path).get(batch_key(scs))
+    return dict(ent["options"]) if ent else dict(BATCH_DEFAULT)
 
 
 def workload_key(sc):
@@ -61,8 +107,13 @@ def time_options(sc, opts, warmup, ticks):
 
 def tune(sc, warmup=20, ticks=100, space=None, log=print):
     space = SPACE if space is None else space
-    best = dict(DEFAULT)
-    best_us = time_options(sc, best, warmup, ticks)
+    return _descend(lambda o: time_options(sc, o, warmup, ticks), DEFAULT, space, log)
+
+
+def _descend(timer, default, space, log):
+    """Coordinate descent: each knob in turn, keeping the best value found."""
+    best = dict(default)
+    best_us = timer(best)
     log(f"  start {best} -> {best_us:.1f} us/tick")
     trials = [(dict(best), best_us)]
     for knob, values in space.items():
@@ -71,7 +122,7 @@ def tune(sc, warmup=20, ticks=100, space=None, log=print):
                 continue
             cand = dict(best)
             cand[knob] = v
-            us = time_options(sc, cand, warmup, ticks)
+            us = timer(cand)
             trials.append((cand, us))
             log(f"  {knob}={v}: {us:.1f} us/tick" + (" (new best)" if us < best_us else ""))
             if us < best_us:
@@ -98,12 +149,29 @@ def main():
     from msim import scenarios as S
 
     ap = argparse.ArgumentParser()
-    ap.add_argument("--config", type=int, nargs="+", default=[2])
+    ap.add_argument("--config", type=int, nargs="*", default=None, help="plain configurations (default 2, "
+                                                                            "none with --batch)")
+    ap.add_argument("--batch", action="store_true", help="tune the batched workload")
     ap.add_argument("--ticks", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--out", default=CACHE)
     args = ap.parse_args()
     cache = load_cache(args.out)
+    if args.config is None:
+        args.config = [] if args.batch else [2]
+    if args.batch:
+        scs = S.batch_config()
+        key = batch_key(scs)
+        print(f"batched workload ({key})", flush=True)
+        t0 = time.time()
+        best, us, trials = tune_batch(scs, log=lambda m: print(m, flush=True))
+        n = sum(sc.entities.shape[0] for sc in scs)
+        cache[key] = {"config": 4, "options": best, "us_per_tick": us, "agent_ticks_per_s": n / us * 1e6,
+                      "default_us_per_tick": trials[0][1], "trials": len(trials),
+                      "tuned_s": round(time.time() - t0, 1)}
+        print(json.dumps(cache[key]), flush=True)
+        with open(args.out, "w") as f:
+            json.dump(cache, f, indent=1, sort_keys=True)
     for k in args.config:
         sc = S.config(k)
         key = workload_key(sc)

@@ -31,14 +31,14 @@ assert B_FUSED == 64
 def kernel_bytes(name, uniform_speed=False):
     """Algorithmic bytes per agent for one launch of the named kernel.
 
-    k_advance 28 (24 with the shader's constant speed); k_bin_count 8 (bucket
+    k_advance / k_advance4 28 (24 with the shader's constant speed); k_bin_count 8 (bucket
     offsets) and k_bin_place 12 (binned position + id) in the segment path, or
     k_scan_down 8 and k_scatter 12 in the atomic path; k_collide 16.  The
     batched kernels k_bbin_count, k_bbin_place and k_bcollide do the same
     work per agent as their plain counterparts.  k_route (arrival path), the
     single-workgroup scan/finish passes, k_bfinish and k_reorder: 0."""
     name = name.split("<")[0]
-    if name in ("k_advance", "k_advance_bin"):
+    if name in ("k_advance", "k_advance4", "k_advance_bin"):
         return B_ADVANCE_UNIFORM_SPEED if uniform_speed else B_ADVANCE
     if name in ("k_bin_count", "k_bbin_count", "k_scan_down"):
         return B_SCAN
