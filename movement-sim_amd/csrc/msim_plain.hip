@@ -1098,4 +1098,7 @@ extern "C" const char* msim_last_error(const msim_sim* s) { return s ? s->err.c_
 
 extern "C" const char* msim_get_last_error(void) { return g_last_error.c_str(); }
 
-extern "C" const char* msim_version(void) { return "msim 0.1.0 gfx950 fp-contract=off"; }
+#ifndef MSIM_FP_CONTRACT
+#define MSIM_FP_CONTRACT "off"
+#endif
+extern "C" const char* msim_version(void) { return "msim 0.1.0 gfx950 fp-contract=" MSIM_FP_CONTRACT; }

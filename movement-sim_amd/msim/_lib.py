@@ -12,6 +12,9 @@ from .types import ENTITY_DTYPE, PUSH_DTYPE, ROAD_DTYPE
 
 PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_ROOT, "lib", "libmsim.so")
+# the same library built with -ffp-contract=fast (integer-deciding arithmetic
+# stays single-rounding; positions may differ from LIB_PATH's by fused roundings)
+LIB_CONTRACT_PATH = os.path.join(PKG_ROOT, "lib", "libmsim_contract.so")
 
 MSIM_OK = 0
 STATUS_NAMES = {1: "MSIM_ERR_INVALID", 2: "MSIM_ERR_HIP", 3: "MSIM_ERR_OOM", 4: "MSIM_ERR_STATE",
@@ -141,8 +144,8 @@ class Simulation:
     array, speed: float32 array or None (constant 1.4 m/tick), pc: PUSH_DTYPE.
     """
 
-    def __init__(self, roads, connections, entities, pc, speed=None, **options):
-        self.lib = load_library()
+    def __init__(self, roads, connections, entities, pc, speed=None, library=None, **options):
+        self.lib = library if library is not None else load_library()
         roads = np.ascontiguousarray(roads, ROAD_DTYPE)
         conns = np.ascontiguousarray(connections, np.uint32)
         ents = np.ascontiguousarray(entities, ENTITY_DTYPE)
