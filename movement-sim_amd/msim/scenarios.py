@@ -230,7 +230,9 @@ def _rank_in_node(order, ends, node_ptr, n_roads):
 def star_map():
     """Four roads from the corners of a 100 m square to its centre: the
     structure of the reference's test_map.json (own-first groups; corner
-    endpoints count 1, centre endpoints count 4), re-derived here."""
+    endpoints count 1, centre endpoints count 4), re-derived here; the centre
+    groups list the other roads in ascending order (the fixture's own order
+    differs for roads 2 and 3), plus a guard entry."""
     nodes = np.array([[50, 50], [5, 5], [95, 5], [5, 95], [95, 95]], np.float32)
     edges = np.array([[1, 0], [2, 0], [3, 0], [4, 0]])
     roads, conns = build_roads(nodes, edges, "own_first")

@@ -283,3 +283,23 @@ def test_boundary_pairs_match_brute_force(radius):
     pairs = orc.L.orc_collide_brute(n, orc._p(np.ascontiguousarray(p)), radius, None, orc._p(c))
     np.testing.assert_array_equal(st["ncount"], c)
     assert g.pairs()[0] == pairs
+
+
+def test_reference_map_file_drives_the_hip_path():
+    """The reference's fixture map, loaded from its JSON (guard entry added),
+    with the shader's constants: every dispatch against the oracle."""
+    import os
+
+    from msim import mapio
+
+    m = mapio.load_map(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "test_map.json"))
+    conns = mapio.with_guard(m.conns, m.roads)
+    ents = S.reference_agents(m.roads, 96, 17)
+    pc = msim.push_consts(m.width, m.height, 10.0)
+    g = msim.Simulation(m.roads, conns, ents, pc)
+    o = orc.OracleSim(m.roads, conns, ents, pc)
+    for t in range(400):
+        g.dispatch()
+        o.dispatch()
+    assert_same(g.state(), o.st, "reference map")
+    assert g.pairs()[0] == o.pairs_total
