@@ -690,6 +690,61 @@ static __global__ __launch_bounds__(kBlock) void k_reorder(AgentsDev src, Agents
     g.bid[s] = s;
 }
 
+// ---- Entity records (64 B, the layout of sim::Entity, Entity.hpp:33-46) ----
+// @0 colour f32x4, @16 randomState u32x4, @32 pos, @40 target, @48 direction
+// f32x2 each, @56 roadIndex, @60 initialized: four 16-byte words per record.
+
+// AoS records -> the SoA state (internal order = caller order) and the
+// records' colours.
+static __global__ __launch_bounds__(kBlock) void k_unpack_entities(const uint4* __restrict__ rec, AgentsDev a,
+                                                                   float4* __restrict__ color0) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= a.n) {
+        return;
+    }
+    const uint4* r = rec + 4ull * i;
+    const uint4 c = r[0], st = r[1], pt = r[2], dr = r[3];
+    color0[i] = make_float4(__uint_as_float(c.x), __uint_as_float(c.y), __uint_as_float(c.z), __uint_as_float(c.w));
+    a.rng[i] = st;
+    a.pos[i] = make_float2(__uint_as_float(pt.x), __uint_as_float(pt.y));
+    a.tgt[i] = make_float2(__uint_as_float(pt.z), __uint_as_float(pt.w));
+    a.dir[i] = make_float2(__uint_as_float(dr.x), __uint_as_float(dr.y));
+    a.road[i] = dr.z;
+    a.init[i] = dr.w ? 1u : 0u;
+    a.ncount[i] = 0u;
+    a.perm[i] = i;
+}
+
+// The SoA state -> records of caller indices [first, first + count), at
+// out[index - first].  coloured: blue for agents in a pair in the last
+// collision dispatch, green otherwise (random_move.comp:545-548, 876);
+// otherwise the uploaded colours.
+static __global__ __launch_bounds__(kBlock) void k_pack_entities(AgentsDev a, const float4* __restrict__ color0,
+                                                                 uint32_t coloured, uint32_t first, uint32_t count,
+                                                                 uint4* __restrict__ out) {
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= a.n) {
+        return;
+    }
+    const uint32_t i = a.perm[k];
+    const uint32_t j = i - first;
+    if (j >= count) {
+        return;
+    }
+    float4 c;
+    if (coloured) {
+        c = a.ncount[k] ? make_float4(0.0f, 0.0f, 1.0f, 1.0f) : make_float4(0.0f, 1.0f, 0.0f, 1.0f);
+    } else {
+        c = color0[i];
+    }
+    const float2 p = a.pos[k], t = a.tgt[k], d = a.dir[k];
+    uint4* r = out + 4ull * j;
+    r[0] = make_uint4(__float_as_uint(c.x), __float_as_uint(c.y), __float_as_uint(c.z), __float_as_uint(c.w));
+    r[1] = a.rng[k];
+    r[2] = make_uint4(__float_as_uint(p.x), __float_as_uint(p.y), __float_as_uint(t.x), __float_as_uint(t.y));
+    r[3] = make_uint4(__float_as_uint(d.x), __float_as_uint(d.y), a.road[k], a.init[k]);
+}
+
 // Per-agent cell export in caller order.
 static __global__ __launch_bounds__(kBlock) void k_export_cells(AgentsDev a, float inv_cell, int2* out) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;

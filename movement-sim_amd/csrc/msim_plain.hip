@@ -86,7 +86,8 @@ struct msim_sim {
     msim_push_consts pc{};
     uint32_t n_uninit = 0;
     bool any_collide = false;
-    std::vector<msim_rgba> color0;
+    float4* color0 = nullptr;       // [n] uploaded colours, caller order
+    msim_entity* ent_stage = nullptr;  // [n] Entity records (upload at creation, readback staging)
     uint64_t ticks_simulated = 0;
     uint32_t reorder_interval = 0;
     uint32_t since_reorder = 0;
@@ -732,6 +733,8 @@ extern "C" int msim_create(const msim_road* roads, uint32_t n_roads, const uint3
     ALLOC(s->part_stats, kSlots * kSlotStride);
     ALLOC(s->part_pairs, kSlots * kSlotStride);
     ALLOC(s->hist, s->hist_cap);
+    ALLOC(s->color0, n);
+    ALLOC(s->ent_stage, n);
 #undef ALLOC
     if (rc != MSIM_OK) {
         std::string e = s->err;
@@ -739,23 +742,8 @@ extern "C" int msim_create(const msim_road* roads, uint32_t n_roads, const uint3
         return fail(nullptr, rc, "%s", e.c_str());
     }
 
-    // Entity AoS -> SoA.
-    std::vector<float2> pos(n), tgt(n), dir(n);
-    std::vector<uint32_t> road(n), init(n), perm(n), zeros(n, 0u);
-    std::vector<uint4> rng(n);
-    s->color0.resize(n);
-    for (uint32_t i = 0; i < n; i++) {
-        const msim_entity& e = entities[i];
-        pos[i] = make_float2(e.pos.x, e.pos.y);
-        tgt[i] = make_float2(e.target.x, e.target.y);
-        dir[i] = make_float2(e.direction.x, e.direction.y);
-        road[i] = e.roadIndex;
-        rng[i] = make_uint4(e.randomState.x, e.randomState.y, e.randomState.z, e.randomState.w);
-        init[i] = e.initialized ? 1u : 0u;
-        s->n_uninit += init[i] ? 0u : 1u;
-        perm[i] = i;
-        s->color0[i] = e.color;
-    }
+    // The Entity records go up as they are; k_unpack_entities splits them.
+    for (uint32_t i = 0; i < n; i++) s->n_uninit += entities[i].initialized ? 0u : 1u;
     auto up = [&](void* dst, const void* src, size_t bytes) {
         if (rc == MSIM_OK && bytes) {
             hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
@@ -765,15 +753,8 @@ extern "C" int msim_create(const msim_road* roads, uint32_t n_roads, const uint3
     Buffers& b0 = s->buf[0];
     up(s->roads, roads, sizeof(msim_road) * n_roads);
     up(s->conns, connections, sizeof(uint32_t) * n_connections);
-    up(b0.pos, pos.data(), sizeof(float2) * n);
-    up(b0.tgt, tgt.data(), sizeof(float2) * n);
-    up(b0.dir, dir.data(), sizeof(float2) * n);
+    up(s->ent_stage, entities, sizeof(msim_entity) * n);
     if (!s->uniform) up(b0.spd, speed, sizeof(float) * n);
-    up(b0.road, road.data(), sizeof(uint32_t) * n);
-    up(b0.rng, rng.data(), sizeof(uint4) * n);
-    up(b0.init, init.data(), sizeof(uint32_t) * n);
-    up(b0.ncount, zeros.data(), sizeof(uint32_t) * n);
-    up(b0.perm, perm.data(), sizeof(uint32_t) * n);
     if (rc == MSIM_OK) {
         hipError_t e = hipMemset(g.cnt, 0, sizeof(uint32_t) * g.m);
         if (e == hipSuccess) e = hipMemset(s->ctr, 0, sizeof(Counters));
@@ -792,6 +773,11 @@ extern "C" int msim_create(const msim_road* roads, uint32_t n_roads, const uint3
     if (rc == MSIM_OK) {
         hipError_t e = hipStreamSynchronize(nullptr);
         if (e != hipSuccess) rc = fail(s, MSIM_ERR_HIP, "upload failed: %s", hipGetErrorString(e));
+    }
+    if (rc == MSIM_OK && n) {
+        hipLaunchKernelGGL(k_unpack_entities, dim3(agent_blocks(n)), dim3(kBlock), 0, s->stream,
+                           reinterpret_cast<const uint4*>(s->ent_stage), s->agents(0), s->color0);
+        rc = check_launch(s, "k_unpack_entities");
     }
     // Initial binning, then renumber agents into bucket order for locality.
     if (rc == MSIM_OK) rc = enqueue_move(s, kBinOnly, false);
@@ -902,26 +888,16 @@ extern "C" int msim_read_entities(msim_sim* s, msim_entity* out, uint32_t first,
     if (static_cast<uint64_t>(first) + count > s->n) {
         return fail(s, MSIM_ERR_INVALID, "range [%u, %u) exceeds %u agents", first, first + count, s->n);
     }
-    HostState h;
-    int rc = fetch_state(s, h, true, true);
+    if (count == 0) return msim_sync(s);
+    flush_stats(s);
+    hipLaunchKernelGGL(k_pack_entities, dim3(agent_blocks(s->n)), dim3(kBlock), 0, s->stream, s->agents(s->cur),
+                       s->color0, s->any_collide ? 1u : 0u, first, count, reinterpret_cast<uint4*>(s->ent_stage));
+    int rc = check_launch(s, "k_pack_entities");
     if (rc != MSIM_OK) return rc;
-    for (uint32_t k = 0; k < s->n; k++) {
-        const uint32_t i = h.perm[k];
-        if (i < first || i >= first + count) continue;
-        msim_entity& e = out[i - first];
-        if (s->any_collide) {
-            e.color = h.ncount[k] ? msim_rgba{0.f, 0.f, 1.f, 1.f} : msim_rgba{0.f, 1.f, 0.f, 1.f};
-        } else {
-            e.color = s->color0[i];
-        }
-        e.randomState = msim_vec4u{h.rng[k].x, h.rng[k].y, h.rng[k].z, h.rng[k].w};
-        e.pos = msim_vec2{h.pos[k].x, h.pos[k].y};
-        e.target = msim_vec2{h.tgt[k].x, h.tgt[k].y};
-        e.direction = msim_vec2{h.dir[k].x, h.dir[k].y};
-        e.roadIndex = h.road[k];
-        e.initialized = h.init[k];
-    }
-    return MSIM_OK;
+    MSIM_TRY_HIP(s, hipMemcpyAsync(out, s->ent_stage, sizeof(msim_entity) * count, hipMemcpyDeviceToHost,
+                                   s->stream));
+    rc = check_faults(s);
+    return rc;
 }
 
 extern "C" int msim_read_state(msim_sim* s, float* pos, float* target, float* direction,

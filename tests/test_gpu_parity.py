@@ -153,7 +153,9 @@ def test_entity_readback_layout_and_colours():
     sc = S.config(1, scale=0.1)
     g = msim.Simulation(sc.roads, sc.conns, sc.entities, sc.pc, speed=sc.speed)
     e0 = g.entities()
-    np.testing.assert_array_equal(e0["color"], sc.entities["color"])  # before any collision dispatch
+    # before any dispatch the records read back exactly as uploaded (device pack of the
+    # device-unpacked state)
+    np.testing.assert_array_equal(e0.view(np.uint8), sc.entities.view(np.uint8))
     assert (e0["initialized"] == 0).all()
     g.run_ticks(5)
     e = g.entities()
@@ -167,6 +169,10 @@ def test_entity_readback_layout_and_colours():
     np.testing.assert_array_equal(e["color"][~blue], np.tile([0, 1, 0, 1], ((~blue).sum(), 1)))
     part = g.entities(10, 7)
     np.testing.assert_array_equal(part.view(np.uint8), e[10:17].view(np.uint8))
+    # after spatial renumbering every few ticks the records still come back in caller order
+    h = msim.Simulation(sc.roads, sc.conns, sc.entities, sc.pc, speed=sc.speed, reorder_interval=3)
+    h.run_ticks(5)
+    np.testing.assert_array_equal(h.entities().view(np.uint8), e.view(np.uint8))
 
 
 def test_cells_export_matches_oracle():
