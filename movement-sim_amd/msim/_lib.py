@@ -65,9 +65,12 @@ class Info(ctypes.Structure):
     ]
 
 
-# Every symbol include/msim/msim.h declares, with its ctypes signature.
+# Every symbol include/msim/msim.h and include/msim/msim_host.h declare, with
+# its ctypes signature.
 _VP = ctypes.c_void_p
 _U32 = ctypes.c_uint32
+_I64 = ctypes.c_int64
+_U64 = ctypes.c_uint64
 SIGNATURES = {
     "msim_create": (ctypes.c_int, [_VP, _U32, _VP, _U32, _VP, _U32, _VP, _VP, _VP, ctypes.POINTER(_VP)]),
     "msim_dispatch": (ctypes.c_int, [_VP, _VP]),
@@ -100,6 +103,36 @@ SIGNATURES = {
     "msim_batch_diff_forward": (ctypes.c_int, [_VP, _VP, ctypes.c_float, ctypes.c_int, _VP]),
     "msim_batch_diff_backward": (ctypes.c_int, [_VP, _VP, _VP, _VP]),
     "msim_batch_diff_read_schedule": (ctypes.c_int, [_VP, _U32, _VP, _VP, _VP]),
+    # host driver (msim_host.h)
+    "msim_host_map_load": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_VP)]),
+    "msim_host_map_info": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP, _VP]),
+    "msim_host_map_read": (ctypes.c_int, [_VP, _VP, _VP]),
+    "msim_host_map_add_guard": (ctypes.c_int, [_VP]),
+    "msim_host_map_free": (None, [_VP]),
+    "msim_host_make_entities": (ctypes.c_int, [_VP, _U32, _U64, _VP]),
+    "msim_tick_rate_new": (_VP, []),
+    "msim_tick_rate_tick_at": (None, [_VP, _I64]),
+    "msim_tick_rate_get": (ctypes.c_double, [_VP]),
+    "msim_tick_rate_free": (None, [_VP]),
+    "msim_tick_history_new": (_VP, []),
+    "msim_tick_history_add": (None, [_VP, _I64]),
+    "msim_tick_history_avg_ns": (_I64, [_VP]),
+    "msim_tick_history_avg_str": (ctypes.c_int, [_VP, ctypes.c_char_p, ctypes.c_size_t]),
+    "msim_tick_history_free": (None, [_VP]),
+    "msim_host_format_time_of_day": (ctypes.c_int, [_U64, ctypes.c_char_p, ctypes.c_size_t]),
+    "msim_host_sim_create": (ctypes.c_int, [ctypes.c_char_p, _U32, ctypes.c_float, _U64, ctypes.c_char_p,
+                                            ctypes.c_int, _U64, ctypes.POINTER(_VP)]),
+    "msim_host_sim_start_worker": (ctypes.c_int, [_VP]),
+    "msim_host_sim_stop_worker": (ctypes.c_int, [_VP]),
+    "msim_host_sim_continue": (ctypes.c_int, [_VP]),
+    "msim_host_sim_pause": (ctypes.c_int, [_VP]),
+    "msim_host_sim_is_simulating": (ctypes.c_int, [_VP]),
+    "msim_host_sim_wait_ticks": (ctypes.c_int, [_VP, _U64, _U32]),
+    "msim_host_sim_stats": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP, _VP]),
+    "msim_host_sim_take_entities": (ctypes.c_int, [_VP, _VP, _U32, _VP]),
+    "msim_host_sim_handle": (_VP, [_VP]),
+    "msim_host_sim_destroy": (ctypes.c_int, [_VP]),
+    "msim_host_last_error": (ctypes.c_char_p, []),
 }
 
 _lib = None
