@@ -446,25 +446,68 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-constexpr uint32_t kStage = 128;  // binned positions staged per wave round (16 B per lane)
+constexpr uint32_t kStage = 384;  // binned positions staged per wave round (3 x 16 B per lane, 3 KB)
+constexpr uint32_t kStageLoads = kStage / 128;
+
+// Pairs of staged candidates [j0, j1) of one lane within range of `me`; two
+// LDS reads are issued before either is used.
+__device__ __forceinline__ uint32_t count_staged(const float2* stage, uint32_t j0, uint32_t j1, float2 me,
+                                                 float T) {
+    uint32_t cnt = 0;
+    uint32_t j = j0;
+    for (; j + 2u <= j1; j += 2u) {
+        const float2 q0 = stage[j];
+        const float2 q1 = stage[j + 1u];
+        cnt += (in_range(q0, me, T) ? 1u : 0u) + (in_range(q1, me, T) ? 1u : 0u);
+    }
+    if (j < j1) {
+        cnt += in_range(stage[j], me, T) ? 1u : 0u;
+    }
+    return cnt;
+}
+
+// Stages up to kStage binned positions: candidate c (even) of the round
+// comes from slot a0 + c (c < off1), a1 + c - off1 (c < off2) or a2 + c -
+// off2.  Every lane issues its kStageLoads 16-byte loads and LDS writes
+// unconditionally (past U they re-read slot a0 and fill the unused rest of
+// the kStage-slot area), all loads before the first write.  A load may read
+// the slot after a row's end (at most slot n, inside the n + 2 allocation);
+// that candidate is never scanned.
+__device__ __forceinline__ void stage_rows(float2* stage, const float2* __restrict__ bpos, uint32_t U, uint32_t a0,
+                                           uint32_t a1, uint32_t a2, uint32_t off1, uint32_t off2) {
+    const uint32_t lane = __lane_id();
+    float4 v[kStageLoads];
+#pragma unroll
+    for (uint32_t u = 0; u < kStageLoads; u++) {
+        const uint32_t c = 2u * (u * 64u + lane);
+        const uint32_t cc = c < U ? c : 0u;
+        const uint32_t slot = cc < off1 ? a0 + cc : (cc < off2 ? a1 + (cc - off1) : a2 + (cc - off2));
+        v[u] = *reinterpret_cast<const float4*>(bpos + slot);
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < kStageLoads; u++) {
+        *reinterpret_cast<float4*>(stage + 2u * (u * 64u + lane)) = v[u];
+    }
+}
 
 // count_in_cells for a whole wave whose valid lanes (a prefix, lane 0 valid,
 // the last one last_lane) hold consecutive slots of one bucket table, hence
 // nondecreasing buckets.  Each lane's main range in a row, its columns
 // clipped to the row ([col-1, col+1] within [0, wmask]), then starts and ends
 // no earlier than the previous lane's, so lane 0's start and the last lane's
-// end bound the wave's candidates in that row: they are loaded once per wave,
-// 128 slots per 16-byte-per-lane load, into LDS, and every lane scans its own
-// range there.  Rows where a lane's neighbour row wraps around the table's
-// top or bottom edge, and the wrapped column of lanes in the first or last
-// table column, are read per lane from global memory.  The agent's own slot
-// is included in the count.
+// end bound the wave's candidates in that row.  When the three rows' unions
+// fit kStage slots they are loaded in one round (one 16-byte load per lane
+// per 128 slots, all issued before any is used) into LDS and every lane scans
+// its own ranges there; larger unions are staged row by row, kStage slots a
+// round.  Rows where a lane's neighbour row wraps around the table's top or
+// bottom edge, and the wrapped column of lanes in the first or last table
+// column, are read per lane from global memory.  The agent's own slot is
+// included in the count.
 __device__ __forceinline__ uint32_t wave_count_in_cells(float2 me, bool valid, uint32_t last_lane, int32_t cx,
                                                         int32_t cy, const uint32_t* __restrict__ start,
                                                         uint32_t lw, uint32_t wmask, uint32_t hmask,
                                                         const float2* __restrict__ bpos, float T,
                                                         float2* stage) {
-    const uint32_t lane = __lane_id();
     const uint32_t col = static_cast<uint32_t>(cx) & wmask;
     const uint32_t row = static_cast<uint32_t>(cy) & hmask;
     const uint32_t cl = col ? col - 1u : 0u;
@@ -477,9 +520,13 @@ __device__ __forceinline__ uint32_t wave_count_in_cells(float2 me, bool valid, u
         hi[r] = start[rowb + ch + 1u];
     }
     uint32_t cnt = 0;
-#pragma unroll 1
+    // wave-uniform row blocks: first slot a (even), staged length n (even)
+    uint32_t a[3], n[3];
+#pragma unroll
     for (int r = 0; r < 3; r++) {
         const bool edge = (r == 0 && row == 0u) || (r == 2 && row == hmask);
+        n[r] = 0u;
+        a[r] = 0u;
         if (__ballot(valid && edge) != 0ull) {  // the wave's rows are not monotone here
             if (valid) {
                 for (uint32_t k = lo[r]; k < hi[r]; k++) {
@@ -491,21 +538,40 @@ __device__ __forceinline__ uint32_t wave_count_in_cells(float2 me, bool valid, u
         const uint32_t u_lo = __builtin_amdgcn_readfirstlane(lo[r]);
         const uint32_t u_hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(hi[r]),
                                                                               static_cast<int>(last_lane)));
-        for (uint32_t base = u_lo & ~1u; base < u_hi; base += kStage) {
-            const uint32_t i = base + 2u * lane;
-            if (i < u_hi) {
-                const float4 v = *reinterpret_cast<const float4*>(bpos + i);
-                stage[2u * lane] = make_float2(v.x, v.y);
-                stage[2u * lane + 1u] = make_float2(v.z, v.w);
-            }
-            wave_lds_sync();
-            if (valid) {
-                const uint32_t k1 = min(hi[r], base + kStage);
-                for (uint32_t k = max(lo[r], base); k < k1; k++) {
-                    cnt += in_range(stage[k - base], me, T) ? 1u : 0u;
+        a[r] = u_lo & ~1u;
+        n[r] = (u_hi - a[r] + 1u) & ~1u;
+    }
+    const uint32_t off[3] = {0u, n[0], n[0] + n[1]};
+    const uint32_t U = off[2] + n[2];
+    if (U <= kStage) {
+        stage_rows(stage, bpos, U, a[0], a[1], a[2], off[1], off[2]);
+        wave_lds_sync();
+        if (valid) {
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+                if (n[r]) {
+                    cnt += count_staged(stage, off[r] + lo[r] - a[r], off[r] + hi[r] - a[r], me, T);
                 }
             }
-            wave_lds_sync();
+        }
+        wave_lds_sync();
+    } else {
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+#pragma unroll 1
+            for (uint32_t base = a[r]; base < a[r] + n[r]; base += kStage) {
+                const uint32_t m = min(kStage, a[r] + n[r] - base);
+                stage_rows(stage, bpos, m, base, base, base, kStage, kStage);
+                wave_lds_sync();
+                if (valid) {
+                    const uint32_t k0 = max(lo[r], base);
+                    const uint32_t k1 = min(hi[r], base + m);
+                    if (k0 < k1) {
+                        cnt += count_staged(stage, k0 - base, k1 - base, me, T);
+                    }
+                }
+                wave_lds_sync();
+            }
         }
     }
     if (valid && (col == 0u || col == wmask) && wmask != 0u) {  // the column across the table's edge
