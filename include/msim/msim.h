@@ -53,11 +53,17 @@ typedef struct msim_options {
     uint32_t segment_log2;     /* 0: auto; buckets per rebinning workgroup
                                   (batched: for every scenario, 9..its table) */
     uint32_t binning;          /* 0: segment-owned rebinning (LDS histograms,
-                                  per-segment offsets); 1: global-atomic counting */
+                                  per-segment offsets, count and place passes);
+                                  1: global-atomic counting; 2: segment-owned
+                                  in one pass (count, look-back over the
+                                  earlier segments' totals, place); batched:
+                                  0 or 2                                        */
     uint32_t route_queue;      /* 0: arrivals routed inside the advance kernel;
                                   1: queued and routed by a separate pass       */
     uint32_t collide_mode;     /* 0: each wave stages its candidates in LDS with
-                                  16-byte loads; 1: every lane loads its own     */
+                                  16-byte loads, 2 slots per lane; 1: every
+                                  lane loads its own; 2: staged, 1 slot per
+                                  lane                                          */
 } msim_options;
 
 typedef struct msim_info {
@@ -182,8 +188,8 @@ int msim_destroy(msim_sim* sim);
  * value; pc->tick must be odd (the next dispatch is a move), and only
  * pc->collisionRadius and pc->tick are used.  speed may be NULL (1.4 m/tick for
  * all).  Options: cell_size, reorder_interval (default 20), graph_ticks
- * (default 100), device and watchdog_ms apply; the table layout is chosen per
- * scenario.  msim_batch_last_error(NULL) reports a failed create. */
+ * (default 100), segment_log2, binning (0 or 2), collide_mode, device and
+ * watchdog_ms apply; the table layout is chosen per scenario.  msim_batch_last_error(NULL) reports a failed create. */
 typedef struct msim_batch msim_batch;
 
 int msim_batch_create(uint32_t n_scen, const uint32_t* road_off, const msim_road* roads,
@@ -214,8 +220,9 @@ int msim_batch_read_pairs(msim_batch* b, uint64_t* totals, uint32_t* last);
 int msim_batch_time_ticks(msim_batch* b, uint32_t n_ticks, float* ms_total, float* kernel_ms,
                           uint32_t n_kernel_ms);
 
-/* Name of the k-th kernel of a batched tick, or NULL past the last one. */
-const char* msim_batch_kernel_name(uint32_t k);
+/* Name of the k-th kernel of the handle's batched tick, or NULL past the
+ * last one. */
+const char* msim_batch_kernel_name(const msim_batch* b, uint32_t k);
 
 int msim_batch_destroy(msim_batch* b);
 const char* msim_batch_last_error(const msim_batch* b);

@@ -51,16 +51,25 @@ int benqueue_move(msim_batch* b, bool write_dir) {
     bb.new_bpos = b->bbpos[nb];
     bb.new_bid = b->bbid[nb];
     bb.seg_total = b->seg_total;
+    bb.ticket = b->bin_ticket;
+    bb.status = b->seg_status;
     bb.faults = b->faults;
     bb.inv_cell = b->inv_cell;
+    bb.n_seg = b->n_seg;
     const size_t lds = sizeof(uint32_t) << b->max_seg_log;
     const float2* pos = b->agents(b->cur).pos;
-    bprobe_pre(b);
-    hipLaunchKernelGGL(k_bbin_count, dim3(b->n_seg), dim3(kBinThreads), lds, b->stream, pos, bb);
-    bprobe_post(b);
-    bprobe_pre(b);
-    hipLaunchKernelGGL(k_bbin_place, dim3(b->n_seg), dim3(kBinThreads), lds, b->stream, pos, bb);
-    bprobe_post(b);
+    if (b->binning == 2) {
+        bprobe_pre(b);
+        hipLaunchKernelGGL(k_bbin_fused, dim3(b->n_seg), dim3(kBinThreads), lds, b->stream, pos, bb);
+        bprobe_post(b);
+    } else {
+        bprobe_pre(b);
+        hipLaunchKernelGGL(k_bbin_count, dim3(b->n_seg), dim3(kBinThreads), lds, b->stream, pos, bb);
+        bprobe_post(b);
+        bprobe_pre(b);
+        hipLaunchKernelGGL(k_bbin_place, dim3(b->n_seg), dim3(kBinThreads), lds, b->stream, pos, bb);
+        bprobe_post(b);
+    }
     b->bcur = nb;
     return bcheck(b, "batched move dispatch");
 }
@@ -188,6 +197,12 @@ extern "C" int msim_batch_create(uint32_t n_scen, const uint32_t* road_off, cons
     if (!(radius > 0.0f) || !std::isfinite(radius) || !(radius * radius < 3.0e38f)) {
         return bfail(nullptr, MSIM_ERR_INVALID, "collisionRadius must be positive and finite");
     }
+    if (o.collide_mode > 2) {
+        return bfail(nullptr, MSIM_ERR_INVALID, "collide_mode must be 0, 1 or 2");
+    }
+    if (o.binning != 0 && o.binning != 2) {
+        return bfail(nullptr, MSIM_ERR_INVALID, "batched binning must be 0 (two passes) or 2 (one pass)");
+    }
     const float cell = o.cell_size ? o.cell_size : pow2_cell(radius);
     if (!is_pow2_float(cell) || cell < radius) {
         return bfail(nullptr, MSIM_ERR_INVALID, "cell_size must be a power of two >= collisionRadius");
@@ -228,7 +243,8 @@ extern "C" int msim_batch_create(uint32_t n_scen, const uint32_t* road_off, cons
     b->graph_ticks = o.graph_ticks ? o.graph_ticks : 100u;
     b->reorder_interval = opt ? o.reorder_interval : 20u;
     b->seg_ticks = segment_ticks(b->graph_ticks, b->reorder_interval);
-    b->collide_per_lane = o.collide_mode != 0;
+    b->collide_mode = static_cast<int>(o.collide_mode);
+    b->binning = static_cast<int>(o.binning);
     b->watchdog_ms = o.watchdog_ms ? o.watchdog_ms : 600000;
     b->agent_off_h.assign(agent_off, agent_off + n_scen + 1);
     b->road_off_h.assign(road_off, road_off + n_scen + 1);
@@ -307,6 +323,8 @@ extern "C" int msim_batch_create(uint32_t n_scen, const uint32_t* road_off, cons
     BALLOC(b->blk_scen, bblocks(n));
     BALLOC(b->seg_scen, b->n_seg);
     BALLOC(b->seg_total, b->n_seg);
+    BALLOC(b->bin_ticket, 1);
+    BALLOC(b->seg_status, b->n_seg);
     BALLOC(b->part, static_cast<size_t>(n_scen) * kScenSlots * kScenSlotStride);
     BALLOC(b->part_stats, kSlots * kSlotStride);
     BALLOC(b->totals, n_scen);
@@ -387,6 +405,8 @@ extern "C" int msim_batch_create(uint32_t n_scen, const uint32_t* road_off, cons
         if (e == hipSuccess) e = hipMemset(b->totals, 0, sizeof(unsigned long long) * n_scen);
         if (e == hipSuccess) e = hipMemset(b->last, 0, sizeof(uint32_t) * n_scen);
         if (e == hipSuccess) e = hipMemset(b->faults, 0, sizeof(unsigned int));
+        if (e == hipSuccess) e = hipMemset(b->bin_ticket, 0, sizeof(unsigned long long));
+        if (e == hipSuccess) e = hipMemset(b->seg_status, 0, sizeof(unsigned long long) * b->n_seg);
         if (e == hipSuccess) e = hipMemset(b->qdummy, 0, sizeof(uint32_t) * kQueues * kQueueStride);
         if (e == hipSuccess) e = hipMemset(b->bstart[1], 0, sizeof(uint32_t) * start_len);
         if (e != hipSuccess) rc = bfail(b, MSIM_ERR_HIP, "hipMemset failed: %s", hipGetErrorString(e));
@@ -454,7 +474,11 @@ extern "C" int msim_batch_sync(msim_batch* b) {
     if (e != hipSuccess) return bfail(b, MSIM_ERR_HIP, "fault readback: %s", hipGetErrorString(e));
     int rc = bwait(b);
     if (rc != MSIM_OK) return rc;
-    if (f) return bfail(b, MSIM_ERR_STATE, "device-side check failed (fault bits 0x%x)", f);
+    if (f) {
+        return bfail(b, MSIM_ERR_STATE, "device-side check failed (fault bits 0x%x%s%s)", f,
+                     (f & kFaultBinCount) ? ": binned agent count != agent count" : "",
+                     (f & kFaultLookback) ? ": rebinning look-back timed out" : "");
+    }
     return MSIM_OK;
 }
 
@@ -549,9 +573,10 @@ extern "C" int msim_batch_time_ticks(msim_batch* b, uint32_t n_ticks, float* ms_
             t0 = 1;
         }
         // kernel k of a tick timed alone, the tick's other launches untimed
-        double acc[kBatchKernelCount] = {0, 0, 0, 0, 0};
+        const int nk = batch_kernel_count(b);
+        double acc[kBatchKernelMax] = {0, 0, 0, 0, 0};
         for (uint32_t t = t0; t < n_ticks && rc == MSIM_OK; t++) {
-            const int k = static_cast<int>((t - t0) % kBatchKernelCount);
+            const int k = static_cast<int>((t - t0) % static_cast<uint32_t>(nk));
             b->probe_kernel = k;
             b->probe_ev[0] = ev[0];
             b->probe_ev[1] = ev[1];
@@ -560,10 +585,10 @@ extern "C" int msim_batch_time_ticks(msim_batch* b, uint32_t n_ticks, float* ms_
             if (rc == MSIM_OK) rc = bwait(b);
             float ms = 0.f;
             if (rc == MSIM_OK && hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) {
-                acc[k] += static_cast<double>(ms) * kBatchKernelCount;
+                acc[k] += static_cast<double>(ms) * nk;
             }
         }
-        for (uint32_t k = 0; k < n_kernel_ms && k < static_cast<uint32_t>(kBatchKernelCount); k++) {
+        for (uint32_t k = 0; k < n_kernel_ms && k < static_cast<uint32_t>(nk); k++) {
             kernel_ms[k] = static_cast<float>(acc[k]);
         }
     }
@@ -573,8 +598,9 @@ extern "C" int msim_batch_time_ticks(msim_batch* b, uint32_t n_ticks, float* ms_
     return rc;
 }
 
-extern "C" const char* msim_batch_kernel_name(uint32_t k) {
-    return k < static_cast<uint32_t>(kBatchKernelCount) ? kBatchKernels[k] : nullptr;
+extern "C" const char* msim_batch_kernel_name(const msim_batch* b, uint32_t k) {
+    if (!b || k >= static_cast<uint32_t>(batch_kernel_count(b))) return nullptr;
+    return batch_kernel_name(b, static_cast<int>(k));
 }
 
 extern "C" int msim_batch_destroy(msim_batch* b) {

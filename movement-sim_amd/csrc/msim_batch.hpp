@@ -40,8 +40,11 @@ struct BatchBin {
     float2* new_bpos;
     uint32_t* new_bid;
     uint32_t* seg_total;
+    unsigned long long* ticket;  // k_bbin_fused segment tickets (never reset)
+    unsigned long long* status;  // [n_seg] k_bbin_fused published words
     unsigned int* faults;
     float inv_cell;
+    uint32_t n_seg;
 };
 
 __device__ __forceinline__ TableView scen_view(const ScenDev& sc, const BatchBin& b) {
@@ -82,6 +85,29 @@ static __global__ __launch_bounds__(kBinThreads) void k_bbin_place(const float2*
                       b.new_start + sc.start_base, b.new_bpos, b.new_bid);
 }
 
+// Single-pass rebinning of every scenario's table: segments are handed out
+// by ticket, and each scenario's look-back chain starts at its first segment
+// (prefix = the scenario's first slot).
+static __global__ __launch_bounds__(kBinThreads) void k_bbin_fused(const float2* __restrict__ pos, BatchBin b) {
+    extern __shared__ uint32_t hist[];
+    __shared__ uint32_t lds[kBinThreads / 64];
+    __shared__ uint32_t sh[3];
+    take_segment(b.ticket, b.n_seg, sh);
+    const uint32_t g = sh[0];
+    const ScenDev sc = b.scen[b.seg_scen[g]];
+    uint32_t base, total;
+    bin_fused_segment(pos, scen_view(sc, b), (g - sc.seg_first) << sc.seg_log, hist, lds, &sh[2],
+                      b.new_start + sc.start_base, b.new_bpos, b.new_bid, b.status, g, sc.seg_first, sc.agent_off,
+                      sh[1], b.faults, &base, &total);
+    if (g == sc.seg_first + sc.n_seg - 1 && threadIdx.x == 0) {
+        const uint32_t end = base + total;
+        b.new_start[sc.start_base + sc.m] = end;
+        if (end != sc.agent_off + sc.n) {
+            atomicOr(b.faults, kFaultBinCount);
+        }
+    }
+}
+
 // Table parameters of the scenario holding a block's first slot, and the
 // slot where that scenario ends (read with scalar loads: block-uniform).
 struct BlockScen {
@@ -102,61 +128,17 @@ struct BatchCollide {
     unsigned long long* part;  // [n_scen][kScenSlots * kScenSlotStride]
 };
 
-// Neighbour pass over every binned slot of every scenario, each slot on its
-// own scenario's bucket table (count_in_cells, as in the plain pass).
-template <bool STAGED>
-static __global__ __launch_bounds__(kBlock) void k_bcollide(BatchCollide c) {
-    __shared__ float2 stage_all[STAGED ? kBlock / 64 * kStage : 1];
-    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
-    const uint32_t s = blk * kBlock + threadIdx.x;
-    const bool valid = s < c.n;
-    const BlockScen bs = c.blk[blk];
-    uint32_t cnt = 0;
-    uint32_t sid = bs.sid;
-    const float2 me = valid ? c.bpos[s] : make_float2(0.0f, 0.0f);
-    const int32_t cx = cell_coord(me.x, c.inv_cell);
-    const int32_t cy = cell_coord(me.y, c.inv_cell);
-    if (valid && s >= bs.end) {  // a later scenario starts inside this block
-        sid++;
-        while (s >= c.agent_off[sid + 1]) sid++;
-    }
-    const uint32_t w0 = s - __lane_id();
-    bool done = false;
-    if (STAGED && w0 < c.n) {
-        const uint32_t sid0 = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(sid)));
-        if (__ballot(valid && sid != sid0) == 0ull) {  // the wave lies in one scenario
-            uint32_t base = bs.start_base, lw = bs.lw, wmask = bs.wmask, hmask = bs.hmask;
-            if (sid0 != bs.sid) {
-                const ScenDev& sc = c.scen[sid0];
-                base = sc.start_base;
-                lw = sc.lw;
-                wmask = sc.wmask;
-                hmask = sc.hmask;
-            }
-            cnt = wave_count_in_cells(me, valid, min(63u, c.n - 1u - w0), cx, cy, c.start + base, lw, wmask,
-                                      hmask, c.bpos, c.d2_limit, stage_all + (threadIdx.x >> 6) * kStage);
-            done = true;
-        }
-    }
-    if (valid && !done) {
-        const ScenDev& sc = c.scen[sid];
-        cnt = count_in_cells(me, s, cx, cy, c.start + sc.start_base, sc.lw, sc.wmask, sc.hmask, c.bpos,
-                             c.d2_limit);
-    }
-    if (valid) {
-        cnt -= 1u;  // own slot
-        c.ncount[c.bid[s]] = cnt;
-    }
-    // per-scenario partial sums: one atomic per wave when the wave is one
-    // scenario, per lane at scenario boundaries
+// Per-scenario partial sums of a wave's counts: one atomic per wave when
+// all its valid slots lie in one scenario, one per slot otherwise.
+__device__ __forceinline__ void bpart_add(const BatchCollide& c, uint32_t wave, bool valid, uint32_t sid,
+                                          uint32_t cnt) {
     const unsigned long long act = __ballot(valid);
     const int first = act ? __ffsll(static_cast<long long>(act)) - 1 : 0;
     const uint32_t s0 = __shfl(sid, first, 64);
     const bool one = __ballot(valid && sid != s0) == 0ull;
-    const uint32_t wave = (blk * kBlock + threadIdx.x) >> 6;
     const uint32_t slot = wave % kScenSlots;
     if (one) {
-        unsigned long long v = cnt;
+        unsigned long long v = valid ? cnt : 0u;
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) {
             v += __shfl_xor(v, o, 64);
@@ -167,6 +149,122 @@ static __global__ __launch_bounds__(kBlock) void k_bcollide(BatchCollide c) {
     } else if (valid && cnt) {
         atomicAdd(&c.part[(static_cast<size_t>(sid) * kScenSlots + slot) * kScenSlotStride],
                   static_cast<unsigned long long>(cnt));
+    }
+}
+
+// The scenario of slot s, given the BlockScen of the 256-slot block holding
+// it (a later scenario may start inside the block).
+__device__ __forceinline__ uint32_t slot_scen(const BatchCollide& c, const BlockScen& bs, uint32_t s) {
+    uint32_t sid = bs.sid;
+    if (s >= bs.end) {
+        sid++;
+        while (s >= c.agent_off[sid + 1]) sid++;
+    }
+    return sid;
+}
+
+// Neighbour pass over every binned slot of every scenario, each slot on its
+// own scenario's bucket table, one slot per lane (count_in_cells, as in the
+// plain pass).
+static __global__ __launch_bounds__(kBlock) void k_bcollide(BatchCollide c) {
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t s = blk * kBlock + threadIdx.x;
+    const bool valid = s < c.n;
+    const BlockScen bs = c.blk[blk];
+    uint32_t cnt = 0;
+    const float2 me = valid ? c.bpos[s] : make_float2(0.0f, 0.0f);
+    const int32_t cx = cell_coord(me.x, c.inv_cell);
+    const int32_t cy = cell_coord(me.y, c.inv_cell);
+    const uint32_t sid = valid ? slot_scen(c, bs, s) : bs.sid;
+    if (valid) {
+        const ScenDev& sc = c.scen[sid];
+        cnt = count_in_cells(me, s, cx, cy, c.start + sc.start_base, sc.lw, sc.wmask, sc.hmask, c.bpos,
+                             c.d2_limit) - 1u;  // own slot
+        c.ncount[c.bid[s]] = cnt;
+    }
+    bpart_add(c, (blk * kBlock + threadIdx.x) >> 6, valid, sid, cnt);
+}
+
+// Staged neighbour pass: each wave takes 64 P consecutive slots, P per
+// lane, and runs wave_count_in_cells once per scenario its slots touch
+// (usually one), on that scenario's table; each scenario's partial pair sum
+// takes one atomic per wave (grid: ceil(n / (kBlock P)) workgroups).  At
+// most 64 VGPRs (8 waves per SIMD).
+template <int P>
+static __global__ __launch_bounds__(kBlock, 8) void k_bcollide_staged(BatchCollide c) {
+    __shared__ float2 stage_all[kBlock / 64 * kStage];
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t wave = blk * (kBlock / 64) + (threadIdx.x >> 6);
+    const uint32_t w0 = wave * (64u * P);
+    if (w0 >= c.n) {
+        return;  // wave-uniform: no barrier follows
+    }
+    const float T = c.d2_limit;
+    const uint32_t lane = __lane_id();
+    // the wave's 64 P slots lie in one 256-slot block, whose entry names the
+    // scenario of its first slot and where that scenario ends
+    const BlockScen bs = c.blk[w0 / kBlock];
+    float2 me[P];
+    bool valid[P];
+    int32_t cx[P], cy[P];
+    uint32_t id[P], cnt[P];
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+        const uint32_t s = w0 + 64u * p + lane;
+        valid[p] = s < c.n;
+        me[p] = valid[p] ? c.bpos[s] : make_float2(0.0f, 0.0f);
+        id[p] = valid[p] ? c.bid[s] : 0u;
+        cx[p] = cell_coord(me[p].x, c.inv_cell);
+        cy[p] = cell_coord(me[p].y, c.inv_cell);
+        cnt[p] = 0u;
+    }
+    const uint32_t wend = min(w0 + 64u * P, c.n);
+    uint32_t sid = bs.sid, s_end = bs.end;
+    uint32_t base = bs.start_base, lw = bs.lw, wmask = bs.wmask, hmask = bs.hmask;
+    uint32_t first = w0;
+#pragma unroll 1
+    while (first < wend) {  // one scenario's slots [first, last] per round
+        if (first >= s_end) {
+            do {
+                sid++;
+                s_end = c.agent_off[sid + 1];
+            } while (first >= s_end);
+            const ScenDev& sc = c.scen[sid];
+            base = sc.start_base;
+            lw = sc.lw;
+            wmask = sc.wmask;
+            hmask = sc.hmask;
+        }
+        const uint32_t last = min(wend, s_end) - 1u;
+        bool sub[P];
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+            const uint32_t s = w0 + 64u * p + lane;
+            sub[p] = s >= first && s <= last;
+        }
+        const uint32_t f = first - w0, l = last - w0;
+        wave_count_in_cells<P>(me, sub, f >> 6, f & 63u, l >> 6, l & 63u, cx, cy, c.start + base, lw, wmask, hmask,
+                               c.bpos, T, stage_all + (threadIdx.x >> 6) * kStage, cnt);
+        uint32_t v = 0;
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+            v += sub[p] ? cnt[p] - 1u : 0u;  // own slot
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            v += __shfl_xor(v, o, 64);
+        }
+        if (lane == 0 && v) {
+            atomicAdd(&c.part[(static_cast<size_t>(sid) * kScenSlots + wave % kScenSlots) * kScenSlotStride],
+                      static_cast<unsigned long long>(v));
+        }
+        first = last + 1u;
+    }
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+        if (valid[p]) {
+            c.ncount[id[p]] = cnt[p] - 1u;  // own slot
+        }
     }
 }
 
@@ -225,6 +323,8 @@ struct msim_batch {
     BlockScen* blk_scen = nullptr;
     uint32_t* seg_scen = nullptr;
     uint32_t* seg_total = nullptr;
+    unsigned long long* bin_ticket = nullptr;  // [1]
+    unsigned long long* seg_status = nullptr;  // [n_seg]
     uint32_t* bstart[2] = {nullptr, nullptr};
     float2* bbpos[2] = {nullptr, nullptr};
     uint32_t* bbid[2] = {nullptr, nullptr};
@@ -236,7 +336,8 @@ struct msim_batch {
     unsigned int* faults = nullptr;
     uint32_t* qdummy = nullptr;
     bool first_tick_insert_only = false;
-    bool collide_per_lane = false;
+    int collide_mode = 0;  // 0: staged, 2 slots per lane; 1: per lane; 2: staged, 1 slot per lane
+    int binning = 0;  // 0: k_bbin_count + k_bbin_place; 2: k_bbin_fused
     uint32_t tick = 1;
     uint32_t graph_ticks = 100, seg_ticks = 100;
     uint32_t reorder_interval = 20, since_reorder = 0;
@@ -305,7 +406,12 @@ inline int bcheck(msim_batch* b, const char* what) {
 
 // Launch order of a batched tick, for msim_batch_time_ticks.
 inline const char* const kBatchKernels[] = {"k_advance", "k_bbin_count", "k_bbin_place", "k_bcollide", "k_bfinish"};
-constexpr int kBatchKernelCount = 5;
+inline const char* const kBatchKernelsFused[] = {"k_advance", "k_bbin_fused", "k_bcollide", "k_bfinish"};
+constexpr int kBatchKernelMax = 5;
+inline int batch_kernel_count(const msim_batch* b) { return b->binning == 2 ? 4 : 5; }
+inline const char* batch_kernel_name(const msim_batch* b, int k) {
+    return b->binning == 2 ? kBatchKernelsFused[k] : kBatchKernels[k];
+}
 
 inline void bprobe_pre(msim_batch* b) {
     if (b->probe_kernel == b->launch_no) (void)hipEventRecord(b->probe_ev[0], b->stream);
@@ -353,10 +459,14 @@ inline int benqueue_collide(msim_batch* b) {
     c.d2_limit = b->d2_limit;
     c.part = b->part;
     bprobe_pre(b);
-    if (b->collide_per_lane) {
-        hipLaunchKernelGGL(k_bcollide<false>, dim3(bblocks(b->n)), dim3(kBlock), 0, b->stream, c);
+    if (b->collide_mode == 1) {
+        hipLaunchKernelGGL(k_bcollide, dim3(bblocks(b->n)), dim3(kBlock), 0, b->stream, c);
+    } else if (b->collide_mode == 2) {
+        hipLaunchKernelGGL(k_bcollide_staged<1>, dim3(bblocks(b->n)), dim3(kBlock), 0, b->stream, c);
     } else {
-        hipLaunchKernelGGL(k_bcollide<true>, dim3(bblocks(b->n)), dim3(kBlock), 0, b->stream, c);
+        const uint32_t per = kBlock * 2u;
+        hipLaunchKernelGGL(k_bcollide_staged<2>, dim3(std::max(1u, (b->n + per - 1) / per)), dim3(kBlock), 0,
+                           b->stream, c);
     }
     bprobe_post(b);
     bprobe_pre(b);

@@ -283,6 +283,8 @@ struct BinDev {
     float2* new_bpos;           // [n]
     uint32_t* new_bid;          // [n]
     uint32_t* seg_total;        // [n_seg]
+    unsigned long long* ticket;  // [1] single-pass rebinning: segments handed out in start order
+    unsigned long long* status;  // [n_seg] single-pass rebinning: published totals / prefixes
     Counters* ctr;
     float inv_cell;
     uint32_t lw, wmask, hmask, m;
@@ -415,6 +417,150 @@ __device__ __forceinline__ void bin_place_segment(const float2* __restrict__ pos
     bin_window(t, s0, a0, a1, c0, c1);
     bin_pass<true>(pos, t, s0, a0, a1, cursor, out_pos, out_bid);
     bin_pass<true>(pos, t, s0, c0, c1, cursor, out_pos, out_bid);
+}
+
+// ---- single-pass rebinning: count, decoupled look-back, place -------------
+// One workgroup per segment counts its window into LDS, publishes the
+// segment total, finds the sum of the preceding segments' totals by looking
+// back over their published words, and places its agents with a second
+// pass over the same window (which its own count pass has just brought
+// into the L2 of its XCD).  Segments are handed out by an atomic ticket in
+// workgroup start order, so a predecessor has always started: a workgroup
+// only waits for segments that run or have run, never for later ones.
+// Tickets never reset: launch L's tickets are [L * n_seg, (L + 1) * n_seg),
+// and the launch number (the epoch, 30 bits) tags every published word, so
+// a stale word from an earlier launch is never read as current.
+// Word: [63:34] epoch, [33:32] 1 = segment total only, 2 = inclusive prefix,
+// [31:0] value.
+constexpr uint32_t kEpochMask = (1u << 30) - 1u;
+constexpr uint32_t kSegAggregate = 1u;
+constexpr uint32_t kSegPrefix = 2u;
+constexpr uint32_t kLookbackPolls = 1u << 20;  // about a second of polling, then a fault is raised
+
+__device__ __forceinline__ unsigned long long seg_word(uint32_t epoch, uint32_t flag, uint32_t v) {
+    return (static_cast<unsigned long long>(epoch) << 34) | (static_cast<unsigned long long>(flag) << 32) | v;
+}
+
+__device__ __forceinline__ void seg_publish(unsigned long long* w, unsigned long long v) {
+    __hip_atomic_store(w, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Thread-level: publishes segment g's total and returns `first` plus the
+// totals of segments [g_first, g).  Prefix words hold absolute values
+// (`first` included); segment g_first publishes its prefix at once, so every
+// walk ends there at the latest.
+__device__ __forceinline__ uint32_t seg_lookback(unsigned long long* status, uint32_t g, uint32_t g_first,
+                                                 uint32_t first, uint32_t total, uint32_t epoch,
+                                                 unsigned int* faults) {
+    if (g == g_first) {
+        seg_publish(&status[g], seg_word(epoch, kSegPrefix, first + total));
+        return first;
+    }
+    seg_publish(&status[g], seg_word(epoch, kSegAggregate, total));
+    uint32_t excl = 0;
+    uint32_t k = g - 1u;
+    uint32_t polls = 0;
+    for (;;) {
+        const unsigned long long w = __hip_atomic_load(&status[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t flag = static_cast<uint32_t>(w >> 32) & 3u;
+        if (static_cast<uint32_t>(w >> 34) == epoch && flag != 0u) {
+            excl += static_cast<uint32_t>(w);  // a prefix word is absolute (it includes `first`)
+            if (flag == kSegPrefix) {
+                break;
+            }
+            if (k == g_first) {  // cannot happen: g_first publishes its prefix directly
+                atomicOr(faults, kFaultLookback);
+                break;
+            }
+            k--;
+            continue;
+        }
+        if (++polls >= kLookbackPolls) {
+            atomicOr(faults, kFaultLookback);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    seg_publish(&status[g], seg_word(epoch, kSegPrefix, excl + total));
+    return excl;
+}
+
+// Workgroup-level: the segment's ticket (thread 0 takes it) as (segment,
+// epoch) in sh[0], sh[1].
+__device__ __forceinline__ void take_segment(unsigned long long* ticket, uint32_t n_seg, uint32_t* sh) {
+    if (threadIdx.x == 0) {
+        const unsigned long long t = atomicAdd(ticket, 1ull);
+        sh[0] = static_cast<uint32_t>(t % n_seg);
+        sh[1] = static_cast<uint32_t>(t / n_seg) & kEpochMask;
+    }
+    __syncthreads();
+}
+
+// Count, look back, place.  Writes the segment's absolute bucket offsets to
+// start_out[s0 + j]; *base_out / *total_out receive the segment's first slot
+// and agent count (all threads).
+__device__ __forceinline__ void bin_fused_segment(const float2* __restrict__ pos, const TableView& t, uint32_t s0,
+                                                  uint32_t* hist, uint32_t* lds, uint32_t* sh_base,
+                                                  uint32_t* __restrict__ start_out, float2* __restrict__ out_pos,
+                                                  uint32_t* __restrict__ out_bid, unsigned long long* status,
+                                                  uint32_t g, uint32_t g_first, uint32_t first, uint32_t epoch,
+                                                  unsigned int* faults, uint32_t* base_out, uint32_t* total_out) {
+    const uint32_t tid = threadIdx.x;
+    const uint32_t S = 1u << t.seg_log;
+    for (uint32_t j = tid; j < S; j += kBinThreads) {
+        hist[j] = 0u;
+    }
+    __syncthreads();
+    uint32_t a0, a1, c0, c1;
+    bin_window(t, s0, a0, a1, c0, c1);
+    bin_pass<false>(pos, t, s0, a0, a1, hist, nullptr, nullptr);
+    bin_pass<false>(pos, t, s0, c0, c1, hist, nullptr, nullptr);
+    __syncthreads();
+    const uint32_t per = S / kBinThreads;
+    uint32_t run = 0;
+    for (uint32_t q = 0; q < per; q++) {
+        run += hist[tid * per + q];
+    }
+    uint32_t total = 0;
+    uint32_t excl = block_excl_scan<kBinThreads>(run, lds, total);
+    for (uint32_t q = 0; q < per; q++) {  // counts -> segment-local exclusive offsets
+        const uint32_t v = hist[tid * per + q];
+        hist[tid * per + q] = excl;
+        excl += v;
+    }
+    if (tid == 0) {
+        *sh_base = seg_lookback(status, g, g_first, first, total, epoch, faults);
+    }
+    __syncthreads();
+    const uint32_t base = *sh_base;
+    for (uint32_t j = tid; j < S; j += kBinThreads) {
+        const uint32_t v = base + hist[j];
+        hist[j] = v;
+        start_out[s0 + j] = v;
+    }
+    __syncthreads();
+    bin_pass<true>(pos, t, s0, a0, a1, hist, out_pos, out_bid);
+    bin_pass<true>(pos, t, s0, c0, c1, hist, out_pos, out_bid);
+    *base_out = base;
+    *total_out = total;
+}
+
+// Single-pass rebinning of one table (grid: n_seg workgroups).
+static __global__ __launch_bounds__(kBinThreads) void k_bin_fused(const float2* __restrict__ pos, BinDev b) {
+    extern __shared__ uint32_t hist[];
+    __shared__ uint32_t lds[kBinThreads / 64];
+    __shared__ uint32_t sh[3];
+    take_segment(b.ticket, b.n_seg, sh);
+    const uint32_t seg = sh[0];
+    uint32_t base, total;
+    bin_fused_segment(pos, table_view(b), seg << b.seg_log, hist, lds, &sh[2], b.new_start, b.new_bpos, b.new_bid,
+                      b.status, seg, 0u, 0u, sh[1], &b.ctr->faults, &base, &total);
+    if (seg == b.n_seg - 1 && threadIdx.x == 0) {
+        b.new_start[b.m] = base + total;
+        if (base + total != b.n) {
+            atomicOr(&b.ctr->faults, kFaultBinCount);
+        }
+    }
 }
 
 // Counts the window agents that land in segment blockIdx.x, writes the

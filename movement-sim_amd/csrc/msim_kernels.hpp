@@ -66,7 +66,8 @@ struct Counters {
 };
 
 enum FaultBits : unsigned int {
-    kFaultBinCount = 1u  // binned agents != agents (an agent left its rebinning window)
+    kFaultBinCount = 1u,  // binned agents != agents (an agent left its rebinning window)
+    kFaultLookback = 2u   // a segment's look-back gave up waiting for a predecessor
 };
 
 // Per-dispatch partial sums are accumulated into kSlots 64-bit slots, one per
@@ -418,8 +419,8 @@ __device__ __forceinline__ uint32_t count_in_cells(float2 me, uint32_t self, int
             cnt += (lo[r] + u < hi[r] && in_range(q[r][u], me, T)) ? 1u : 0u;
         }
     }
-#pragma unroll 1
-    for (int g = 0; g < 6; g++) {
+#pragma unroll
+    for (int g = 0; g < 6; g++) {  // unrolled: the row arrays stay in registers
         const int r = g >> 1;
         uint32_t k = (g & 1) ? wlo[r] : lo[r] + kNbBatch;
         const uint32_t h = (g & 1) ? whi[r] : hi[r];
@@ -446,7 +447,7 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-constexpr uint32_t kStage = 384;  // binned positions staged per wave round (3 x 16 B per lane, 3 KB)
+constexpr uint32_t kStage = 512;  // binned positions staged per wave round (4 x 16 B per lane, 4 KB)
 constexpr uint32_t kStageLoads = kStage / 128;
 
 // Pairs of staged candidates [j0, j1) of one lane within range of `me`; two
@@ -490,53 +491,75 @@ __device__ __forceinline__ void stage_rows(float2* stage, const float2* __restri
     }
 }
 
-// count_in_cells for a whole wave whose valid lanes (a prefix, lane 0 valid,
-// the last one last_lane) hold consecutive slots of one bucket table, hence
-// nondecreasing buckets.  Each lane's main range in a row, its columns
-// clipped to the row ([col-1, col+1] within [0, wmask]), then starts and ends
-// no earlier than the previous lane's, so lane 0's start and the last lane's
-// end bound the wave's candidates in that row.  When the three rows' unions
-// fit kStage slots they are loaded in one round (one 16-byte load per lane
-// per 128 slots, all issued before any is used) into LDS and every lane scans
-// its own ranges there; larger unions are staged row by row, kStage slots a
-// round.  Rows where a lane's neighbour row wraps around the table's top or
-// bottom edge, and the wrapped column of lanes in the first or last table
-// column, are read per lane from global memory.  The agent's own slot is
-// included in the count.
-__device__ __forceinline__ uint32_t wave_count_in_cells(float2 me, bool valid, uint32_t last_lane, int32_t cx,
-                                                        int32_t cy, const uint32_t* __restrict__ start,
-                                                        uint32_t lw, uint32_t wmask, uint32_t hmask,
-                                                        const float2* __restrict__ bpos, float T,
-                                                        float2* stage) {
-    const uint32_t col = static_cast<uint32_t>(cx) & wmask;
-    const uint32_t row = static_cast<uint32_t>(cy) & hmask;
-    const uint32_t cl = col ? col - 1u : 0u;
-    const uint32_t ch = col < wmask ? col + 1u : wmask;
-    uint32_t lo[3], hi[3];
+// count_in_cells for a whole wave whose lanes hold P slots each: slot
+// w0 + 64 p + lane in element p.  The valid slots (consecutive slots of one
+// bucket table, the first in element first_p of lane first_lane, the last
+// in element last_p of lane last_lane) have nondecreasing buckets; their
+// counts are added to cnt.  Each slot's main
+// range in a row, its columns clipped to the row ([col-1, col+1] within
+// [0, wmask]), then starts and ends no earlier than the previous slot's, so
+// the first slot's start and the last slot's end bound the wave's
+// candidates in that row.  When the three rows' unions fit kStage slots they
+// are loaded in one round (one 16-byte load per lane per 128 slots, all
+// issued before any is used) into LDS and every lane scans its slots'
+// ranges there; larger unions are staged row by row, kStage slots a round.
+// Rows where a slot's neighbour row wraps around the table's top or bottom
+// edge, and the wrapped column of slots in the first or last table column,
+// are read per lane from global memory.  Each slot's own position is
+// included in its count.
+template <int P>
+__device__ __forceinline__ void wave_count_in_cells(const float2 (&me)[P], const bool (&valid)[P], uint32_t first_p,
+                                                    uint32_t first_lane, uint32_t last_p, uint32_t last_lane,
+                                                    const int32_t (&cx)[P],
+                                                    const int32_t (&cy)[P], const uint32_t* __restrict__ start,
+                                                    uint32_t lw, uint32_t wmask, uint32_t hmask,
+                                                    const float2* __restrict__ bpos, float T, float2* stage,
+                                                    uint32_t (&cnt)[P]) {
+    uint32_t col[P], row[P], lo[P][3], hi[P][3];
 #pragma unroll
-    for (int r = 0; r < 3; r++) {
-        const uint32_t rowb = ((row + static_cast<uint32_t>(r - 1)) & hmask) << lw;
-        lo[r] = start[rowb + cl];
-        hi[r] = start[rowb + ch + 1u];
+    for (int p = 0; p < P; p++) {
+        col[p] = static_cast<uint32_t>(cx[p]) & wmask;
+        row[p] = static_cast<uint32_t>(cy[p]) & hmask;
+        const uint32_t cl = col[p] ? col[p] - 1u : 0u;
+        const uint32_t ch = col[p] < wmask ? col[p] + 1u : wmask;
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            const uint32_t rowb = ((row[p] + static_cast<uint32_t>(r - 1)) & hmask) << lw;
+            lo[p][r] = start[rowb + cl];
+            hi[p][r] = start[rowb + ch + 1u];
+        }
     }
-    uint32_t cnt = 0;
     // wave-uniform row blocks: first slot a (even), staged length n (even)
     uint32_t a[3], n[3];
 #pragma unroll
     for (int r = 0; r < 3; r++) {
-        const bool edge = (r == 0 && row == 0u) || (r == 2 && row == hmask);
+        bool edge = false;
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+            edge = edge || (valid[p] && ((r == 0 && row[p] == 0u) || (r == 2 && row[p] == hmask)));
+        }
         n[r] = 0u;
         a[r] = 0u;
-        if (__ballot(valid && edge) != 0ull) {  // the wave's rows are not monotone here
-            if (valid) {
-                for (uint32_t k = lo[r]; k < hi[r]; k++) {
-                    cnt += in_range(bpos[k], me, T) ? 1u : 0u;
+        if (__ballot(edge) != 0ull) {  // the wave's rows are not monotone here
+#pragma unroll
+            for (int p = 0; p < P; p++) {
+                if (valid[p]) {
+                    for (uint32_t k = lo[p][r]; k < hi[p][r]; k++) {
+                        cnt[p] += in_range(bpos[k], me[p], T) ? 1u : 0u;
+                    }
                 }
             }
             continue;
         }
-        const uint32_t u_lo = __builtin_amdgcn_readfirstlane(lo[r]);
-        const uint32_t u_hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(hi[r]),
+        uint32_t lf = lo[0][r], hl = hi[0][r];
+#pragma unroll
+        for (int p = 1; p < P; p++) {
+            lf = (first_p == static_cast<uint32_t>(p)) ? lo[p][r] : lf;
+            hl = (last_p == static_cast<uint32_t>(p)) ? hi[p][r] : hl;
+        }
+        const uint32_t u_lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(lf),
+                                                                              static_cast<int>(first_lane)));
+        const uint32_t u_hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(hl),
                                                                               static_cast<int>(last_lane)));
         a[r] = u_lo & ~1u;
         n[r] = (u_hi - a[r] + 1u) & ~1u;
@@ -546,11 +569,14 @@ __device__ __forceinline__ uint32_t wave_count_in_cells(float2 me, bool valid, u
     if (U <= kStage) {
         stage_rows(stage, bpos, U, a[0], a[1], a[2], off[1], off[2]);
         wave_lds_sync();
-        if (valid) {
 #pragma unroll
-            for (int r = 0; r < 3; r++) {
-                if (n[r]) {
-                    cnt += count_staged(stage, off[r] + lo[r] - a[r], off[r] + hi[r] - a[r], me, T);
+        for (int p = 0; p < P; p++) {
+            if (valid[p]) {
+#pragma unroll
+                for (int r = 0; r < 3; r++) {
+                    if (n[r]) {
+                        cnt[p] += count_staged(stage, off[r] + lo[p][r] - a[r], off[r] + hi[p][r] - a[r], me[p], T);
+                    }
                 }
             }
         }
@@ -563,39 +589,89 @@ __device__ __forceinline__ uint32_t wave_count_in_cells(float2 me, bool valid, u
                 const uint32_t m = min(kStage, a[r] + n[r] - base);
                 stage_rows(stage, bpos, m, base, base, base, kStage, kStage);
                 wave_lds_sync();
-                if (valid) {
-                    const uint32_t k0 = max(lo[r], base);
-                    const uint32_t k1 = min(hi[r], base + m);
-                    if (k0 < k1) {
-                        cnt += count_staged(stage, k0 - base, k1 - base, me, T);
+#pragma unroll
+                for (int p = 0; p < P; p++) {
+                    if (valid[p]) {
+                        const uint32_t k0 = max(lo[p][r], base);
+                        const uint32_t k1 = min(hi[p][r], base + m);
+                        if (k0 < k1) {
+                            cnt[p] += count_staged(stage, k0 - base, k1 - base, me[p], T);
+                        }
                     }
                 }
                 wave_lds_sync();
             }
         }
     }
-    if (valid && (col == 0u || col == wmask) && wmask != 0u) {  // the column across the table's edge
-        const uint32_t wc = col == 0u ? wmask : 0u;
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+        if (valid[p] && (col[p] == 0u || col[p] == wmask) && wmask != 0u) {  // the column across the table's edge
+            const uint32_t wc = col[p] == 0u ? wmask : 0u;
 #pragma unroll 1
-        for (int r = 0; r < 3; r++) {
-            const uint32_t rowb = ((row + static_cast<uint32_t>(r - 1)) & hmask) << lw;
-            const uint32_t k1 = start[rowb + wc + 1u];
-            for (uint32_t k = start[rowb + wc]; k < k1; k++) {
-                cnt += in_range(bpos[k], me, T) ? 1u : 0u;
+            for (int r = 0; r < 3; r++) {
+                const uint32_t rowb = ((row[p] + static_cast<uint32_t>(r - 1)) & hmask) << lw;
+                const uint32_t k1 = start[rowb + wc + 1u];
+                for (uint32_t k = start[rowb + wc]; k < k1; k++) {
+                    cnt[p] += in_range(bpos[k], me[p], T) ? 1u : 0u;
+                }
             }
         }
     }
-    return cnt;
 }
 
-// Neighbour pass over the binned slots.  STAGED: wave_count_in_cells;
-// otherwise count_in_cells per lane.  SLOW: the initialisation dispatch, where
-// agents with Entity.initialized == 0 do not search (random_move.comp:863-867).
-template <bool SLOW, bool STAGED>
+// Staged neighbour pass (collision dispatch of initialised agents): each
+// wave takes 64 P consecutive slots, P per lane (grid: ceil(n / (kBlock P))
+// workgroups).  At most 64 VGPRs (8 waves per SIMD).
+template <int P>
+static __global__ __launch_bounds__(kBlock, 8) void k_collide_staged(AgentsDev a, GridDev g,
+                                                              const float2* __restrict__ bpos,
+                                                              unsigned long long* __restrict__ part_pairs) {
+    __shared__ uint32_t lds[kBlock / 64];
+    __shared__ float2 stage_all[kBlock / 64 * kStage];
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t w0 = (blk * (kBlock / 64) + (threadIdx.x >> 6)) * (64u * P);
+    const float T = g.d2_limit;
+    float2 me[P];
+    bool valid[P];
+    int32_t cx[P], cy[P];
+    uint32_t id[P], cnt[P];
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+        const uint32_t s = w0 + 64u * p + __lane_id();
+        valid[p] = s < a.n;
+        me[p] = valid[p] ? bpos[s] : make_float2(0.0f, 0.0f);
+        id[p] = valid[p] ? g.bid[s] : 0u;
+        cx[p] = cell_coord(me[p].x, g.inv_cell);
+        cy[p] = cell_coord(me[p].y, g.inv_cell);
+        cnt[p] = 0u;
+    }
+    if (w0 < a.n) {
+        const uint32_t last = min(64u * P, a.n - w0) - 1u;
+        wave_count_in_cells<P>(me, valid, 0u, 0u, last >> 6, last & 63u, cx, cy, g.start, g.lw, g.wmask, g.hmask,
+                               bpos, T, stage_all + (threadIdx.x >> 6) * kStage, cnt);
+    }
+    uint32_t tot = 0;
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+        if (valid[p]) {
+            const uint32_t c = cnt[p] - 1u;  // the agent's own slot passed the predicate
+            a.ncount[id[p]] = c;
+            tot += c;
+        }
+    }
+    const uint32_t bs = block_sum<kBlock>(tot, lds);
+    if (threadIdx.x == 0) {
+        slot_add(part_pairs, blk, kWordPairs, bs);
+    }
+}
+
+// Neighbour pass over the binned slots, one slot per lane: count_in_cells.
+// SLOW: the initialisation dispatch, where agents with Entity.initialized ==
+// 0 do not search (random_move.comp:863-867).
+template <bool SLOW>
 static __global__ __launch_bounds__(kBlock) void k_collide(AgentsDev a, GridDev g, const float2* __restrict__ bpos,
                                                     unsigned long long* __restrict__ part_pairs) {
     __shared__ uint32_t lds[kBlock / 64];
-    __shared__ float2 stage_all[STAGED ? kBlock / 64 * kStage : 1];
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t s = blk * kBlock + threadIdx.x;
     const bool valid = s < a.n;
@@ -605,15 +681,7 @@ static __global__ __launch_bounds__(kBlock) void k_collide(AgentsDev a, GridDev 
     const float2 me = valid ? bpos[s] : make_float2(0.0f, 0.0f);
     const int32_t cx = cell_coord(me.x, g.inv_cell);
     const int32_t cy = cell_coord(me.y, g.inv_cell);
-    if (!SLOW && STAGED) {
-        const uint32_t w0 = s - __lane_id();  // the wave's first slot
-        if (w0 < a.n) {
-            const uint32_t last_lane = min(63u, a.n - 1u - w0);
-            cnt = wave_count_in_cells(me, valid, last_lane, cx, cy, g.start, g.lw, g.wmask, g.hmask, bpos, T,
-                                      stage_all + (threadIdx.x >> 6) * kStage);
-        }
-    }
-    if (!SLOW && !STAGED && valid) {
+    if (!SLOW && valid) {
         cnt = count_in_cells(me, s, cx, cy, g.start, g.lw, g.wmask, g.hmask, bpos, T);
     }
     if (SLOW && valid) {

@@ -61,9 +61,10 @@ struct msim_sim {
     uint32_t* bbid[2] = {nullptr, nullptr};
     int bcur = 0;
     bool stats_pending = false;
-    int binning = 0;  // 0: segment-owned (k_bin_count + k_bin_place), 1: atomic counting
+    int binning = 0;  // 0: segment-owned (k_bin_count + k_bin_place), 1: atomic counting,
+                      // 2: segment-owned in one pass (k_bin_fused, look-back)
     bool route_inline = false;  // arrivals routed inside k_advance instead of k_route
-    bool collide_per_lane = false;  // neighbour pass without wave staging
+    int collide_mode = 0;  // 0: staged, 2 slots per lane; 1: per lane; 2: staged, 1 slot per lane
     // msim_time_ticks probe: launch number probe_kernel of the tick is
     // bracketed by probe_ev[0] / probe_ev[1]; -1 = no probe.
     int probe_kernel = -1;
@@ -71,6 +72,8 @@ struct msim_sim {
     hipEvent_t probe_ev[2] = {nullptr, nullptr};
     uint32_t seg_log = 0, n_seg = 0, margin = 0;
     uint32_t* seg_total = nullptr;  // [n_seg] per-segment agent totals
+    unsigned long long* bin_ticket = nullptr;  // [1] k_bin_fused segment tickets (never reset)
+    unsigned long long* seg_status = nullptr;  // [n_seg] k_bin_fused published words
     // Arrival queues of the fast move dispatch; counters are double-buffered by
     // tick parity (k_advance zeroes next tick's set).
     uint32_t* q_items = nullptr;
@@ -123,7 +126,9 @@ struct msim_sim {
 namespace {
 const char* const kSegKernels[] = {"k_bin_count", "k_bin_place"};
 const char* const kAtomicKernels[] = {"k_scan_reduce", "k_scan_partials", "k_scan_down", "k_scatter"};
-uint32_t kernels_per_tick(const msim_sim* s) { return (s->route_inline ? 1u : 2u) + (s->binning ? 4u : 2u) + 2u; }
+const char* const kFusedKernels[] = {"k_bin_fused"};
+uint32_t bin_kernels(const msim_sim* s) { return s->binning == 1 ? 4u : (s->binning == 2 ? 1u : 2u); }
+uint32_t kernels_per_tick(const msim_sim* s) { return (s->route_inline ? 1u : 2u) + bin_kernels(s) + 2u; }
 // Name of launch k of a fused tick.
 const char* tick_kernel_name(const msim_sim* s, uint32_t k) {
     if (k == 0) return "k_advance";
@@ -132,8 +137,8 @@ const char* tick_kernel_name(const msim_sim* s, uint32_t k) {
         k--;
     }
     k--;
-    const uint32_t nbin = s->binning ? 4u : 2u;
-    if (k < nbin) return s->binning ? kAtomicKernels[k] : kSegKernels[k];
+    const uint32_t nbin = bin_kernels(s);
+    if (k < nbin) return s->binning == 1 ? kAtomicKernels[k] : (s->binning == 2 ? kFusedKernels[k] : kSegKernels[k]);
     k -= nbin;
     return k == 0 ? "k_collide" : "k_collide_finish";
 }
@@ -239,7 +244,7 @@ void launch_advance_ri(msim_sim* s, const QueueDev& q) {
     const AgentsDev a = s->agents(s->cur);
     const GridDev g = s->grid_cur();
     const dim3 nb(agent_blocks(s->n));
-    if (s->binning) {
+    if (s->binning == 1) {
         hipLaunchKernelGGL((k_advance<U, W, true, RI>), nb, dim3(kBlock), 0, s->stream, a, g, q, s->part_stats,
                            s->roads, s->conns);
     } else {
@@ -250,7 +255,7 @@ void launch_advance_ri(msim_sim* s, const QueueDev& q) {
 
 template <bool U, bool W>
 void launch_advance_route(msim_sim* s, const QueueDev& q) {
-    if (s->route_inline && !s->binning) {
+    if (s->route_inline && s->binning != 1) {
         const dim3 nb(std::max(1u, (s->n + kAdv * kBlock - 1) / (kAdv * kBlock)));
         hipLaunchKernelGGL((k_advance4<U, W>), nb, dim3(kBlock), 0, s->stream, s->agents(s->cur),
                            s->grid.inv_cell, s->part_stats, s->roads, s->conns);
@@ -293,7 +298,7 @@ int enqueue_move_fast(msim_sim* s, bool write_dir, hipEvent_t* ev) {
     }
     s->qpar = 1 - s->qpar;
     const float2* pos = s->agents(s->cur).pos;
-    if (s->binning == 0) {
+    if (s->binning != 1) {
         const int nb = 1 - s->bcur;
         BinDev b{};
         b.old_start = s->bstart[s->bcur];
@@ -302,6 +307,8 @@ int enqueue_move_fast(msim_sim* s, bool write_dir, hipEvent_t* ev) {
         b.new_bpos = s->bbpos[nb];
         b.new_bid = s->bbid[nb];
         b.seg_total = s->seg_total;
+        b.ticket = s->bin_ticket;
+        b.status = s->seg_status;
         b.ctr = s->ctr;
         b.inv_cell = s->grid.inv_cell;
         b.lw = s->grid.lw;
@@ -313,12 +320,18 @@ int enqueue_move_fast(msim_sim* s, bool write_dir, hipEvent_t* ev) {
         b.margin = s->margin;
         b.n = s->n;
         const size_t lds = sizeof(uint32_t) << s->seg_log;
-        probe_pre(s);
-        hipLaunchKernelGGL(k_bin_count, dim3(s->n_seg), dim3(kBinThreads), lds, s->stream, pos, b);
-        probe_post(s);
-        probe_pre(s);
-        hipLaunchKernelGGL(k_bin_place, dim3(s->n_seg), dim3(kBinThreads), lds, s->stream, pos, b);
-        probe_post(s);
+        if (s->binning == 2) {
+            probe_pre(s);
+            hipLaunchKernelGGL(k_bin_fused, dim3(s->n_seg), dim3(kBinThreads), lds, s->stream, pos, b);
+            probe_post(s);
+        } else {
+            probe_pre(s);
+            hipLaunchKernelGGL(k_bin_count, dim3(s->n_seg), dim3(kBinThreads), lds, s->stream, pos, b);
+            probe_post(s);
+            probe_pre(s);
+            hipLaunchKernelGGL(k_bin_place, dim3(s->n_seg), dim3(kBinThreads), lds, s->stream, pos, b);
+            probe_post(s);
+        }
         s->bcur = nb;
     } else {
         const GridDev g = s->grid_cur();
@@ -392,7 +405,7 @@ int enqueue_collide(msim_sim* s, bool slow, hipEvent_t* ev = nullptr) {
     s->stats_pending = false;
     if (slow) {
         probe_pre(s);
-        hipLaunchKernelGGL((k_collide<true, false>), g, dim3(kBlock), 0, s->stream, a, s->grid_cur(),
+        hipLaunchKernelGGL((k_collide<true>), g, dim3(kBlock), 0, s->stream, a, s->grid_cur(),
                            static_cast<const float2*>(s->bbpos[s->bcur]), s->part_pairs);
         probe_post(s);
         probe_pre(s);
@@ -400,14 +413,18 @@ int enqueue_collide(msim_sim* s, bool slow, hipEvent_t* ev = nullptr) {
                            s->hist, s->hist_cap - 1, s->part_stats, fold);
         hipLaunchKernelGGL(k_set_init, g, dim3(kBlock), 0, s->stream, a.init, s->n);
     } else {
-        if (s->collide_per_lane) {
-            probe_pre(s);
-            hipLaunchKernelGGL((k_collide<false, false>), g, dim3(kBlock), 0, s->stream, a, s->grid_cur(),
+        probe_pre(s);
+        if (s->collide_mode == 1) {
+            hipLaunchKernelGGL((k_collide<false>), g, dim3(kBlock), 0, s->stream, a, s->grid_cur(),
+                               static_cast<const float2*>(s->bbpos[s->bcur]), s->part_pairs);
+        } else if (s->collide_mode == 2) {
+            hipLaunchKernelGGL(k_collide_staged<1>, g, dim3(kBlock), 0, s->stream, a, s->grid_cur(),
                                static_cast<const float2*>(s->bbpos[s->bcur]), s->part_pairs);
         } else {
-            probe_pre(s);
-            hipLaunchKernelGGL((k_collide<false, true>), g, dim3(kBlock), 0, s->stream, a, s->grid_cur(),
-                               static_cast<const float2*>(s->bbpos[s->bcur]), s->part_pairs);
+            const uint32_t per = kBlock * 2u;
+            hipLaunchKernelGGL(k_collide_staged<2>, dim3(std::max(1u, (s->n + per - 1) / per)), dim3(kBlock), 0,
+                               s->stream, a, s->grid_cur(), static_cast<const float2*>(s->bbpos[s->bcur]),
+                               s->part_pairs);
         }
         probe_post(s);
         probe_pre(s);
@@ -627,8 +644,11 @@ extern "C" int msim_create(const msim_road* roads, uint32_t n_roads, const uint3
     if (seg_log < 9 || seg_log > 14 || seg_log > tlog) {
         return fail(nullptr, MSIM_ERR_INVALID, "segment_log2 must be in [9, 14] and <= table_log2");
     }
-    if (o.binning > 1) {
-        return fail(nullptr, MSIM_ERR_INVALID, "binning must be 0 (segment) or 1 (atomic)");
+    if (o.collide_mode > 2) {
+        return fail(nullptr, MSIM_ERR_INVALID, "collide_mode must be 0, 1 or 2");
+    }
+    if (o.binning > 2) {
+        return fail(nullptr, MSIM_ERR_INVALID, "binning must be 0 (segment), 1 (atomic) or 2 (segment, one pass)");
     }
     if (!validate_extent(roads, n_roads, entities, n_entities, 1.0f / cell, &why)) {
         return fail(nullptr, MSIM_ERR_INVALID, "%s", why.c_str());
@@ -685,7 +705,7 @@ extern "C" int msim_create(const msim_road* roads, uint32_t n_roads, const uint3
     g.hmask = (1u << (tlog - g.lw)) - 1u;
     s->binning = static_cast<int>(o.binning);
     s->route_inline = o.route_queue == 0;
-    s->collide_per_lane = o.collide_mode != 0;
+    s->collide_mode = static_cast<int>(o.collide_mode);
     s->seg_log = seg_use;
     s->n_seg = g.m >> seg_use;
     // An agent moves at most max_speed per axis per tick (the heading has
@@ -725,6 +745,8 @@ extern "C" int msim_create(const msim_road* roads, uint32_t n_roads, const uint3
         ALLOC(s->bbid[k], n);
     }
     ALLOC(s->seg_total, s->n_seg);
+    ALLOC(s->bin_ticket, 1);
+    ALLOC(s->seg_status, s->n_seg);
     s->q_cap = ceil_div(agent_blocks(n), kQueues) * kBlock;
     s->q_blocks = std::min(64u, std::max(1u, ceil_div(n, kQueues * kBlock * 16u)));
     ALLOC(s->q_items, static_cast<size_t>(kQueues) * s->q_cap);
@@ -758,6 +780,8 @@ extern "C" int msim_create(const msim_road* roads, uint32_t n_roads, const uint3
     if (rc == MSIM_OK) {
         hipError_t e = hipMemset(g.cnt, 0, sizeof(uint32_t) * g.m);
         if (e == hipSuccess) e = hipMemset(s->ctr, 0, sizeof(Counters));
+        if (e == hipSuccess) e = hipMemset(s->bin_ticket, 0, sizeof(unsigned long long));
+        if (e == hipSuccess) e = hipMemset(s->seg_status, 0, sizeof(unsigned long long) * s->n_seg);
         if (e == hipSuccess) e = hipMemset(s->q_count, 0, sizeof(uint32_t) * 2 * kQueues * kQueueStride);
         if (e == hipSuccess) e = hipMemset(s->part_stats, 0, sizeof(unsigned long long) * kSlots * kSlotStride);
         if (e == hipSuccess) e = hipMemset(s->part_pairs, 0, sizeof(unsigned long long) * kSlots * kSlotStride);
@@ -847,8 +871,9 @@ int check_faults(msim_sim* s) {
     int rc = wait_stream(s);
     if (rc != MSIM_OK) return rc;
     if (c.faults) {
-        return fail(s, MSIM_ERR_STATE, "device-side check failed (fault bits 0x%x%s)", c.faults,
-                    (c.faults & kFaultBinCount) ? ": binned agent count != agent count" : "");
+        return fail(s, MSIM_ERR_STATE, "device-side check failed (fault bits 0x%x%s%s)", c.faults,
+                    (c.faults & kFaultBinCount) ? ": binned agent count != agent count" : "",
+                    (c.faults & kFaultLookback) ? ": rebinning look-back timed out" : "");
     }
     return MSIM_OK;
 }

@@ -96,7 +96,7 @@ SIGNATURES = {
     "msim_batch_read_state": (ctypes.c_int, [_VP, _U32, _VP, _VP, _VP, _VP, _VP, _VP]),
     "msim_batch_read_pairs": (ctypes.c_int, [_VP, _VP, _VP]),
     "msim_batch_time_ticks": (ctypes.c_int, [_VP, _U32, _VP, _VP, _U32]),
-    "msim_batch_kernel_name": (ctypes.c_char_p, [_U32]),
+    "msim_batch_kernel_name": (ctypes.c_char_p, [_VP, _U32]),
     "msim_batch_destroy": (ctypes.c_int, [_VP]),
     "msim_batch_last_error": (ctypes.c_char_p, [_VP]),
     "msim_batch_diff_setup": (ctypes.c_int, [_VP, _U32, _U32, ctypes.c_int]),
@@ -398,7 +398,7 @@ class Batch:
     def kernel_names(self):
         out = []
         while True:
-            nm = self.lib.msim_batch_kernel_name(len(out))
+            nm = self.lib.msim_batch_kernel_name(self.h, len(out))
             if not nm:
                 return out
             out.append(nm.decode())

@@ -11,7 +11,8 @@ a JSON file keyed by the workload's shape.
     python -m msim.autotune --batch             (the 512-scenario batched workload)
 
 The batched handle's knobs: rebinning segment size (for every scenario),
-reorder interval, ticks per captured graph segment and the neighbour walk.
+reorder interval, ticks per captured graph segment, the neighbour walk and
+the rebinning mode (two passes or one).
 """
 import argparse
 import json
@@ -29,10 +30,10 @@ CACHE = os.path.join(HERE, "tuned.json")
 SPACE = {
     "table_width_log2": [0, 5, 6, 7, 8, 9, 10, 11, 12],
     "segment_log2": [0, 9, 10, 11, 12, 13, 14],
-    "binning": [0, 1],
+    "binning": [0, 1, 2],
     "route_queue": [0, 1],
     "reorder_interval": [8, 16, 32, 64],
-    "collide_mode": [0, 1],
+    "collide_mode": [0, 1, 2],
 }
 DEFAULT = {"table_width_log2": 0, "segment_log2": 0, "binning": 0, "route_queue": 0,
            "reorder_interval": 16, "collide_mode": 0}
@@ -42,9 +43,10 @@ BATCH_SPACE = {
     "segment_log2": [0, 11, 12, 13],
     "reorder_interval": [10, 20, 25, 50],
     "graph_ticks": [50, 100, 200],
-    "collide_mode": [0, 1],
+    "collide_mode": [0, 1, 2],
+    "binning": [0, 2],
 }
-BATCH_DEFAULT = {"segment_log2": 0, "reorder_interval": 20, "graph_ticks": 100, "collide_mode": 0}
+BATCH_DEFAULT = {"segment_log2": 0, "reorder_interval": 20, "graph_ticks": 100, "collide_mode": 0, "binning": 0}
 
 
 def batch_key(scs):

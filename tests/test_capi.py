@@ -88,7 +88,7 @@ def test_simulation_raises_on_invalid():
         msim.Simulation(roads, conns[:-1], S.reference_agents(roads, 4, 2), msim.push_consts(w, h, 10.0))
 
 
-def _batch_create(scs, pc):
+def _batch_create(scs, pc, opts=None):
     lib = msim.load_library()
     offs = [np.concatenate([[0], np.cumsum([getattr(s, f).shape[0] for s in scs])]).astype(np.uint32)
             for f in ("roads", "conns", "entities")]
@@ -97,7 +97,8 @@ def _batch_create(scs, pc):
     e = np.concatenate([s.entities for s in scs])
     h = ctypes.c_void_p()
     rc = lib.msim_batch_create(len(scs), _lib._ptr(offs[0]), _lib._ptr(r), _lib._ptr(offs[1]), _lib._ptr(c),
-                               _lib._ptr(offs[2]), _lib._ptr(e), None, _lib._ptr(pc), None, ctypes.byref(h))
+                               _lib._ptr(offs[2]), _lib._ptr(e), None, _lib._ptr(pc),
+                               None if opts is None else ctypes.byref(opts), ctypes.byref(h))
     return rc, lib.msim_batch_last_error(None).decode()
 
 
@@ -127,6 +128,12 @@ def test_batch_rejects_mixed_init_and_even_tick():
     pc["tick"] = 2
     rc, msg = _batch_create(scs, pc)
     assert rc == 1 and "tick" in msg
+
+
+def test_batch_rejects_atomic_binning():
+    scs, pc = _star_scenarios(2)
+    rc, msg = _batch_create(scs, pc, _lib.make_options(binning=1))
+    assert rc == 1 and "binning" in msg
 
 
 def test_batch_binding_rejects_mixed_speed_modes():

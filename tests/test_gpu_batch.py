@@ -91,8 +91,25 @@ def test_batch_ragged_empty_and_heavy_cells():
         assert_state_equal(b.state(s), oracle_state(o), f"star scenario {s} vs oracle")
 
 
+@pytest.mark.parametrize("mode", [0, 2])
+def test_batch_many_tiny_scenarios_share_waves(mode):
+    """60 scenarios of 0 to 9 agents: most waves of the neighbour pass span
+    several scenarios (and skip empty ones)."""
+    scs = [_star(k % 10, 100 + k) for k in range(60)]
+    check_against_plain(scs, 120, collide_mode=mode)
+
+
+@pytest.mark.parametrize("opts", [dict(binning=2), dict(collide_mode=1), dict(collide_mode=2)])
+def test_batch_ragged_empty_and_heavy_cells_other_modes(opts):
+    scs = [_star(64, 3), _star(0, 4), _star(1, 5), _star(700, 6), _star(33, 7)]
+    check_against_plain(scs, 300, **opts)
+
+
 @pytest.mark.parametrize("opts", [dict(reorder_interval=7, graph_ticks=10), dict(reorder_interval=0),
-                                  dict(reorder_interval=1, graph_ticks=3), dict(graph_ticks=1)])
+                                  dict(reorder_interval=1, graph_ticks=3), dict(graph_ticks=1),
+                                  dict(binning=2), dict(binning=2, reorder_interval=1, graph_ticks=3),
+                                  dict(binning=2, segment_log2=9, graph_ticks=1),
+                                  dict(collide_mode=1), dict(collide_mode=2, binning=2)])
 def test_batch_options_do_not_change_results(opts):
     scs = S.batch_config(5, scale=0.01, seed=5)
     check_against_plain(scs, 64, **opts)
@@ -105,17 +122,19 @@ def test_batch_initialised_agents_move_on_the_first_tick():
     check_against_plain(scs, 41)
 
 
-def test_batch_repeated_runs_and_time_ticks():
+@pytest.mark.parametrize("binning", [0, 2])
+def test_batch_repeated_runs_and_time_ticks(binning):
     """run_ticks in pieces equals one run; time_ticks advances the state."""
     scs = S.batch_config(4, scale=0.01, seed=21)
-    a = msim.Batch(scs)
-    b = msim.Batch(scs)
+    a = msim.Batch(scs, binning=binning)
+    b = msim.Batch(scs, binning=binning)
     for k in (1, 5, 100, 2, 133):
         a.run_ticks(k)
     ms, _ = b.time_ticks(141)
     assert ms > 0.0
     ms, kms = b.time_ticks(50, kernels=True)
     assert ms > 0.0 and set(kms) == set(b.kernel_names()) and all(v > 0.0 for v in kms.values())
+    assert ("k_bbin_fused" in kms) == (binning == 2)
     for s in range(len(scs)):
         assert_state_equal(a.state(s), b.state(s), f"scenario {s}")
     np.testing.assert_array_equal(a.pairs()[0], b.pairs()[0])

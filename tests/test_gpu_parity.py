@@ -96,7 +96,11 @@ def test_dispatch_loop_equals_fused_graph_path():
                                   dict(segment_log2=9, table_width_log2=3),
                                   dict(table_log2=16, segment_log2=13, table_width_log2=8),
                                   dict(route_queue=1), dict(collide_mode=1),
-                                  dict(route_queue=1, binning=1, collide_mode=1)])
+                                  dict(route_queue=1, binning=1, collide_mode=1),
+                                  dict(binning=2), dict(binning=2, reorder_interval=5, segment_log2=9),
+                                  dict(binning=2, table_log2=16, segment_log2=13, table_width_log2=8),
+                                  dict(binning=2, route_queue=1, graph_ticks=1),
+                                  dict(collide_mode=2), dict(collide_mode=2, segment_log2=9, table_width_log2=3)])
 def test_options_do_not_change_results(opts):
     sc = S.config(1, scale=0.5)
     g, o = pair(sc, **opts)
@@ -147,6 +151,25 @@ def test_config2_full_size_parity():
     assert_same(g.state(), o.st, "config 2 full")
     total, hist = g.pairs()
     assert total == o.pairs_total and hist.tolist() == o.pair_hist
+
+
+def test_one_pass_binning_equals_two_pass_at_full_size():
+    """1,000,000 agents (512 rebinning segments chained by look-back,
+    across every XCD), 200 ticks: the one-pass rebinning gives the state and
+    pair history of the two-pass one."""
+    sc = S.config(2)
+    a = msim.Simulation(sc.roads, sc.conns, sc.entities, sc.pc, speed=sc.speed)
+    b = msim.Simulation(sc.roads, sc.conns, sc.entities, sc.pc, speed=sc.speed, binning=2)
+    assert b.info()["binning"] == 2
+    a.run_ticks(200)
+    b.run_ticks(200)
+    sa, sb = a.state(), b.state()
+    for k in sa:
+        np.testing.assert_array_equal(sa[k], sb[k], err_msg=k)
+    (ta, ha), (tb, hb) = a.pairs(), b.pairs()
+    assert ta == tb
+    np.testing.assert_array_equal(ha, hb)
+    assert "k_bin_fused" in b.kernel_names()
 
 
 def test_entity_readback_layout_and_colours():
