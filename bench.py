@@ -77,8 +77,9 @@ def plain_line(args, sc):
 
     n = sc.entities.shape[0]
     opts = dict(autotune.DEFAULT) if args.untuned else autotune.tuned_options(sc)
-    if args.binning is not None:
-        opts["binning"] = args.binning
+    for k in ("binning", "collide_mode"):
+        if getattr(args, k) is not None:
+            opts[k] = getattr(args, k)
     sim = msim.Simulation(sc.roads, sc.conns, sc.entities, sc.pc, speed=sc.speed, **opts)
     info = sim.info()
 
@@ -197,8 +198,9 @@ def batched_line(args, scs):
         v = getattr(args, "batch_" + k)
         if v is not None:
             opts[k] = v
-    if args.binning is not None:
-        opts["binning"] = args.binning
+    for k in ("binning", "collide_mode"):
+        if getattr(args, k) is not None:
+            opts[k] = getattr(args, k)
     b = msim.Batch(scs, **opts)
     extra = {}
     if args.compare_ticks > 0:
@@ -284,6 +286,8 @@ def main():
     ap.add_argument("--batch-segment-log2", type=int, default=None, help="override the tuned value")
     ap.add_argument("--binning", type=int, default=None, help="override the tuned rebinning mode (0 two passes, "
                                                                "1 atomic (plain only), 2 one pass)")
+    ap.add_argument("--collide-mode", type=int, default=None, help="override the tuned neighbour walk (0 staged, "
+                                                                    "2 slots per lane; 1 per lane; 2 staged, 1 slot)")
     ap.add_argument("--profile-ticks", type=int, default=240)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")

@@ -64,6 +64,9 @@ typedef struct msim_options {
                                   16-byte loads, 2 slots per lane; 1: every
                                   lane loads its own; 2: staged, 1 slot per
                                   lane                                          */
+    uint32_t bin_unroll;       /* window slots each rebinning thread loads per
+                                  round: 4 or 8 (0: 8 for segments of up to
+                                  2^13 buckets, else 4)                         */
 } msim_options;
 
 typedef struct msim_info {

@@ -58,17 +58,10 @@ int benqueue_move(msim_batch* b, bool write_dir) {
     bb.n_seg = b->n_seg;
     const size_t lds = sizeof(uint32_t) << b->max_seg_log;
     const float2* pos = b->agents(b->cur).pos;
-    if (b->binning == 2) {
-        bprobe_pre(b);
-        hipLaunchKernelGGL(k_bbin_fused, dim3(b->n_seg), dim3(kBinThreads), lds, b->stream, pos, bb);
-        bprobe_post(b);
+    if (b->bin_unroll == 8) {
+        blaunch_bins<8>(b, pos, bb, lds);
     } else {
-        bprobe_pre(b);
-        hipLaunchKernelGGL(k_bbin_count, dim3(b->n_seg), dim3(kBinThreads), lds, b->stream, pos, bb);
-        bprobe_post(b);
-        bprobe_pre(b);
-        hipLaunchKernelGGL(k_bbin_place, dim3(b->n_seg), dim3(kBinThreads), lds, b->stream, pos, bb);
-        bprobe_post(b);
+        blaunch_bins<4>(b, pos, bb, lds);
     }
     b->bcur = nb;
     return bcheck(b, "batched move dispatch");
@@ -200,6 +193,9 @@ extern "C" int msim_batch_create(uint32_t n_scen, const uint32_t* road_off, cons
     if (o.collide_mode > 2) {
         return bfail(nullptr, MSIM_ERR_INVALID, "collide_mode must be 0, 1 or 2");
     }
+    if (o.bin_unroll != 0 && o.bin_unroll != 4 && o.bin_unroll != 8) {
+        return bfail(nullptr, MSIM_ERR_INVALID, "bin_unroll must be 0, 4 or 8");
+    }
     if (o.binning != 0 && o.binning != 2) {
         return bfail(nullptr, MSIM_ERR_INVALID, "batched binning must be 0 (two passes) or 2 (one pass)");
     }
@@ -295,6 +291,7 @@ extern "C" int msim_batch_create(uint32_t n_scen, const uint32_t* road_off, cons
         b->max_seg_log = std::max(b->max_seg_log, seg_log);
     }
     b->n_seg = static_cast<uint32_t>(seg_scen.size());
+    b->bin_unroll = o.bin_unroll ? o.bin_unroll : auto_bin_unroll(b->max_seg_log);
     b->start_len = start_len;
 
     int rc = MSIM_OK;

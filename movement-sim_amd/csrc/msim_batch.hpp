@@ -52,19 +52,21 @@ __device__ __forceinline__ TableView scen_view(const ScenDev& sc, const BatchBin
                      sc.seg_log, sc.margin};
 }
 
+template <uint32_t U>
 static __global__ __launch_bounds__(kBinThreads) void k_bbin_count(const float2* __restrict__ pos, BatchBin b) {
     extern __shared__ uint32_t hist[];
     __shared__ uint32_t lds[kBinThreads / 64];
     const uint32_t g = blockIdx.x;
     const ScenDev sc = b.scen[b.seg_scen[g]];
     const uint32_t s0 = (g - sc.seg_first) << sc.seg_log;
-    const uint32_t total = bin_count_segment(pos, scen_view(sc, b), s0, hist, lds, b.new_start + sc.start_base);
+    const uint32_t total = bin_count_segment<U>(pos, scen_view(sc, b), s0, hist, lds, b.new_start + sc.start_base);
     if (threadIdx.x == 0) {
         b.seg_total[g] = total;
     }
 }
 
-static __global__ __launch_bounds__(kBinThreads) void k_bbin_place(const float2* __restrict__ pos, BatchBin b) {
+template <uint32_t U>
+static __global__ __launch_bounds__(kBinThreads, 8) void k_bbin_place(const float2* __restrict__ pos, BatchBin b) {
     extern __shared__ uint32_t cursor[];
     __shared__ uint32_t lds[kBinThreads / 64];
     const uint32_t g = blockIdx.x;
@@ -81,14 +83,15 @@ static __global__ __launch_bounds__(kBinThreads) void k_bbin_place(const float2*
             atomicOr(b.faults, kFaultBinCount);
         }
     }
-    bin_place_segment(pos, scen_view(sc, b), (g - sc.seg_first) << sc.seg_log, base, cursor,
+    bin_place_segment<U>(pos, scen_view(sc, b), (g - sc.seg_first) << sc.seg_log, base, cursor,
                       b.new_start + sc.start_base, b.new_bpos, b.new_bid);
 }
 
 // Single-pass rebinning of every scenario's table: segments are handed out
 // by ticket, and each scenario's look-back chain starts at its first segment
 // (prefix = the scenario's first slot).
-static __global__ __launch_bounds__(kBinThreads) void k_bbin_fused(const float2* __restrict__ pos, BatchBin b) {
+template <uint32_t U>
+static __global__ __launch_bounds__(kBinThreads, 8) void k_bbin_fused(const float2* __restrict__ pos, BatchBin b) {
     extern __shared__ uint32_t hist[];
     __shared__ uint32_t lds[kBinThreads / 64];
     __shared__ uint32_t sh[3];
@@ -96,7 +99,7 @@ static __global__ __launch_bounds__(kBinThreads) void k_bbin_fused(const float2*
     const uint32_t g = sh[0];
     const ScenDev sc = b.scen[b.seg_scen[g]];
     uint32_t base, total;
-    bin_fused_segment(pos, scen_view(sc, b), (g - sc.seg_first) << sc.seg_log, hist, lds, &sh[2],
+    bin_fused_segment<U>(pos, scen_view(sc, b), (g - sc.seg_first) << sc.seg_log, hist, lds, &sh[2],
                       b.new_start + sc.start_base, b.new_bpos, b.new_bid, b.status, g, sc.seg_first, sc.agent_off,
                       sh[1], b.faults, &base, &total);
     if (g == sc.seg_first + sc.n_seg - 1 && threadIdx.x == 0) {
@@ -185,13 +188,20 @@ static __global__ __launch_bounds__(kBlock) void k_bcollide(BatchCollide c) {
     bpart_add(c, (blk * kBlock + threadIdx.x) >> 6, valid, sid, cnt);
 }
 
+// count_in_cells on a slot's own scenario table (the few waves that
+// straddle a scenario boundary).
+__device__ __forceinline__ uint32_t bcount_slot(const BatchCollide& c, uint32_t sid, float2 me, uint32_t s, int32_t cx,
+                                             int32_t cy) {
+    const ScenDev& sc = c.scen[sid];
+    return count_in_cells(me, s, cx, cy, c.start + sc.start_base, sc.lw, sc.wmask, sc.hmask, c.bpos, c.d2_limit);
+}
+
 // Staged neighbour pass: each wave takes 64 P consecutive slots, P per
-// lane, and runs wave_count_in_cells once per scenario its slots touch
-// (usually one), on that scenario's table; each scenario's partial pair sum
-// takes one atomic per wave (grid: ceil(n / (kBlock P)) workgroups).  At
-// most 64 VGPRs (8 waves per SIMD).
+// lane; a wave whose slots all lie in one scenario runs wave_count_in_cells
+// on that scenario's table, a wave across a scenario boundary walks per slot
+// (grid: ceil(n / (kBlock P)) workgroups).
 template <int P>
-static __global__ __launch_bounds__(kBlock, 8) void k_bcollide_staged(BatchCollide c) {
+static __global__ __launch_bounds__(kBlock) void k_bcollide_staged(BatchCollide c) {
     __shared__ float2 stage_all[kBlock / 64 * kStage];
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t wave = blk * (kBlock / 64) + (threadIdx.x >> 6);
@@ -199,7 +209,6 @@ static __global__ __launch_bounds__(kBlock, 8) void k_bcollide_staged(BatchColli
     if (w0 >= c.n) {
         return;  // wave-uniform: no barrier follows
     }
-    const float T = c.d2_limit;
     const uint32_t lane = __lane_id();
     // the wave's 64 P slots lie in one 256-slot block, whose entry names the
     // scenario of its first slot and where that scenario ends
@@ -208,6 +217,7 @@ static __global__ __launch_bounds__(kBlock, 8) void k_bcollide_staged(BatchColli
     bool valid[P];
     int32_t cx[P], cy[P];
     uint32_t id[P], cnt[P];
+    bool later = false;  // some valid slot lies past scenario bs.sid
 #pragma unroll
     for (int p = 0; p < P; p++) {
         const uint32_t s = w0 + 64u * p + lane;
@@ -216,55 +226,28 @@ static __global__ __launch_bounds__(kBlock, 8) void k_bcollide_staged(BatchColli
         id[p] = valid[p] ? c.bid[s] : 0u;
         cx[p] = cell_coord(me[p].x, c.inv_cell);
         cy[p] = cell_coord(me[p].y, c.inv_cell);
+        later = later || (valid[p] && s >= bs.end);
         cnt[p] = 0u;
     }
-    const uint32_t wend = min(w0 + 64u * P, c.n);
-    uint32_t sid = bs.sid, s_end = bs.end;
-    uint32_t base = bs.start_base, lw = bs.lw, wmask = bs.wmask, hmask = bs.hmask;
-    uint32_t first = w0;
-#pragma unroll 1
-    while (first < wend) {  // one scenario's slots [first, last] per round
-        if (first >= s_end) {
-            do {
-                sid++;
-                s_end = c.agent_off[sid + 1];
-            } while (first >= s_end);
-            const ScenDev& sc = c.scen[sid];
-            base = sc.start_base;
-            lw = sc.lw;
-            wmask = sc.wmask;
-            hmask = sc.hmask;
-        }
-        const uint32_t last = min(wend, s_end) - 1u;
-        bool sub[P];
-#pragma unroll
-        for (int p = 0; p < P; p++) {
-            const uint32_t s = w0 + 64u * p + lane;
-            sub[p] = s >= first && s <= last;
-        }
-        const uint32_t f = first - w0, l = last - w0;
-        wave_count_in_cells<P>(me, sub, f >> 6, f & 63u, l >> 6, l & 63u, cx, cy, c.start + base, lw, wmask, hmask,
-                               c.bpos, T, stage_all + (threadIdx.x >> 6) * kStage, cnt);
-        uint32_t v = 0;
-#pragma unroll
-        for (int p = 0; p < P; p++) {
-            v += sub[p] ? cnt[p] - 1u : 0u;  // own slot
-        }
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            v += __shfl_xor(v, o, 64);
-        }
-        if (lane == 0 && v) {
-            atomicAdd(&c.part[(static_cast<size_t>(sid) * kScenSlots + wave % kScenSlots) * kScenSlotStride],
-                      static_cast<unsigned long long>(v));
-        }
-        first = last + 1u;
+    const bool one = __ballot(later) == 0ull;
+    if (one) {
+        const uint32_t last = min(64u * P, c.n - w0) - 1u;
+        wave_count_in_cells<P>(me, valid, 0u, 0u, last >> 6, last & 63u, cx, cy, c.start + bs.start_base, bs.lw,
+                               bs.wmask, bs.hmask, c.bpos, c.d2_limit, stage_all + (threadIdx.x >> 6) * kStage,
+                               cnt);
     }
 #pragma unroll
     for (int p = 0; p < P; p++) {
+        const uint32_t s = w0 + 64u * p + lane;
+        const uint32_t sid = (one || !valid[p]) ? bs.sid : slot_scen(c, bs, s);
         if (valid[p]) {
-            c.ncount[id[p]] = cnt[p] - 1u;  // own slot
+            if (!one) {
+                cnt[p] = bcount_slot(c, sid, me[p], s, cx[p], cy[p]);
+            }
+            cnt[p] -= 1u;  // own slot
+            c.ncount[id[p]] = cnt[p];
         }
+        bpart_add(c, wave * P + p, valid[p], sid, cnt[p]);
     }
 }
 
@@ -338,6 +321,7 @@ struct msim_batch {
     bool first_tick_insert_only = false;
     int collide_mode = 0;  // 0: staged, 2 slots per lane; 1: per lane; 2: staged, 1 slot per lane
     int binning = 0;  // 0: k_bbin_count + k_bbin_place; 2: k_bbin_fused
+    uint32_t bin_unroll = 4;  // window slots per rebinning thread and load round
     uint32_t tick = 1;
     uint32_t graph_ticks = 100, seg_ticks = 100;
     uint32_t reorder_interval = 20, since_reorder = 0;
@@ -405,12 +389,20 @@ inline int bcheck(msim_batch* b, const char* what) {
 }
 
 // Launch order of a batched tick, for msim_batch_time_ticks.
-inline const char* const kBatchKernels[] = {"k_advance", "k_bbin_count", "k_bbin_place", "k_bcollide", "k_bfinish"};
-inline const char* const kBatchKernelsFused[] = {"k_advance", "k_bbin_fused", "k_bcollide", "k_bfinish"};
+// Launches of a batched tick, in order, by kernel name (template arguments
+// dropped).
 constexpr int kBatchKernelMax = 5;
 inline int batch_kernel_count(const msim_batch* b) { return b->binning == 2 ? 4 : 5; }
 inline const char* batch_kernel_name(const msim_batch* b, int k) {
-    return b->binning == 2 ? kBatchKernelsFused[k] : kBatchKernels[k];
+    if (k == 0) return "k_advance4";
+    if (b->binning == 2) {
+        if (k == 1) return "k_bbin_fused";
+        k++;
+    } else if (k <= 2) {
+        return k == 1 ? "k_bbin_count" : "k_bbin_place";
+    }
+    if (k == 3) return b->collide_mode == 1 ? "k_bcollide" : "k_bcollide_staged";
+    return "k_bfinish";
 }
 
 inline void bprobe_pre(msim_batch* b) {
@@ -420,6 +412,24 @@ inline void bprobe_pre(msim_batch* b) {
 inline void bprobe_post(msim_batch* b) {
     if (b->probe_kernel == b->launch_no) (void)hipEventRecord(b->probe_ev[1], b->stream);
     b->launch_no++;
+}
+
+// Segment rebinning of every scenario: one pass (k_bbin_fused) or count +
+// place.
+template <uint32_t U>
+inline void blaunch_bins(msim_batch* b, const float2* pos, const BatchBin& bb, size_t lds) {
+    if (b->binning == 2) {
+        bprobe_pre(b);
+        hipLaunchKernelGGL(k_bbin_fused<U>, dim3(b->n_seg), dim3(kBinThreads), lds, b->stream, pos, bb);
+        bprobe_post(b);
+    } else {
+        bprobe_pre(b);
+        hipLaunchKernelGGL(k_bbin_count<U>, dim3(b->n_seg), dim3(kBinThreads), lds, b->stream, pos, bb);
+        bprobe_post(b);
+        bprobe_pre(b);
+        hipLaunchKernelGGL(k_bbin_place<U>, dim3(b->n_seg), dim3(kBinThreads), lds, b->stream, pos, bb);
+        bprobe_post(b);
+    }
 }
 
 inline int bwait(msim_batch* b) {

@@ -318,47 +318,46 @@ __device__ __forceinline__ void bin_window(const TableView& t, uint32_t s0, uint
     }
     const uint32_t lo = (s0 + t.m - t.margin) & (t.m - 1);
     const uint32_t hi = (s0 + S + t.margin) & (t.m - 1);
-    a0 = t.old_start[lo];
-    if (lo < hi) {
-        a1 = t.old_start[hi];
-    } else {
-        a1 = t.old_start[t.m];
-        c0 = t.old_start[0];
-        c1 = t.old_start[hi];
-    }
+    const bool wrap = lo >= hi;  // the window wraps around the table's end
+    const uint32_t v_lo = t.old_start[lo], v_hi = t.old_start[hi], v_end = t.old_start[t.m];
+    const uint32_t v_0 = t.old_start[0];
+    a0 = v_lo;
+    a1 = wrap ? v_end : v_hi;
+    c0 = wrap ? v_0 : 0u;
+    c1 = wrap ? v_hi : 0u;
 }
 
 __device__ __forceinline__ uint32_t bucket_of_pos(float2 p, const TableView& t) {
     return bucket_of(cell_coord(p.x, t.inv_cell), cell_coord(p.y, t.inv_cell), t.lw, t.wmask, t.hmask);
 }
 
-constexpr uint32_t kBinUnroll = 4;
 
 // One pass over the window slots [lo, hi): every agent whose new bucket lies
 // in the segment bumps its LDS counter (PLACE: and is written to the slot the
 // counter hands out).  The trip count is block-uniform, so the wave-level rank
-// helper always runs with the whole wave; kBinUnroll independent loads per
+// helper always runs with the whole wave; U independent loads per
 // thread keep enough requests in flight.
-template <bool PLACE>
+template <bool PLACE, uint32_t U>
 __device__ __forceinline__ void bin_pass(const float2* __restrict__ pos, const TableView& t, uint32_t s0,
                                          uint32_t lo, uint32_t hi, uint32_t* table, float2* __restrict__ out_pos,
                                          uint32_t* __restrict__ out_bid) {
     const uint32_t S = 1u << t.seg_log;
-    for (uint32_t base = lo; base < hi; base += kBinUnroll * kBinThreads) {
-        uint32_t id[kBinUnroll];
-        float2 p[kBinUnroll];
+    for (uint32_t base = lo; base < hi; base += U * kBinThreads) {
+        uint32_t id[U];
+        float2 p[U];
+        // unconditional loads (past hi they re-read slot hi - 1), so that all
+        // of them are in flight before the first is used
 #pragma unroll
-        for (uint32_t u = 0; u < kBinUnroll; u++) {
+        for (uint32_t u = 0; u < U; u++) {
             const uint32_t k = base + u * kBinThreads + threadIdx.x;
-            id[u] = k < hi ? t.old_bid[k] : 0u;
+            id[u] = t.old_bid[min(k, hi - 1u)];
         }
 #pragma unroll
-        for (uint32_t u = 0; u < kBinUnroll; u++) {
-            const uint32_t k = base + u * kBinThreads + threadIdx.x;
-            p[u] = k < hi ? pos[id[u]] : make_float2(0.0f, 0.0f);
+        for (uint32_t u = 0; u < U; u++) {
+            p[u] = pos[id[u]];
         }
 #pragma unroll
-        for (uint32_t u = 0; u < kBinUnroll; u++) {
+        for (uint32_t u = 0; u < U; u++) {
             const uint32_t k = base + u * kBinThreads + threadIdx.x;
             const uint32_t off = bucket_of_pos(p[u], t) - s0;
             const bool mine = (k < hi) && (off < S);
@@ -373,6 +372,7 @@ __device__ __forceinline__ void bin_pass(const float2* __restrict__ pos, const T
 
 // Segment count: window agents landing in [s0, s0 + S) counted in LDS, local
 // exclusive offsets written to start_out[s0 + j]; returns the segment total.
+template <uint32_t U>
 __device__ __forceinline__ uint32_t bin_count_segment(const float2* __restrict__ pos, const TableView& t,
                                                       uint32_t s0, uint32_t* hist, uint32_t* lds,
                                                       uint32_t* __restrict__ start_out) {
@@ -384,8 +384,8 @@ __device__ __forceinline__ uint32_t bin_count_segment(const float2* __restrict__
     __syncthreads();
     uint32_t a0, a1, c0, c1;
     bin_window(t, s0, a0, a1, c0, c1);
-    bin_pass<false>(pos, t, s0, a0, a1, hist, nullptr, nullptr);
-    bin_pass<false>(pos, t, s0, c0, c1, hist, nullptr, nullptr);
+    bin_pass<false, U>(pos, t, s0, a0, a1, hist, nullptr, nullptr);
+    bin_pass<false, U>(pos, t, s0, c0, c1, hist, nullptr, nullptr);
     __syncthreads();
     const uint32_t per = S / kBinThreads;
     uint32_t run = 0;
@@ -403,6 +403,7 @@ __device__ __forceinline__ uint32_t bin_count_segment(const float2* __restrict__
 }
 
 // Segment place: start_out[s0 + j] += base (absolute slots), agents placed.
+template <uint32_t U>
 __device__ __forceinline__ void bin_place_segment(const float2* __restrict__ pos, const TableView& t, uint32_t s0,
                                                   uint32_t base, uint32_t* cursor, uint32_t* __restrict__ start_out,
                                                   float2* __restrict__ out_pos, uint32_t* __restrict__ out_bid) {
@@ -415,8 +416,8 @@ __device__ __forceinline__ void bin_place_segment(const float2* __restrict__ pos
     __syncthreads();
     uint32_t a0, a1, c0, c1;
     bin_window(t, s0, a0, a1, c0, c1);
-    bin_pass<true>(pos, t, s0, a0, a1, cursor, out_pos, out_bid);
-    bin_pass<true>(pos, t, s0, c0, c1, cursor, out_pos, out_bid);
+    bin_pass<true, U>(pos, t, s0, a0, a1, cursor, out_pos, out_bid);
+    bin_pass<true, U>(pos, t, s0, c0, c1, cursor, out_pos, out_bid);
 }
 
 // ---- single-pass rebinning: count, decoupled look-back, place -------------
@@ -499,6 +500,7 @@ __device__ __forceinline__ void take_segment(unsigned long long* ticket, uint32_
 // Count, look back, place.  Writes the segment's absolute bucket offsets to
 // start_out[s0 + j]; *base_out / *total_out receive the segment's first slot
 // and agent count (all threads).
+template <uint32_t U>
 __device__ __forceinline__ void bin_fused_segment(const float2* __restrict__ pos, const TableView& t, uint32_t s0,
                                                   uint32_t* hist, uint32_t* lds, uint32_t* sh_base,
                                                   uint32_t* __restrict__ start_out, float2* __restrict__ out_pos,
@@ -513,8 +515,8 @@ __device__ __forceinline__ void bin_fused_segment(const float2* __restrict__ pos
     __syncthreads();
     uint32_t a0, a1, c0, c1;
     bin_window(t, s0, a0, a1, c0, c1);
-    bin_pass<false>(pos, t, s0, a0, a1, hist, nullptr, nullptr);
-    bin_pass<false>(pos, t, s0, c0, c1, hist, nullptr, nullptr);
+    bin_pass<false, U>(pos, t, s0, a0, a1, hist, nullptr, nullptr);
+    bin_pass<false, U>(pos, t, s0, c0, c1, hist, nullptr, nullptr);
     __syncthreads();
     const uint32_t per = S / kBinThreads;
     uint32_t run = 0;
@@ -539,21 +541,22 @@ __device__ __forceinline__ void bin_fused_segment(const float2* __restrict__ pos
         start_out[s0 + j] = v;
     }
     __syncthreads();
-    bin_pass<true>(pos, t, s0, a0, a1, hist, out_pos, out_bid);
-    bin_pass<true>(pos, t, s0, c0, c1, hist, out_pos, out_bid);
+    bin_pass<true, U>(pos, t, s0, a0, a1, hist, out_pos, out_bid);
+    bin_pass<true, U>(pos, t, s0, c0, c1, hist, out_pos, out_bid);
     *base_out = base;
     *total_out = total;
 }
 
 // Single-pass rebinning of one table (grid: n_seg workgroups).
-static __global__ __launch_bounds__(kBinThreads) void k_bin_fused(const float2* __restrict__ pos, BinDev b) {
+template <uint32_t U>
+static __global__ __launch_bounds__(kBinThreads, 8) void k_bin_fused(const float2* __restrict__ pos, BinDev b) {
     extern __shared__ uint32_t hist[];
     __shared__ uint32_t lds[kBinThreads / 64];
     __shared__ uint32_t sh[3];
     take_segment(b.ticket, b.n_seg, sh);
     const uint32_t seg = sh[0];
     uint32_t base, total;
-    bin_fused_segment(pos, table_view(b), seg << b.seg_log, hist, lds, &sh[2], b.new_start, b.new_bpos, b.new_bid,
+    bin_fused_segment<U>(pos, table_view(b), seg << b.seg_log, hist, lds, &sh[2], b.new_start, b.new_bpos, b.new_bid,
                       b.status, seg, 0u, 0u, sh[1], &b.ctr->faults, &base, &total);
     if (seg == b.n_seg - 1 && threadIdx.x == 0) {
         b.new_start[b.m] = base + total;
@@ -565,18 +568,20 @@ static __global__ __launch_bounds__(kBinThreads) void k_bin_fused(const float2* 
 
 // Counts the window agents that land in segment blockIdx.x, writes the
 // segment-local exclusive offsets to new_start and the total to seg_total.
+template <uint32_t U>
 static __global__ __launch_bounds__(kBinThreads) void k_bin_count(const float2* __restrict__ pos, BinDev b) {
     extern __shared__ uint32_t hist[];
     __shared__ uint32_t lds[kBinThreads / 64];
     const uint32_t seg = blockIdx.x;
-    const uint32_t total = bin_count_segment(pos, table_view(b), seg << b.seg_log, hist, lds, b.new_start);
+    const uint32_t total = bin_count_segment<U>(pos, table_view(b), seg << b.seg_log, hist, lds, b.new_start);
     if (threadIdx.x == 0) {
         b.seg_total[seg] = total;
     }
 }
 
 // Adds the segment's global offset and places its agents.
-static __global__ __launch_bounds__(kBinThreads) void k_bin_place(const float2* __restrict__ pos, BinDev b) {
+template <uint32_t U>
+static __global__ __launch_bounds__(kBinThreads, 8) void k_bin_place(const float2* __restrict__ pos, BinDev b) {
     extern __shared__ uint32_t cursor[];
     __shared__ uint32_t lds[kBinThreads / 64];
     const uint32_t seg = blockIdx.x;
@@ -592,7 +597,7 @@ static __global__ __launch_bounds__(kBinThreads) void k_bin_place(const float2* 
             atomicOr(&b.ctr->faults, kFaultBinCount);
         }
     }
-    bin_place_segment(pos, table_view(b), seg << b.seg_log, base, cursor, b.new_start, b.new_bpos, b.new_bid);
+    bin_place_segment<U>(pos, table_view(b), seg << b.seg_log, base, cursor, b.new_start, b.new_bpos, b.new_bid);
 }
 
 }  // namespace msim

@@ -59,6 +59,11 @@ inline uint32_t auto_segment_log2(uint32_t tlog, uint32_t lw) {
     return l;
 }
 
+// Window slots each rebinning thread loads per round: 8 keeps more loads in
+// flight; segments of 2^14 buckets (64 KB of LDS counters, two workgroups
+// per CU) measured faster with 4.
+inline uint32_t auto_bin_unroll(uint32_t seg_log) { return seg_log <= 13 ? 8u : 4u; }
+
 // floor(log2(rows spanned by the roads and agents, in cells)), at least 2.
 inline uint32_t extent_rows_log2(const msim_road* roads, uint32_t n_roads, const msim_entity* e, uint32_t n,
                                  float cell) {

@@ -65,6 +65,7 @@ struct msim_sim {
                       // 2: segment-owned in one pass (k_bin_fused, look-back)
     bool route_inline = false;  // arrivals routed inside k_advance instead of k_route
     int collide_mode = 0;  // 0: staged, 2 slots per lane; 1: per lane; 2: staged, 1 slot per lane
+    uint32_t bin_unroll = 4;  // window slots per rebinning thread and load round
     // msim_time_ticks probe: launch number probe_kernel of the tick is
     // bracketed by probe_ev[0] / probe_ev[1]; -1 = no probe.
     int probe_kernel = -1;
@@ -129,9 +130,10 @@ const char* const kAtomicKernels[] = {"k_scan_reduce", "k_scan_partials", "k_sca
 const char* const kFusedKernels[] = {"k_bin_fused"};
 uint32_t bin_kernels(const msim_sim* s) { return s->binning == 1 ? 4u : (s->binning == 2 ? 1u : 2u); }
 uint32_t kernels_per_tick(const msim_sim* s) { return (s->route_inline ? 1u : 2u) + bin_kernels(s) + 2u; }
-// Name of launch k of a fused tick.
+// Name of launch k of a fused tick (the kernel's name, template arguments
+// dropped).
 const char* tick_kernel_name(const msim_sim* s, uint32_t k) {
-    if (k == 0) return "k_advance";
+    if (k == 0) return (s->route_inline && s->binning != 1) ? "k_advance4" : "k_advance";
     if (!s->route_inline) {
         if (k == 1) return "k_route";
         k--;
@@ -140,7 +142,8 @@ const char* tick_kernel_name(const msim_sim* s, uint32_t k) {
     const uint32_t nbin = bin_kernels(s);
     if (k < nbin) return s->binning == 1 ? kAtomicKernels[k] : (s->binning == 2 ? kFusedKernels[k] : kSegKernels[k]);
     k -= nbin;
-    return k == 0 ? "k_collide" : "k_collide_finish";
+    if (k == 0) return s->collide_mode == 1 ? "k_collide" : "k_collide_staged";
+    return "k_collide_finish";
 }
 }  // namespace
 
@@ -235,6 +238,23 @@ void probe_post(msim_sim* s) {
     s->launch_no++;
 }
 
+// Segment rebinning: one pass (k_bin_fused) or count + place.
+template <uint32_t U>
+void launch_bins(msim_sim* s, const float2* pos, const BinDev& b, size_t lds) {
+    if (s->binning == 2) {
+        probe_pre(s);
+        hipLaunchKernelGGL(k_bin_fused<U>, dim3(s->n_seg), dim3(kBinThreads), lds, s->stream, pos, b);
+        probe_post(s);
+    } else {
+        probe_pre(s);
+        hipLaunchKernelGGL(k_bin_count<U>, dim3(s->n_seg), dim3(kBinThreads), lds, s->stream, pos, b);
+        probe_post(s);
+        probe_pre(s);
+        hipLaunchKernelGGL(k_bin_place<U>, dim3(s->n_seg), dim3(kBinThreads), lds, s->stream, pos, b);
+        probe_post(s);
+    }
+}
+
 // Fast move dispatch: streaming advance, route choice for the arrivals, then
 // the bucket table rebuild (segment-owned into the other binning buffer, or
 // atomic counting + scan + scatter into the current one).  ev (optional)
@@ -320,17 +340,10 @@ int enqueue_move_fast(msim_sim* s, bool write_dir, hipEvent_t* ev) {
         b.margin = s->margin;
         b.n = s->n;
         const size_t lds = sizeof(uint32_t) << s->seg_log;
-        if (s->binning == 2) {
-            probe_pre(s);
-            hipLaunchKernelGGL(k_bin_fused, dim3(s->n_seg), dim3(kBinThreads), lds, s->stream, pos, b);
-            probe_post(s);
+        if (s->bin_unroll == 8) {
+            launch_bins<8>(s, pos, b, lds);
         } else {
-            probe_pre(s);
-            hipLaunchKernelGGL(k_bin_count, dim3(s->n_seg), dim3(kBinThreads), lds, s->stream, pos, b);
-            probe_post(s);
-            probe_pre(s);
-            hipLaunchKernelGGL(k_bin_place, dim3(s->n_seg), dim3(kBinThreads), lds, s->stream, pos, b);
-            probe_post(s);
+            launch_bins<4>(s, pos, b, lds);
         }
         s->bcur = nb;
     } else {
@@ -647,6 +660,9 @@ extern "C" int msim_create(const msim_road* roads, uint32_t n_roads, const uint3
     if (o.collide_mode > 2) {
         return fail(nullptr, MSIM_ERR_INVALID, "collide_mode must be 0, 1 or 2");
     }
+    if (o.bin_unroll != 0 && o.bin_unroll != 4 && o.bin_unroll != 8) {
+        return fail(nullptr, MSIM_ERR_INVALID, "bin_unroll must be 0, 4 or 8");
+    }
     if (o.binning > 2) {
         return fail(nullptr, MSIM_ERR_INVALID, "binning must be 0 (segment), 1 (atomic) or 2 (segment, one pass)");
     }
@@ -708,6 +724,7 @@ extern "C" int msim_create(const msim_road* roads, uint32_t n_roads, const uint3
     s->collide_mode = static_cast<int>(o.collide_mode);
     s->seg_log = seg_use;
     s->n_seg = g.m >> seg_use;
+    s->bin_unroll = o.bin_unroll ? o.bin_unroll : auto_bin_unroll(seg_use);
     // An agent moves at most max_speed per axis per tick (the heading has
     // |dir| = speed up to rounding), i.e. at most D cells; in linear bucket
     // order that is at most (D + 1) rows plus D columns.

@@ -43,6 +43,7 @@ class Options(ctypes.Structure):
         ("binning", ctypes.c_uint32),
         ("route_queue", ctypes.c_uint32),
         ("collide_mode", ctypes.c_uint32),
+        ("bin_unroll", ctypes.c_uint32),
     ]
 
 

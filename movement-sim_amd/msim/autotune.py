@@ -2,8 +2,8 @@
 
 Coordinate descent over the msim_options knobs that change speed but never
 results (every value is covered by the parity tests): bucket-table row width,
-rebinning segment size, binning mode, inline vs queued route choice, spatial
-reorder interval and the neighbour walk.  Each candidate is timed on the real
+rebinning segment size, binning mode and load batching, inline vs queued
+route choice, spatial reorder interval and the neighbour walk.  Each candidate is timed on the real
 workload with HIP events around graph-replayed ticks; the winner is cached in
 a JSON file keyed by the workload's shape.
 
@@ -34,9 +34,10 @@ SPACE = {
     "route_queue": [0, 1],
     "reorder_interval": [8, 16, 32, 64],
     "collide_mode": [0, 1, 2],
+    "bin_unroll": [0, 4, 8],
 }
 DEFAULT = {"table_width_log2": 0, "segment_log2": 0, "binning": 0, "route_queue": 0,
-           "reorder_interval": 16, "collide_mode": 0}
+           "reorder_interval": 16, "collide_mode": 0, "bin_unroll": 0}
 
 
 BATCH_SPACE = {
@@ -45,8 +46,10 @@ BATCH_SPACE = {
     "graph_ticks": [50, 100, 200],
     "collide_mode": [0, 1, 2],
     "binning": [0, 2],
+    "bin_unroll": [0, 4, 8],
 }
-BATCH_DEFAULT = {"segment_log2": 0, "reorder_interval": 20, "graph_ticks": 100, "collide_mode": 0, "binning": 0}
+BATCH_DEFAULT = {"segment_log2": 0, "reorder_interval": 20, "graph_ticks": 100, "collide_mode": 0, "binning": 0,
+                 "bin_unroll": 0}
 
 
 def batch_key(scs):

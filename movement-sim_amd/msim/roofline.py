@@ -34,9 +34,10 @@ def kernel_bytes(name, uniform_speed=False):
     k_advance / k_advance4 28 (24 with the shader's constant speed); k_bin_count 8 (bucket
     offsets) and k_bin_place 12 (binned position + id) in the segment path, or
     k_scan_down 8 and k_scatter 12 in the atomic path, k_bin_fused 20 (both)
-    in the one-pass segment path; k_collide 16.  The batched kernels
-    k_bbin_count, k_bbin_place, k_bbin_fused and k_bcollide do the same work
-    per agent as their plain counterparts.  k_route (arrival path), the
+    in the one-pass segment path; k_collide / k_collide_staged 16.  The
+    batched kernels k_bbin_count, k_bbin_place, k_bbin_fused, k_bcollide and
+    k_bcollide_staged do the same work per agent as their plain
+    counterparts.  k_route (arrival path), the
     single-workgroup scan/finish passes, k_bfinish and k_reorder: 0."""
     name = name.split("<")[0]
     if name in ("k_advance", "k_advance4", "k_advance_bin"):
@@ -47,6 +48,6 @@ def kernel_bytes(name, uniform_speed=False):
         return B_SCATTER
     if name in ("k_bin_fused", "k_bbin_fused"):
         return B_SCAN + B_SCATTER
-    if name in ("k_collide", "k_bcollide"):
+    if name in ("k_collide", "k_bcollide", "k_collide_staged", "k_bcollide_staged"):
         return B_COLLIDE
     return 0
