@@ -77,7 +77,7 @@ def plain_line(args, sc):
 
     n = sc.entities.shape[0]
     opts = dict(autotune.DEFAULT) if args.untuned else autotune.tuned_options(sc)
-    for k in ("binning", "collide_mode"):
+    for k in ("binning", "collide_mode", "bin_unroll"):
         if getattr(args, k) is not None:
             opts[k] = getattr(args, k)
     sim = msim.Simulation(sc.roads, sc.conns, sc.entities, sc.pc, speed=sc.speed, **opts)
@@ -198,7 +198,7 @@ def batched_line(args, scs):
         v = getattr(args, "batch_" + k)
         if v is not None:
             opts[k] = v
-    for k in ("binning", "collide_mode"):
+    for k in ("binning", "collide_mode", "bin_unroll"):
         if getattr(args, k) is not None:
             opts[k] = getattr(args, k)
     b = msim.Batch(scs, **opts)
@@ -288,6 +288,7 @@ def main():
                                                                "1 atomic (plain only), 2 one pass)")
     ap.add_argument("--collide-mode", type=int, default=None, help="override the tuned neighbour walk (0 staged, "
                                                                     "2 slots per lane; 1 per lane; 2 staged, 1 slot)")
+    ap.add_argument("--bin-unroll", type=int, default=None, help="override the tuned rebinning load batching (4, 8)")
     ap.add_argument("--profile-ticks", type=int, default=240)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")

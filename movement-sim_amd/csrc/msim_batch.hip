@@ -58,11 +58,7 @@ int benqueue_move(msim_batch* b, bool write_dir) {
     bb.n_seg = b->n_seg;
     const size_t lds = sizeof(uint32_t) << b->max_seg_log;
     const float2* pos = b->agents(b->cur).pos;
-    if (b->bin_unroll == 8) {
-        blaunch_bins<8>(b, pos, bb, lds);
-    } else {
-        blaunch_bins<4>(b, pos, bb, lds);
-    }
+    blaunch_bins(b, pos, bb, lds);
     b->bcur = nb;
     return bcheck(b, "batched move dispatch");
 }

@@ -52,30 +52,30 @@ __device__ __forceinline__ TableView scen_view(const ScenDev& sc, const BatchBin
                      sc.seg_log, sc.margin};
 }
 
-template <uint32_t U>
-static __global__ __launch_bounds__(kBinThreads) void k_bbin_count(const float2* __restrict__ pos, BatchBin b) {
+template <uint32_t U, uint32_t T>
+static __global__ __launch_bounds__(T) void k_bbin_count(const float2* __restrict__ pos, BatchBin b) {
     extern __shared__ uint32_t hist[];
-    __shared__ uint32_t lds[kBinThreads / 64];
+    __shared__ uint32_t lds[T / 64];
     const uint32_t g = blockIdx.x;
     const ScenDev sc = b.scen[b.seg_scen[g]];
     const uint32_t s0 = (g - sc.seg_first) << sc.seg_log;
-    const uint32_t total = bin_count_segment<U>(pos, scen_view(sc, b), s0, hist, lds, b.new_start + sc.start_base);
+    const uint32_t total = bin_count_segment<U, T>(pos, scen_view(sc, b), s0, hist, lds, b.new_start + sc.start_base);
     if (threadIdx.x == 0) {
         b.seg_total[g] = total;
     }
 }
 
-template <uint32_t U>
-static __global__ __launch_bounds__(kBinThreads, 8) void k_bbin_place(const float2* __restrict__ pos, BatchBin b) {
+template <uint32_t U, uint32_t T>
+static __global__ __launch_bounds__(T, 8) void k_bbin_place(const float2* __restrict__ pos, BatchBin b) {
     extern __shared__ uint32_t cursor[];
-    __shared__ uint32_t lds[kBinThreads / 64];
+    __shared__ uint32_t lds[T / 64];
     const uint32_t g = blockIdx.x;
     const ScenDev sc = b.scen[b.seg_scen[g]];
     uint32_t before = 0;
-    for (uint32_t k = sc.seg_first + threadIdx.x; k < g; k += kBinThreads) {
+    for (uint32_t k = sc.seg_first + threadIdx.x; k < g; k += T) {
         before += b.seg_total[k];
     }
-    const uint32_t base = sc.agent_off + block_sum<kBinThreads>(before, lds);
+    const uint32_t base = sc.agent_off + block_sum<T>(before, lds);
     if (g == sc.seg_first + sc.n_seg - 1 && threadIdx.x == 0) {
         const uint32_t end = base + b.seg_total[g];
         b.new_start[sc.start_base + sc.m] = end;
@@ -83,23 +83,23 @@ static __global__ __launch_bounds__(kBinThreads, 8) void k_bbin_place(const floa
             atomicOr(b.faults, kFaultBinCount);
         }
     }
-    bin_place_segment<U>(pos, scen_view(sc, b), (g - sc.seg_first) << sc.seg_log, base, cursor,
+    bin_place_segment<U, T>(pos, scen_view(sc, b), (g - sc.seg_first) << sc.seg_log, base, cursor,
                       b.new_start + sc.start_base, b.new_bpos, b.new_bid);
 }
 
 // Single-pass rebinning of every scenario's table: segments are handed out
 // by ticket, and each scenario's look-back chain starts at its first segment
 // (prefix = the scenario's first slot).
-template <uint32_t U>
-static __global__ __launch_bounds__(kBinThreads, 8) void k_bbin_fused(const float2* __restrict__ pos, BatchBin b) {
+template <uint32_t U, uint32_t T>
+static __global__ __launch_bounds__(T, 8) void k_bbin_fused(const float2* __restrict__ pos, BatchBin b) {
     extern __shared__ uint32_t hist[];
-    __shared__ uint32_t lds[kBinThreads / 64];
+    __shared__ uint32_t lds[T / 64];
     __shared__ uint32_t sh[3];
     take_segment(b.ticket, b.n_seg, sh);
     const uint32_t g = sh[0];
     const ScenDev sc = b.scen[b.seg_scen[g]];
     uint32_t base, total;
-    bin_fused_segment<U>(pos, scen_view(sc, b), (g - sc.seg_first) << sc.seg_log, hist, lds, &sh[2],
+    bin_fused_segment<U, T>(pos, scen_view(sc, b), (g - sc.seg_first) << sc.seg_log, hist, lds, &sh[2],
                       b.new_start + sc.start_base, b.new_bpos, b.new_bid, b.status, g, sc.seg_first, sc.agent_off,
                       sh[1], b.faults, &base, &total);
     if (g == sc.seg_first + sc.n_seg - 1 && threadIdx.x == 0) {
@@ -201,7 +201,7 @@ __device__ __forceinline__ uint32_t bcount_slot(const BatchCollide& c, uint32_t 
 // on that scenario's table, a wave across a scenario boundary walks per slot
 // (grid: ceil(n / (kBlock P)) workgroups).
 template <int P>
-static __global__ __launch_bounds__(kBlock) void k_bcollide_staged(BatchCollide c) {
+static __global__ __launch_bounds__(kBlock, P == 1 ? 8 : 6) void k_bcollide_staged(BatchCollide c) {
     __shared__ float2 stage_all[kBlock / 64 * kStage];
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t wave = blk * (kBlock / 64) + (threadIdx.x >> 6);
@@ -416,19 +416,30 @@ inline void bprobe_post(msim_batch* b) {
 
 // Segment rebinning of every scenario: one pass (k_bbin_fused) or count +
 // place.
-template <uint32_t U>
+template <uint32_t U, uint32_t T>
 inline void blaunch_bins(msim_batch* b, const float2* pos, const BatchBin& bb, size_t lds) {
     if (b->binning == 2) {
         bprobe_pre(b);
-        hipLaunchKernelGGL(k_bbin_fused<U>, dim3(b->n_seg), dim3(kBinThreads), lds, b->stream, pos, bb);
+        hipLaunchKernelGGL((k_bbin_fused<U, T>), dim3(b->n_seg), dim3(T), lds, b->stream, pos, bb);
         bprobe_post(b);
     } else {
         bprobe_pre(b);
-        hipLaunchKernelGGL(k_bbin_count<U>, dim3(b->n_seg), dim3(kBinThreads), lds, b->stream, pos, bb);
+        hipLaunchKernelGGL((k_bbin_count<U, T>), dim3(b->n_seg), dim3(T), lds, b->stream, pos, bb);
         bprobe_post(b);
         bprobe_pre(b);
-        hipLaunchKernelGGL(k_bbin_place<U>, dim3(b->n_seg), dim3(kBinThreads), lds, b->stream, pos, bb);
+        hipLaunchKernelGGL((k_bbin_place<U, T>), dim3(b->n_seg), dim3(T), lds, b->stream, pos, bb);
         bprobe_post(b);
+    }
+}
+
+// The rebinning launch for the handle's load batching.  Workgroups keep
+// kBinThreads threads: every scenario's segments (down to 2^9 buckets) run
+// in the same launch.
+inline void blaunch_bins(msim_batch* b, const float2* pos, const BatchBin& bb, size_t lds) {
+    if (b->bin_unroll == 8) {
+        blaunch_bins<8, kBinThreads>(b, pos, bb, lds);
+    } else {
+        blaunch_bins<4, kBinThreads>(b, pos, bb, lds);
     }
 }
 

@@ -25,6 +25,11 @@
 namespace msim {
 
 constexpr uint32_t kBinThreads = 512;
+
+// Rebinning workgroup size: segments of 2^14 buckets hold 64 KB of LDS
+// counters, so at most two workgroups share a CU; 1024 threads then keep
+// 32 waves per CU resident, as 512-thread groups do at smaller segments.
+inline uint32_t bin_threads_for(uint32_t seg_log) { return seg_log >= 14 ? 1024u : kBinThreads; }
 constexpr uint32_t kQueues = 64;  // arrival queues, one counter per 128-B line
 constexpr uint32_t kQueueStride = 32;  // counter stride in words
 
@@ -337,19 +342,19 @@ __device__ __forceinline__ uint32_t bucket_of_pos(float2 p, const TableView& t) 
 // counter hands out).  The trip count is block-uniform, so the wave-level rank
 // helper always runs with the whole wave; U independent loads per
 // thread keep enough requests in flight.
-template <bool PLACE, uint32_t U>
+template <bool PLACE, uint32_t U, uint32_t T>
 __device__ __forceinline__ void bin_pass(const float2* __restrict__ pos, const TableView& t, uint32_t s0,
                                          uint32_t lo, uint32_t hi, uint32_t* table, float2* __restrict__ out_pos,
                                          uint32_t* __restrict__ out_bid) {
     const uint32_t S = 1u << t.seg_log;
-    for (uint32_t base = lo; base < hi; base += U * kBinThreads) {
+    for (uint32_t base = lo; base < hi; base += U * T) {
         uint32_t id[U];
         float2 p[U];
         // unconditional loads (past hi they re-read slot hi - 1), so that all
         // of them are in flight before the first is used
 #pragma unroll
         for (uint32_t u = 0; u < U; u++) {
-            const uint32_t k = base + u * kBinThreads + threadIdx.x;
+            const uint32_t k = base + u * T + threadIdx.x;
             id[u] = t.old_bid[min(k, hi - 1u)];
         }
 #pragma unroll
@@ -358,7 +363,7 @@ __device__ __forceinline__ void bin_pass(const float2* __restrict__ pos, const T
         }
 #pragma unroll
         for (uint32_t u = 0; u < U; u++) {
-            const uint32_t k = base + u * kBinThreads + threadIdx.x;
+            const uint32_t k = base + u * T + threadIdx.x;
             const uint32_t off = bucket_of_pos(p[u], t) - s0;
             const bool mine = (k < hi) && (off < S);
             const uint32_t slot = wave_rank_add(table, off, mine);
@@ -372,28 +377,28 @@ __device__ __forceinline__ void bin_pass(const float2* __restrict__ pos, const T
 
 // Segment count: window agents landing in [s0, s0 + S) counted in LDS, local
 // exclusive offsets written to start_out[s0 + j]; returns the segment total.
-template <uint32_t U>
+template <uint32_t U, uint32_t T>
 __device__ __forceinline__ uint32_t bin_count_segment(const float2* __restrict__ pos, const TableView& t,
                                                       uint32_t s0, uint32_t* hist, uint32_t* lds,
                                                       uint32_t* __restrict__ start_out) {
     const uint32_t tid = threadIdx.x;
     const uint32_t S = 1u << t.seg_log;
-    for (uint32_t j = tid; j < S; j += kBinThreads) {
+    for (uint32_t j = tid; j < S; j += T) {
         hist[j] = 0u;
     }
     __syncthreads();
     uint32_t a0, a1, c0, c1;
     bin_window(t, s0, a0, a1, c0, c1);
-    bin_pass<false, U>(pos, t, s0, a0, a1, hist, nullptr, nullptr);
-    bin_pass<false, U>(pos, t, s0, c0, c1, hist, nullptr, nullptr);
+    bin_pass<false, U, T>(pos, t, s0, a0, a1, hist, nullptr, nullptr);
+    bin_pass<false, U, T>(pos, t, s0, c0, c1, hist, nullptr, nullptr);
     __syncthreads();
-    const uint32_t per = S / kBinThreads;
+    const uint32_t per = S / T;
     uint32_t run = 0;
     for (uint32_t q = 0; q < per; q++) {
         run += hist[tid * per + q];
     }
     uint32_t total = 0;
-    uint32_t excl = block_excl_scan<kBinThreads>(run, lds, total);
+    uint32_t excl = block_excl_scan<T>(run, lds, total);
     for (uint32_t q = 0; q < per; q++) {
         const uint32_t v = hist[tid * per + q];
         start_out[s0 + tid * per + q] = excl;
@@ -403,12 +408,12 @@ __device__ __forceinline__ uint32_t bin_count_segment(const float2* __restrict__
 }
 
 // Segment place: start_out[s0 + j] += base (absolute slots), agents placed.
-template <uint32_t U>
+template <uint32_t U, uint32_t T>
 __device__ __forceinline__ void bin_place_segment(const float2* __restrict__ pos, const TableView& t, uint32_t s0,
                                                   uint32_t base, uint32_t* cursor, uint32_t* __restrict__ start_out,
                                                   float2* __restrict__ out_pos, uint32_t* __restrict__ out_bid) {
     const uint32_t S = 1u << t.seg_log;
-    for (uint32_t j = threadIdx.x; j < S; j += kBinThreads) {
+    for (uint32_t j = threadIdx.x; j < S; j += T) {
         const uint32_t v = base + start_out[s0 + j];
         cursor[j] = v;
         start_out[s0 + j] = v;
@@ -416,8 +421,8 @@ __device__ __forceinline__ void bin_place_segment(const float2* __restrict__ pos
     __syncthreads();
     uint32_t a0, a1, c0, c1;
     bin_window(t, s0, a0, a1, c0, c1);
-    bin_pass<true, U>(pos, t, s0, a0, a1, cursor, out_pos, out_bid);
-    bin_pass<true, U>(pos, t, s0, c0, c1, cursor, out_pos, out_bid);
+    bin_pass<true, U, T>(pos, t, s0, a0, a1, cursor, out_pos, out_bid);
+    bin_pass<true, U, T>(pos, t, s0, c0, c1, cursor, out_pos, out_bid);
 }
 
 // ---- single-pass rebinning: count, decoupled look-back, place -------------
@@ -500,7 +505,7 @@ __device__ __forceinline__ void take_segment(unsigned long long* ticket, uint32_
 // Count, look back, place.  Writes the segment's absolute bucket offsets to
 // start_out[s0 + j]; *base_out / *total_out receive the segment's first slot
 // and agent count (all threads).
-template <uint32_t U>
+template <uint32_t U, uint32_t T>
 __device__ __forceinline__ void bin_fused_segment(const float2* __restrict__ pos, const TableView& t, uint32_t s0,
                                                   uint32_t* hist, uint32_t* lds, uint32_t* sh_base,
                                                   uint32_t* __restrict__ start_out, float2* __restrict__ out_pos,
@@ -509,22 +514,22 @@ __device__ __forceinline__ void bin_fused_segment(const float2* __restrict__ pos
                                                   unsigned int* faults, uint32_t* base_out, uint32_t* total_out) {
     const uint32_t tid = threadIdx.x;
     const uint32_t S = 1u << t.seg_log;
-    for (uint32_t j = tid; j < S; j += kBinThreads) {
+    for (uint32_t j = tid; j < S; j += T) {
         hist[j] = 0u;
     }
     __syncthreads();
     uint32_t a0, a1, c0, c1;
     bin_window(t, s0, a0, a1, c0, c1);
-    bin_pass<false, U>(pos, t, s0, a0, a1, hist, nullptr, nullptr);
-    bin_pass<false, U>(pos, t, s0, c0, c1, hist, nullptr, nullptr);
+    bin_pass<false, U, T>(pos, t, s0, a0, a1, hist, nullptr, nullptr);
+    bin_pass<false, U, T>(pos, t, s0, c0, c1, hist, nullptr, nullptr);
     __syncthreads();
-    const uint32_t per = S / kBinThreads;
+    const uint32_t per = S / T;
     uint32_t run = 0;
     for (uint32_t q = 0; q < per; q++) {
         run += hist[tid * per + q];
     }
     uint32_t total = 0;
-    uint32_t excl = block_excl_scan<kBinThreads>(run, lds, total);
+    uint32_t excl = block_excl_scan<T>(run, lds, total);
     for (uint32_t q = 0; q < per; q++) {  // counts -> segment-local exclusive offsets
         const uint32_t v = hist[tid * per + q];
         hist[tid * per + q] = excl;
@@ -535,28 +540,28 @@ __device__ __forceinline__ void bin_fused_segment(const float2* __restrict__ pos
     }
     __syncthreads();
     const uint32_t base = *sh_base;
-    for (uint32_t j = tid; j < S; j += kBinThreads) {
+    for (uint32_t j = tid; j < S; j += T) {
         const uint32_t v = base + hist[j];
         hist[j] = v;
         start_out[s0 + j] = v;
     }
     __syncthreads();
-    bin_pass<true, U>(pos, t, s0, a0, a1, hist, out_pos, out_bid);
-    bin_pass<true, U>(pos, t, s0, c0, c1, hist, out_pos, out_bid);
+    bin_pass<true, U, T>(pos, t, s0, a0, a1, hist, out_pos, out_bid);
+    bin_pass<true, U, T>(pos, t, s0, c0, c1, hist, out_pos, out_bid);
     *base_out = base;
     *total_out = total;
 }
 
 // Single-pass rebinning of one table (grid: n_seg workgroups).
-template <uint32_t U>
-static __global__ __launch_bounds__(kBinThreads, 8) void k_bin_fused(const float2* __restrict__ pos, BinDev b) {
+template <uint32_t U, uint32_t T>
+static __global__ __launch_bounds__(T, 8) void k_bin_fused(const float2* __restrict__ pos, BinDev b) {
     extern __shared__ uint32_t hist[];
-    __shared__ uint32_t lds[kBinThreads / 64];
+    __shared__ uint32_t lds[T / 64];
     __shared__ uint32_t sh[3];
     take_segment(b.ticket, b.n_seg, sh);
     const uint32_t seg = sh[0];
     uint32_t base, total;
-    bin_fused_segment<U>(pos, table_view(b), seg << b.seg_log, hist, lds, &sh[2], b.new_start, b.new_bpos, b.new_bid,
+    bin_fused_segment<U, T>(pos, table_view(b), seg << b.seg_log, hist, lds, &sh[2], b.new_start, b.new_bpos, b.new_bid,
                       b.status, seg, 0u, 0u, sh[1], &b.ctr->faults, &base, &total);
     if (seg == b.n_seg - 1 && threadIdx.x == 0) {
         b.new_start[b.m] = base + total;
@@ -568,28 +573,28 @@ static __global__ __launch_bounds__(kBinThreads, 8) void k_bin_fused(const float
 
 // Counts the window agents that land in segment blockIdx.x, writes the
 // segment-local exclusive offsets to new_start and the total to seg_total.
-template <uint32_t U>
-static __global__ __launch_bounds__(kBinThreads) void k_bin_count(const float2* __restrict__ pos, BinDev b) {
+template <uint32_t U, uint32_t T>
+static __global__ __launch_bounds__(T) void k_bin_count(const float2* __restrict__ pos, BinDev b) {
     extern __shared__ uint32_t hist[];
-    __shared__ uint32_t lds[kBinThreads / 64];
+    __shared__ uint32_t lds[T / 64];
     const uint32_t seg = blockIdx.x;
-    const uint32_t total = bin_count_segment<U>(pos, table_view(b), seg << b.seg_log, hist, lds, b.new_start);
+    const uint32_t total = bin_count_segment<U, T>(pos, table_view(b), seg << b.seg_log, hist, lds, b.new_start);
     if (threadIdx.x == 0) {
         b.seg_total[seg] = total;
     }
 }
 
 // Adds the segment's global offset and places its agents.
-template <uint32_t U>
-static __global__ __launch_bounds__(kBinThreads, 8) void k_bin_place(const float2* __restrict__ pos, BinDev b) {
+template <uint32_t U, uint32_t T>
+static __global__ __launch_bounds__(T, 8) void k_bin_place(const float2* __restrict__ pos, BinDev b) {
     extern __shared__ uint32_t cursor[];
-    __shared__ uint32_t lds[kBinThreads / 64];
+    __shared__ uint32_t lds[T / 64];
     const uint32_t seg = blockIdx.x;
     uint32_t before = 0;
-    for (uint32_t k = threadIdx.x; k < seg; k += kBinThreads) {
+    for (uint32_t k = threadIdx.x; k < seg; k += T) {
         before += b.seg_total[k];
     }
-    const uint32_t base = block_sum<kBinThreads>(before, lds);
+    const uint32_t base = block_sum<T>(before, lds);
     if (seg == b.n_seg - 1 && threadIdx.x == 0) {
         const uint32_t total = base + b.seg_total[seg];
         b.new_start[b.m] = total;
@@ -597,7 +602,7 @@ static __global__ __launch_bounds__(kBinThreads, 8) void k_bin_place(const float
             atomicOr(&b.ctr->faults, kFaultBinCount);
         }
     }
-    bin_place_segment<U>(pos, table_view(b), seg << b.seg_log, base, cursor, b.new_start, b.new_bpos, b.new_bid);
+    bin_place_segment<U, T>(pos, table_view(b), seg << b.seg_log, base, cursor, b.new_start, b.new_bpos, b.new_bid);
 }
 
 }  // namespace msim

@@ -393,8 +393,8 @@ void dbins(msim_batch* b, const DiffState& d, const uint32_t* old_start, const u
     bb.inv_cell = b->inv_cell;
     const size_t lds = sizeof(uint32_t) << b->max_seg_log;
     const float2* pos = b->buf[b->cur].pos;
-    hipLaunchKernelGGL(k_bbin_count<4>, dim3(b->n_seg), dim3(kBinThreads), lds, b->stream, pos, bb);
-    hipLaunchKernelGGL(k_bbin_place<4>, dim3(b->n_seg), dim3(kBinThreads), lds, b->stream, pos, bb);
+    hipLaunchKernelGGL((k_bbin_count<4, kBinThreads>), dim3(b->n_seg), dim3(kBinThreads), lds, b->stream, pos, bb);
+    hipLaunchKernelGGL((k_bbin_place<4, kBinThreads>), dim3(b->n_seg), dim3(kBinThreads), lds, b->stream, pos, bb);
 }
 
 DiffCells dcells(msim_batch* b, const uint32_t* start, const float2* bpos, const uint32_t* bid) {

@@ -450,8 +450,9 @@ __device__ __forceinline__ void wave_lds_sync() {
 constexpr uint32_t kStage = 512;  // binned positions staged per wave round (4 x 16 B per lane, 4 KB)
 constexpr uint32_t kStageLoads = kStage / 128;
 
-// Pairs of staged candidates [j0, j1) of one lane within range of `me`; two
-// LDS reads are issued before either is used.
+// Staged candidates [j0, j1) of one lane within range of `me`; two LDS reads
+// are issued before either is used.  (Four per round measured slower: the
+// lanes' remainders diverge.)
 __device__ __forceinline__ uint32_t count_staged(const float2* stage, uint32_t j0, uint32_t j1, float2 me,
                                                  float T) {
     uint32_t cnt = 0;

@@ -239,18 +239,18 @@ void probe_post(msim_sim* s) {
 }
 
 // Segment rebinning: one pass (k_bin_fused) or count + place.
-template <uint32_t U>
+template <uint32_t U, uint32_t T>
 void launch_bins(msim_sim* s, const float2* pos, const BinDev& b, size_t lds) {
     if (s->binning == 2) {
         probe_pre(s);
-        hipLaunchKernelGGL(k_bin_fused<U>, dim3(s->n_seg), dim3(kBinThreads), lds, s->stream, pos, b);
+        hipLaunchKernelGGL((k_bin_fused<U, T>), dim3(s->n_seg), dim3(T), lds, s->stream, pos, b);
         probe_post(s);
     } else {
         probe_pre(s);
-        hipLaunchKernelGGL(k_bin_count<U>, dim3(s->n_seg), dim3(kBinThreads), lds, s->stream, pos, b);
+        hipLaunchKernelGGL((k_bin_count<U, T>), dim3(s->n_seg), dim3(T), lds, s->stream, pos, b);
         probe_post(s);
         probe_pre(s);
-        hipLaunchKernelGGL(k_bin_place<U>, dim3(s->n_seg), dim3(kBinThreads), lds, s->stream, pos, b);
+        hipLaunchKernelGGL((k_bin_place<U, T>), dim3(s->n_seg), dim3(T), lds, s->stream, pos, b);
         probe_post(s);
     }
 }
@@ -340,10 +340,13 @@ int enqueue_move_fast(msim_sim* s, bool write_dir, hipEvent_t* ev) {
         b.margin = s->margin;
         b.n = s->n;
         const size_t lds = sizeof(uint32_t) << s->seg_log;
+        const bool wide = bin_threads_for(s->seg_log) == 1024u;
         if (s->bin_unroll == 8) {
-            launch_bins<8>(s, pos, b, lds);
+            if (wide) launch_bins<8, 1024>(s, pos, b, lds);
+            else launch_bins<8, 512>(s, pos, b, lds);
         } else {
-            launch_bins<4>(s, pos, b, lds);
+            if (wide) launch_bins<4, 1024>(s, pos, b, lds);
+            else launch_bins<4, 512>(s, pos, b, lds);
         }
         s->bcur = nb;
     } else {

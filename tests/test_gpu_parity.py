@@ -101,7 +101,8 @@ def test_dispatch_loop_equals_fused_graph_path():
                                   dict(binning=2, table_log2=16, segment_log2=13, table_width_log2=8),
                                   dict(binning=2, route_queue=1, graph_ticks=1),
                                   dict(collide_mode=2), dict(collide_mode=2, segment_log2=9, table_width_log2=3),
-                                  dict(bin_unroll=4), dict(bin_unroll=8, binning=2, table_log2=16, segment_log2=14)])
+                                  dict(bin_unroll=4), dict(bin_unroll=8, binning=2, table_log2=16, segment_log2=14),
+                                  dict(table_log2=16, segment_log2=14)])
 def test_options_do_not_change_results(opts):
     sc = S.config(1, scale=0.5)
     g, o = pair(sc, **opts)
