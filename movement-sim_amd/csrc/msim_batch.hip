@@ -65,8 +65,7 @@ int benqueue_move(msim_batch* b, bool write_dir) {
 
 int breorder(msim_batch* b) {
     const int nxt = 1 - b->cur;
-    hipLaunchKernelGGL(k_reorder, dim3(bblocks(b->n)), dim3(kBlock), 0, b->stream, b->agents(b->cur),
-                       b->agents(nxt), bgrid(b));
+    launch_reorder(b->agents(b->cur), b->agents(nxt), b->bbid[b->bcur], b->n, b->stream);
     int rc = bcheck(b, "k_reorder");
     if (rc != MSIM_OK) return rc;
     b->cur = nxt;

@@ -803,26 +803,49 @@ static __global__ __launch_bounds__(64) void k_collide_finish(unsigned long long
 
 // ---- spatial renumbering -------------------------------------------------
 
-// Gathers every per-agent array into bucket order so that agents that are
-// close in space are close in memory; afterwards bid is the identity.
-static __global__ __launch_bounds__(kBlock) void k_reorder(AgentsDev src, AgentsDev dst, GridDev g) {
+// Gathers the per-agent arrays into bucket order so that agents that are
+// close in space are close in memory: workgroup row blockIdx.y gathers one
+// array (0 pos, 1 tgt, 2 dir, 3 road, 4 rng, 5 init, 6 ncount, 7 perm,
+// 8 spd when present).  Rows are dispatched one after the other, so the
+// resident workgroups mostly gather from the same array and share its cache
+// lines (a gather of all arrays per slot spread its reads over every
+// array's lines at once).  k_bid_identity then resets bid.
+static __global__ __launch_bounds__(kBlock) void k_reorder(AgentsDev src, AgentsDev dst,
+                                                    const uint32_t* __restrict__ bid) {
     const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (s >= src.n) {
         return;
     }
-    const uint32_t j = g.bid[s];
-    dst.pos[s] = src.pos[j];
-    dst.tgt[s] = src.tgt[j];
-    dst.dir[s] = src.dir[j];
-    if (src.spd) {
-        dst.spd[s] = src.spd[j];
+    const uint32_t j = bid[s];
+    switch (blockIdx.y) {
+        case 0: dst.pos[s] = src.pos[j]; break;
+        case 1: dst.tgt[s] = src.tgt[j]; break;
+        case 2: dst.dir[s] = src.dir[j]; break;
+        case 3: dst.road[s] = src.road[j]; break;
+        case 4: dst.rng[s] = src.rng[j]; break;
+        case 5: dst.init[s] = src.init[j]; break;
+        case 6: dst.ncount[s] = src.ncount[j]; break;
+        case 7: dst.perm[s] = src.perm[j]; break;
+        default: dst.spd[s] = src.spd[j]; break;
     }
-    dst.road[s] = src.road[j];
-    dst.rng[s] = src.rng[j];
-    dst.init[s] = src.init[j];
-    dst.ncount[s] = src.ncount[j];
-    dst.perm[s] = src.perm[j];
-    g.bid[s] = s;
+}
+
+// Agent index = slot after a renumbering.
+static __global__ __launch_bounds__(kBlock) void k_bid_identity(uint32_t* __restrict__ bid, uint32_t n) {
+    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    if (s < n) {
+        bid[s] = s;
+    }
+}
+
+// Launches the renumbering of n agents from src into dst (bid: the binning's
+// slot -> agent map, reset to the identity).
+inline void launch_reorder(const AgentsDev& src, const AgentsDev& dst, uint32_t* bid, uint32_t n,
+                           hipStream_t stream) {
+    const uint32_t nb = std::max(1u, (n + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_reorder, dim3(nb, src.spd ? 9u : 8u), dim3(kBlock), 0, stream, src, dst,
+                       static_cast<const uint32_t*>(bid));
+    hipLaunchKernelGGL(k_bid_identity, dim3(nb), dim3(kBlock), 0, stream, bid, n);
 }
 
 // ---- Entity records (64 B, the layout of sim::Entity, Entity.hpp:33-46) ----

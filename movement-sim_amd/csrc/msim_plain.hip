@@ -466,8 +466,7 @@ void drop_graphs(msim_sim* s) {
 
 int reorder(msim_sim* s) {
     const int nxt = 1 - s->cur;
-    hipLaunchKernelGGL(k_reorder, dim3(agent_blocks(s->n)), dim3(kBlock), 0, s->stream, s->agents(s->cur),
-                       s->agents(nxt), s->grid_cur());
+    launch_reorder(s->agents(s->cur), s->agents(nxt), s->grid_cur().bid, s->n, s->stream);
     int rc = check_launch(s, "k_reorder");
     if (rc != MSIM_OK) {
         return rc;
