@@ -455,15 +455,17 @@ constexpr uint32_t kStageLoads = kStage / 128;
 // lanes' remainders diverge.)
 __device__ __forceinline__ uint32_t count_staged(const float2* stage, uint32_t j0, uint32_t j1, float2 me,
                                                  float T) {
+    // one induction variable (the LDS address) and one compare per round
     uint32_t cnt = 0;
-    uint32_t j = j0;
-    for (; j + 2u <= j1; j += 2u) {
-        const float2 q0 = stage[j];
-        const float2 q1 = stage[j + 1u];
+    const float2* q = stage + j0;
+    const float2* const pair_end = stage + (j0 + ((j1 - j0) & ~1u));
+    for (; q != pair_end; q += 2) {
+        const float2 q0 = q[0];
+        const float2 q1 = q[1];
         cnt += (in_range(q0, me, T) ? 1u : 0u) + (in_range(q1, me, T) ? 1u : 0u);
     }
-    if (j < j1) {
-        cnt += in_range(stage[j], me, T) ? 1u : 0u;
+    if ((j1 - j0) & 1u) {
+        cnt += in_range(*q, me, T) ? 1u : 0u;
     }
     return cnt;
 }
