@@ -236,7 +236,7 @@ def adjoint_line(args):
     from msim import scenarios as S
 
     scs = S.adjoint_config()
-    res = calibrate.calibrate(scs, ticks=1000, steps=args.steps,
+    res = calibrate.calibrate(scs, ticks=1000, steps=args.steps, arrival_gradient=args.arrival_gradient,
                               log=lambda m: print("# " + m, file=sys.stderr, flush=True))
     n = res["agents"]
     sec = res["seconds_per_gradient"]
@@ -257,6 +257,7 @@ def adjoint_line(args):
         "config": {"workload": WORKLOADS[5], "agents": n, "roads_per_map": res["roads_per_map"],
                    "ticks": res["ticks"], "checkpoint_every": 50},
         "adjoint": {
+            "arrival_gradient": res["arrival_gradient"],
             "seconds_per_gradient": sec,
             "hidden_run_seconds": res["hidden_run_seconds"],
             "loss_m2_first_last": [h[0]["loss_m2"], h[-1]["loss_m2"]],
@@ -289,6 +290,8 @@ def main():
     ap.add_argument("--collide-mode", type=int, default=None, help="override the tuned neighbour walk (0 staged, "
                                                                     "2 slots per lane; 1 per lane; 2 staged, 1 slot)")
     ap.add_argument("--bin-unroll", type=int, default=None, help="override the tuned rebinning load batching (4, 8)")
+    ap.add_argument("--arrival-gradient", choices=("frozen", "carry"), default="carry",
+                    help="calibration (--config 5): how the adjoint treats arrivals")
     ap.add_argument("--profile-ticks", type=int, default=240)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")

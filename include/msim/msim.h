@@ -260,8 +260,18 @@ int msim_batch_diff_forward(msim_batch* b, const float* theta, float alpha, int 
 int msim_batch_diff_backward(msim_batch* b, const float* grad_pos, double* grad_theta, double* grad_alpha);
 
 /* One tick's recorded decisions (host, caller order, packed road indices):
- * target (2 floats per agent), road, flag (bit 0 arrived, bit 1 no move). */
+ * target (2 floats per agent), road, flag (bit 0 arrived, bit 1 no move).
+ * tick == n_ticks gives the decisions in force after the run (flag 0). */
 int msim_batch_diff_read_schedule(msim_batch* b, uint32_t tick, float* target, uint32_t* road, uint8_t* flag);
+
+/* How msim_batch_diff_backward treats arrivals: 0 (default) differentiates
+ * the frozen-decision run, in which an arrival's snap to its target cuts the
+ * gradient; 1 differentiates each arrival as if it carried its leftover time
+ * onto the next road, p' = T + (1 - L/s) s2 u2 (L: distance to the target T
+ * before the move, s and s2: the two roads' speeds, u2: the next road's
+ * direction), so earlier roads' multipliers see how they shift later
+ * positions.  Mode 1 is a surrogate: the run itself still snaps. */
+int msim_batch_diff_set_arrival_gradient(msim_batch* b, int mode);
 
 const char* msim_last_error(const msim_sim* sim);
 const char* msim_get_last_error(void);

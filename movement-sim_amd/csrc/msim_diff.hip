@@ -216,10 +216,18 @@ static __global__ __launch_bounds__(kBlock) void k_davoid_adj(DiffCells c, doubl
 }
 
 // Move adjoint in agent order, binary64 from the run's binary32 values.
+// An arrival's snap is constant under frozen decisions (λ = 0).  With carry,
+// it is differentiated as if the arrival carried its leftover time onto the
+// next road (nxt_tgt, nxt_road: the decisions after this tick),
+// p' = T + (1 - L/s) s2 u2 with L = |T - p|, u2 the next road's direction and
+// s, s2 the two roads' speeds: λ = (s2/s) u (u2·λ'), ∂/∂θ[road] += v L s2/s²
+// (u2·λ'), ∂/∂θ[next road] += v (1 - L/s)(u2·λ').
 template <bool UNIFORM>
 static __global__ __launch_bounds__(kBlock) void k_dmove_adj(uint32_t n, const float* __restrict__ spd,
                                                       float spd_const, const float* __restrict__ theta,
-                                                      DiffTapeDev tp, const double2* __restrict__ lam_in,
+                                                      DiffTapeDev tp, const float2* __restrict__ nxt_tgt,
+                                                      const uint32_t* __restrict__ nxt_road, uint32_t carry,
+                                                      const double2* __restrict__ lam_in,
                                                       double2* __restrict__ lam_out,
                                                       double* __restrict__ gtheta) {
     const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
@@ -229,7 +237,32 @@ static __global__ __launch_bounds__(kBlock) void k_dmove_adj(uint32_t n, const f
     const double2 l = lam_in[k];
     const uint8_t flag = tp.flag[k];
     if (flag & kDiffArrived) {
-        lam_out[k] = make_double2(0.0, 0.0);
+        double2 out = make_double2(0.0, 0.0);
+        if (carry) {
+            const float2 p = tp.pos[k];
+            const float2 T = tp.tgt[k];
+            const float2 T2 = nxt_tgt[k];
+            const double ex = static_cast<double>(T2.x) - T.x;
+            const double ey = static_cast<double>(T2.y) - T.y;
+            const double E = sqrt(ex * ex + ey * ey);
+            if (E > 0.0) {
+                const uint32_t r = tp.road[k];
+                const uint32_t r2 = nxt_road[k];
+                const float v = UNIFORM ? spd_const : spd[k];
+                const double s = static_cast<double>(v * theta[r]);
+                const double s2 = static_cast<double>(v * theta[r2]);
+                const double dx = static_cast<double>(T.x) - p.x;
+                const double dy = static_cast<double>(T.y) - p.y;
+                const double L = sqrt(dx * dx + dy * dy);
+                const double ux = L > 0.0 ? dx / L : 0.0;
+                const double uy = L > 0.0 ? dy / L : 0.0;
+                const double g2 = (ex / E) * l.x + (ey / E) * l.y;
+                out = make_double2(ux * (s2 / s) * g2, uy * (s2 / s) * g2);
+                atomicAdd(&gtheta[r], static_cast<double>(v) * (L * s2 / (s * s)) * g2);
+                atomicAdd(&gtheta[r2], static_cast<double>(v) * (1.0 - L / s) * g2);
+            }
+        }
+        lam_out[k] = out;
         return;
     }
     if (flag & kDiffNoMove) {
@@ -288,6 +321,7 @@ struct DiffState {
     uint32_t T = 0, C = 50, n_ck = 0;
     bool keep_schedule = false;
     bool no_move_first = false;
+    bool carry = false;  // arrivals differentiated as carrying their leftover time
     bool have_forward = false;
     float alpha = 0.0f;      // parameters of the last recorded forward run
     float theta_max = 1.0f;
@@ -301,7 +335,7 @@ struct DiffState {
     uint32_t* tp_start = nullptr;  // [C][start_len]
     uint32_t* tp_bid = nullptr;    // [C][n]
     float2* tp_bpos = nullptr;     // [C][n + 2]
-    DiffTapeDev sched{};     // [T][n] decisions of the whole run (keep_schedule)
+    DiffTapeDev sched{};     // [T + 1][n] decisions of the whole run and those after it (keep_schedule)
     double2* lam = nullptr;  // adjoint state, agent order
     double2* lam2 = nullptr;
     double* gtheta = nullptr;
@@ -498,10 +532,10 @@ extern "C" int msim_batch_diff_setup(msim_batch* b, uint32_t n_ticks, uint32_t c
     if (rc == MSIM_OK) rc = balloc_n(b, &d->tp_bid, n * d->C);
     if (rc == MSIM_OK) rc = balloc_n(b, &d->tp_bpos, (n + 2) * d->C);
     if (rc == MSIM_OK && d->keep_schedule) {
-        rc = balloc_n(b, &d->sched.pos, n * n_ticks);
-        if (rc == MSIM_OK) rc = balloc_n(b, &d->sched.tgt, n * n_ticks);
-        if (rc == MSIM_OK) rc = balloc_n(b, &d->sched.road, n * n_ticks);
-        if (rc == MSIM_OK) rc = balloc_n(b, &d->sched.flag, n * n_ticks);
+        rc = balloc_n(b, &d->sched.pos, n * (n_ticks + 1u));
+        if (rc == MSIM_OK) rc = balloc_n(b, &d->sched.tgt, n * (n_ticks + 1u));
+        if (rc == MSIM_OK) rc = balloc_n(b, &d->sched.road, n * (n_ticks + 1u));
+        if (rc == MSIM_OK) rc = balloc_n(b, &d->sched.flag, n * (n_ticks + 1u));
     }
     if (rc == MSIM_OK) rc = balloc_n(b, &d->lam, n);
     if (rc == MSIM_OK) rc = balloc_n(b, &d->lam2, n);
@@ -597,6 +631,20 @@ extern "C" int msim_batch_diff_forward(msim_batch* b, const float* theta, float 
         b->bcur = nb;
         rc = bcheck(b, "differentiable tick");
     }
+    if (rc == MSIM_OK && !replay && d.keep_schedule && b->n) {
+        // the decisions in force after the last tick (schedule entry T)
+        const DiffTapeDev fin = tape_at(d.sched, b->n, d.T);
+        const AgentsDev a = b->agents(b->cur);
+        hipError_t e = hipMemcpyAsync(fin.pos, a.pos, sizeof(float2) * b->n, hipMemcpyDeviceToDevice, b->stream);
+        if (e == hipSuccess) {
+            e = hipMemcpyAsync(fin.tgt, a.tgt, sizeof(float2) * b->n, hipMemcpyDeviceToDevice, b->stream);
+        }
+        if (e == hipSuccess) {
+            e = hipMemcpyAsync(fin.road, a.road, sizeof(uint32_t) * b->n, hipMemcpyDeviceToDevice, b->stream);
+        }
+        if (e == hipSuccess) e = hipMemsetAsync(fin.flag, 0, b->n, b->stream);
+        if (e != hipSuccess) rc = bfail(b, MSIM_ERR_HIP, "schedule copy: %s", hipGetErrorString(e));
+    }
     if (rc == MSIM_OK && final_pos) {
         hipLaunchKernelGGL(k_dscatter, dim3(bblocks(b->n)), dim3(kBlock), 0, b->stream, b->n, b->buf[b->cur].perm,
                            b->buf[b->cur].pos, reinterpret_cast<float2*>(final_pos));
@@ -645,12 +693,17 @@ extern "C" int msim_batch_diff_backward(msim_batch* b, const float* grad_pos, do
             hipLaunchKernelGGL(k_davoid_adj, g, dim3(kBlock), 0, b->stream, c, static_cast<double>(d.alpha), d.lam,
                                d.lam2, d.galpha);
             const AgentsDev a = b->agents(b->cur);
+            // the decisions after tick t: the tape's next entry, or after the
+            // segment's last tick the agent state its replay ended in
+            const float2* nt = t + 1 < t1 ? tape_at(d.tape, n, j + 1).tgt : a.tgt;
+            const uint32_t* nr = t + 1 < t1 ? tape_at(d.tape, n, j + 1).road : a.road;
+            const uint32_t carry = d.carry ? 1u : 0u;
             if (b->uniform) {
                 hipLaunchKernelGGL(k_dmove_adj<true>, g, dim3(kBlock), 0, b->stream, b->n, a.spd, a.spd_const,
-                                   d.theta, tape_at(d.tape, n, j), d.lam2, d.lam, d.gtheta);
+                                   d.theta, tape_at(d.tape, n, j), nt, nr, carry, d.lam2, d.lam, d.gtheta);
             } else {
                 hipLaunchKernelGGL(k_dmove_adj<false>, g, dim3(kBlock), 0, b->stream, b->n, a.spd, a.spd_const,
-                                   d.theta, tape_at(d.tape, n, j), d.lam2, d.lam, d.gtheta);
+                                   d.theta, tape_at(d.tape, n, j), nt, nr, carry, d.lam2, d.lam, d.gtheta);
             }
             rc = bcheck(b, "adjoint tick");
         }
@@ -663,12 +716,19 @@ extern "C" int msim_batch_diff_backward(msim_batch* b, const float* grad_pos, do
     return rc;
 }
 
+extern "C" int msim_batch_diff_set_arrival_gradient(msim_batch* b, int mode) {
+    if (!b || !b->diff) return bfail(b, MSIM_ERR_STATE, "msim_batch_diff_setup first");
+    if (mode != 0 && mode != 1) return bfail(b, MSIM_ERR_INVALID, "arrival gradient mode must be 0 or 1");
+    b->diff->carry = mode == 1;
+    return MSIM_OK;
+}
+
 extern "C" int msim_batch_diff_read_schedule(msim_batch* b, uint32_t tick, float* target, uint32_t* road,
                                              uint8_t* flag) {
     if (!b || !b->diff) return bfail(b, MSIM_ERR_STATE, "msim_batch_diff_setup first");
     DiffState& d = *b->diff;
     if (!(d.keep_schedule && d.have_forward)) return bfail(b, MSIM_ERR_STATE, "no recorded schedule");
-    if (tick >= d.T) return bfail(b, MSIM_ERR_INVALID, "tick %u beyond the run's %u ticks", tick, d.T);
+    if (tick > d.T) return bfail(b, MSIM_ERR_INVALID, "tick %u beyond the run's %u ticks", tick, d.T);
     int rc = msim_batch_sync(b);
     if (rc != MSIM_OK) return rc;
     const size_t n = b->n;

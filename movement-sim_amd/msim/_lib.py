@@ -104,6 +104,7 @@ SIGNATURES = {
     "msim_batch_diff_forward": (ctypes.c_int, [_VP, _VP, ctypes.c_float, ctypes.c_int, _VP]),
     "msim_batch_diff_backward": (ctypes.c_int, [_VP, _VP, _VP, _VP]),
     "msim_batch_diff_read_schedule": (ctypes.c_int, [_VP, _U32, _VP, _VP, _VP]),
+    "msim_batch_diff_set_arrival_gradient": (ctypes.c_int, [_VP, ctypes.c_int]),
     # host driver (msim_host.h)
     "msim_host_map_load": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_VP)]),
     "msim_host_map_info": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP, _VP]),
@@ -427,8 +428,13 @@ class Batch:
                                                       ctypes.c_void_p(grad_theta_ptr),
                                                       ctypes.c_void_p(grad_alpha_ptr)))
 
+    def diff_set_arrival_gradient(self, mode):
+        """0: arrivals cut the gradient (frozen run); 1: carry-over surrogate."""
+        self._check(self.lib.msim_batch_diff_set_arrival_gradient(self.h, int(mode)))
+
     def diff_schedule(self, tick):
-        """(targets [n, 2], packed roads [n], flags [n]) of one recorded tick."""
+        """(targets [n, 2], packed roads [n], flags [n]) of one recorded tick;
+        tick == ticks gives the decisions in force after the run."""
         t = np.zeros((self.n, 2), np.float32)
         r = np.zeros(self.n, np.uint32)
         f = np.zeros(self.n, np.uint8)

@@ -9,7 +9,9 @@ the hand-written adjoint of libmsim (msim_batch_diff_*) as its backward.
 theta holds one multiplier per packed road of the batch (scenarios sharing a
 map tie theirs with theta_map.repeat(n_scen)).  Every call simulates from the
 batch's state at setup.  The gradient freezes the run's decisions (arrival
-ticks, route choices) and neighbour cells.
+ticks, route choices) and neighbour cells.  arrival_gradient="carry"
+differentiates each arrival as if it carried its leftover time onto the next
+road (a surrogate: with "frozen", the snap to the target cuts the gradient).
 """
 import torch
 
@@ -38,8 +40,11 @@ class _DiffFn(torch.autograd.Function):
 class DiffRun:
     """A batched handle set up for differentiable runs of `ticks` ticks."""
 
-    def __init__(self, batch, ticks, checkpoint_every=50, keep_schedule=False):
+    def __init__(self, batch, ticks, checkpoint_every=50, keep_schedule=False, arrival_gradient="frozen"):
+        if arrival_gradient not in ("frozen", "carry"):
+            raise ValueError("arrival_gradient must be 'frozen' or 'carry'")
         batch.diff_setup(ticks, checkpoint_every, keep_schedule)
+        batch.diff_set_arrival_gradient(1 if arrival_gradient == "carry" else 0)
         self.batch = batch
         self.ticks = ticks
 

@@ -4,12 +4,17 @@ by gradient descent through the differentiable tick.
 A hidden-parameter run (theta_true, alpha_true) of the scenarios gives the
 observed final positions; starting from theta = 1 and alpha = alpha0, Adam
 minimises the mean squared distance between simulated and observed final
-positions, with gradients from the hand-written adjoint (msim.adjoint).
+positions, with gradients from the hand-written adjoint (msim.adjoint).  By
+default the adjoint differentiates arrivals as carrying their leftover time
+onto the next road (arrival_gradient="carry"): with the frozen run's own
+gradient ("frozen"), an arrival's snap cuts every earlier road's influence
+and the multipliers barely improve.
 Scenarios share one map, so theta has one entry per map road and is tied
 across the scenarios' packed copies.  theta = exp(phi) and alpha = exp(psi)
 keep both positive.
 
     python -m msim.calibrate [--scenarios 64] [--agents 100000] [--ticks 1000] [--steps 50]
+                             [--arrival-gradient frozen|carry]
 """
 import argparse
 import json
@@ -24,7 +29,7 @@ from .adjoint import DiffRun
 
 
 def calibrate(scs, ticks=1000, steps=50, lr=0.01, lr_alpha=0.05, alpha_true=0.02, alpha0=0.01, sigma=0.2,
-              checkpoint_every=50, seed=0, log=None):
+              checkpoint_every=50, seed=0, arrival_gradient="carry", log=None):
     dev = torch.device("cuda")
     n_map = scs[0].roads.shape[0]
     rng = np.random.default_rng(seed)
@@ -38,7 +43,7 @@ def calibrate(scs, ticks=1000, steps=50, lr=0.01, lr_alpha=0.05, alpha_true=0.02
     t_hidden = time.perf_counter() - t0
     truth.batch.close()
 
-    run = DiffRun(Batch(scs), ticks, checkpoint_every)
+    run = DiffRun(Batch(scs), ticks, checkpoint_every, arrival_gradient=arrival_gradient)
     phi = torch.zeros(n_map, device=dev, requires_grad=True)
     psi = torch.tensor(float(np.log(alpha0)), device=dev, requires_grad=True)
     opt = torch.optim.Adam([{"params": [phi], "lr": lr}, {"params": [psi], "lr": lr_alpha}])
@@ -71,6 +76,7 @@ def calibrate(scs, ticks=1000, steps=50, lr=0.01, lr_alpha=0.05, alpha_true=0.02
         "roads_per_map": int(n_map),
         "ticks": ticks,
         "steps": steps,
+        "arrival_gradient": arrival_gradient,
         "alpha_true": alpha_true,
         "hidden_run_seconds": t_hidden,
         "seconds_per_gradient": float(np.median(grad_s)) if grad_s else None,
@@ -86,9 +92,12 @@ def main():
     ap.add_argument("--nodes", type=int, default=20_000)
     ap.add_argument("--ticks", type=int, default=1000)
     ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--lr", type=float, default=0.01)
+    ap.add_argument("--arrival-gradient", choices=("frozen", "carry"), default="carry")
     a = ap.parse_args()
     scs = S.adjoint_config(a.scenarios, a.agents, a.nodes)
-    out = calibrate(scs, a.ticks, a.steps, log=lambda m: print(m, flush=True))
+    out = calibrate(scs, a.ticks, a.steps, lr=a.lr, arrival_gradient=a.arrival_gradient,
+                    log=lambda m: print(m, flush=True))
     print(json.dumps(out))
 
 

@@ -202,9 +202,47 @@ int orc_diff_record(uint32_t n, uint32_t n_scen, const uint32_t* agent_off, floa
 
 /* binary64 forward under a recorded schedule; traj (nullable) receives the
  * pre-move and post-move positions of every tick: [T][2][n][2]. */
+/* The decision in force after tick t for agent i: the next tick's target and
+ * road, or the run's final ones after its last tick. */
+static void next_decision(uint32_t n, uint32_t n_ticks, uint32_t t, uint32_t i, const float* sched_tgt,
+                          const uint32_t* sched_road, const float* final_tgt, const uint32_t* final_road,
+                          double* tx, double* ty, uint32_t* r) {
+    if (t + 1 < n_ticks) {
+        *tx = sched_tgt[((size_t)(t + 1) * n + i) * 2];
+        *ty = sched_tgt[((size_t)(t + 1) * n + i) * 2 + 1];
+        *r = sched_road[(size_t)(t + 1) * n + i];
+    } else {
+        *tx = final_tgt[2 * (size_t)i];
+        *ty = final_tgt[2 * (size_t)i + 1];
+        *r = final_road[i];
+    }
+}
+
+/* Carry-over geometry of an arrival from p at target T, the next target T2:
+ * the distance L to T, the unit direction u towards T (0 when L = 0) and u2
+ * along the next road.  Returns 0 when the next road has no direction
+ * (T2 == T), where nothing is carried. */
+static int carry_geom(double px, double py, double Tx, double Ty, double T2x, double T2y, double* L, double* ux,
+                      double* uy, double* u2x, double* u2y) {
+    double dx = Tx - px, dy = Ty - py;
+    double ex = T2x - Tx, ey = T2y - Ty;
+    double E = sqrt(ex * ex + ey * ey);
+    *L = sqrt(dx * dx + dy * dy);
+    if (!(E > 0.0)) return 0;
+    *ux = *L > 0.0 ? dx / *L : 0.0;
+    *uy = *L > 0.0 ? dy / *L : 0.0;
+    *u2x = ex / E;
+    *u2y = ey / E;
+    return 1;
+}
+
+/* carry != 0: an arrival carries its leftover time onto the next road,
+ * p' = T + (1 - L/s) s2 u2 (s2: the next road's speed), instead of stopping
+ * at T; final_tgt/final_road give the decisions after the last tick. */
 static int replay64(uint32_t n, uint32_t n_scen, const uint32_t* agent_off, const float* pos0, const float* speed,
                     const double* theta, double alpha, double radius, uint32_t n_ticks, const float* sched_tgt,
-                    const uint32_t* sched_road, const uint8_t* sched_flag, double* pos_out, double* traj) {
+                    const uint32_t* sched_road, const uint8_t* sched_flag, const float* final_tgt,
+                    const uint32_t* final_road, int carry, double* pos_out, double* traj) {
     const double cell = (double)orc_cell_size((float)radius);
     double* p = (double*)malloc(sizeof(double) * 2 * (n ? n : 1));
     double* pt = (double*)malloc(sizeof(double) * 2 * (n ? n : 1));
@@ -225,6 +263,17 @@ static int replay64(uint32_t n, uint32_t n_scen, const uint32_t* agent_off, cons
             } else if (f & 1) {
                 pt[2 * i] = Tx;
                 pt[2 * i + 1] = Ty;
+                double T2x, T2y, L, ux, uy, u2x, u2y;
+                uint32_t r2;
+                if (carry) {
+                    next_decision(n, n_ticks, t, i, sched_tgt, sched_road, final_tgt, final_road, &T2x, &T2y, &r2);
+                    if (carry_geom(p[2 * i], p[2 * i + 1], Tx, Ty, T2x, T2y, &L, &ux, &uy, &u2x, &u2y)) {
+                        double v = (double)(speed ? speed[i] : ORC_SPEED);
+                        double sp = v * theta[sched_road[(size_t)t * n + i]], s2 = v * theta[r2];
+                        pt[2 * i] = Tx + (1.0 - L / sp) * s2 * u2x;
+                        pt[2 * i + 1] = Ty + (1.0 - L / sp) * s2 * u2y;
+                    }
+                }
             } else {
                 double s = (double)(speed ? speed[i] : ORC_SPEED) * theta[sched_road[(size_t)t * n + i]];
                 double dx = Tx - p[2 * i], dy = Ty - p[2 * i + 1];
@@ -273,13 +322,31 @@ int orc_diff_replay64(uint32_t n, uint32_t n_scen, const uint32_t* agent_off, co
                       const float* sched_tgt, const uint32_t* sched_road, const uint8_t* sched_flag,
                       double* pos_out) {
     return replay64(n, n_scen, agent_off, pos0, speed, theta, alpha, radius, n_ticks, sched_tgt, sched_road,
-                    sched_flag, pos_out, NULL);
+                    sched_flag, NULL, NULL, 0, pos_out, NULL);
+}
+
+int orc_diff_replay64_carry(uint32_t n, uint32_t n_scen, const uint32_t* agent_off, const float* pos0,
+                            const float* speed, const double* theta, double alpha, double radius, uint32_t n_ticks,
+                            const float* sched_tgt, const uint32_t* sched_road, const uint8_t* sched_flag,
+                            const float* final_tgt, const uint32_t* final_road, double* pos_out) {
+    return replay64(n, n_scen, agent_off, pos0, speed, theta, alpha, radius, n_ticks, sched_tgt, sched_road,
+                    sched_flag, final_tgt, final_road, 1, pos_out, NULL);
 }
 
 int orc_diff_adjoint64(uint32_t n, uint32_t n_scen, const uint32_t* agent_off, const float* pos0,
                        const float* speed, const double* theta, uint32_t n_roads, double alpha, double radius,
                        uint32_t n_ticks, const float* sched_tgt, const uint32_t* sched_road,
                        const uint8_t* sched_flag, const double* grad_pos, double* grad_theta, double* grad_alpha) {
+    return orc_diff_adjoint64_ex(n, n_scen, agent_off, pos0, speed, theta, n_roads, alpha, radius, n_ticks, sched_tgt,
+                                 sched_road, sched_flag, NULL, NULL, 0, 0, grad_pos, grad_theta, grad_alpha);
+}
+
+int orc_diff_adjoint64_ex(uint32_t n, uint32_t n_scen, const uint32_t* agent_off, const float* pos0,
+                          const float* speed, const double* theta, uint32_t n_roads, double alpha, double radius,
+                          uint32_t n_ticks, const float* sched_tgt, const uint32_t* sched_road,
+                          const uint8_t* sched_flag, const float* final_tgt, const uint32_t* final_road,
+                          int traj_carry, int adj_carry, const double* grad_pos, double* grad_theta,
+                          double* grad_alpha) {
     const double cell = (double)orc_cell_size((float)radius);
     const double inv_r2 = 1.0 / (radius * radius);
     double* traj = (double*)malloc(sizeof(double) * 4 * (size_t)n * (n_ticks ? n_ticks : 1));
@@ -289,7 +356,7 @@ int orc_diff_adjoint64(uint32_t n, uint32_t n_scen, const uint32_t* agent_off, c
     int rc = -1;
     if (!traj || !fin || !lam || !lt) goto done;
     if (replay64(n, n_scen, agent_off, pos0, speed, theta, alpha, radius, n_ticks, sched_tgt, sched_road,
-                 sched_flag, fin, traj) != 0) {
+                 sched_flag, final_tgt, final_road, traj_carry, fin, traj) != 0) {
         goto done;
     }
     memset(grad_theta, 0, sizeof(double) * n_roads);
@@ -330,8 +397,27 @@ int orc_diff_adjoint64(uint32_t n, uint32_t n_scen, const uint32_t* agent_off, c
         for (uint32_t i = 0; i < n; i++) {
             uint8_t f = sched_flag[(size_t)t * n + i];
             if (f & 1) {
+                /* frozen: the snap is constant; carry: the derivatives of
+                 * p' = T + (1 - L/s) s2 u2 (L = |T - p|, s, s2 the two roads' speeds) */
+                double T2x, T2y, L, ux, uy, u2x, u2y;
+                uint32_t r2;
+                double lx = lt[2 * i], ly = lt[2 * i + 1];
                 lam[2 * i] = 0.0;
                 lam[2 * i + 1] = 0.0;
+                if (adj_carry) {
+                    next_decision(n, n_ticks, t, i, sched_tgt, sched_road, final_tgt, final_road, &T2x, &T2y, &r2);
+                    double Tx = sched_tgt[((size_t)t * n + i) * 2], Ty = sched_tgt[((size_t)t * n + i) * 2 + 1];
+                    if (carry_geom(pre[2 * i], pre[2 * i + 1], Tx, Ty, T2x, T2y, &L, &ux, &uy, &u2x, &u2y)) {
+                        uint32_t r = sched_road[(size_t)t * n + i];
+                        double v = (double)(speed ? speed[i] : ORC_SPEED);
+                        double sp = v * theta[r], s2 = v * theta[r2];
+                        double g2 = u2x * lx + u2y * ly;
+                        lam[2 * i] = ux * (s2 / sp) * g2;
+                        lam[2 * i + 1] = uy * (s2 / sp) * g2;
+                        grad_theta[r] += v * (L * s2 / (sp * sp)) * g2;
+                        grad_theta[r2] += v * (1.0 - L / sp) * g2;
+                    }
+                }
                 continue;
             }
             if (f & 2) {

@@ -117,6 +117,23 @@ int orc_diff_adjoint64(uint32_t n, uint32_t n_scen, const uint32_t* agent_off, c
                        const float* speed, const double* theta, uint32_t n_roads, double alpha, double radius,
                        uint32_t n_ticks, const float* sched_tgt, const uint32_t* sched_road,
                        const uint8_t* sched_flag, const double* grad_pos, double* grad_theta, double* grad_alpha);
+/* Arrival carry-over (the surrogate the adjoint can use through arrivals):
+ * orc_diff_replay64_carry replays with every arrival carrying its leftover
+ * time onto the next road, p' = T + (1 - L/s) s2 u2; final_tgt/final_road are
+ * the decisions after the last tick.  orc_diff_adjoint64_ex runs the reverse
+ * pass on the snapped (traj_carry = 0) or carried (1) trajectory, with
+ * arrivals cutting the gradient (adj_carry = 0) or differentiated as the
+ * carry-over (1). */
+int orc_diff_replay64_carry(uint32_t n, uint32_t n_scen, const uint32_t* agent_off, const float* pos0,
+                            const float* speed, const double* theta, double alpha, double radius, uint32_t n_ticks,
+                            const float* sched_tgt, const uint32_t* sched_road, const uint8_t* sched_flag,
+                            const float* final_tgt, const uint32_t* final_road, double* pos_out);
+int orc_diff_adjoint64_ex(uint32_t n, uint32_t n_scen, const uint32_t* agent_off, const float* pos0,
+                          const float* speed, const double* theta, uint32_t n_roads, double alpha, double radius,
+                          uint32_t n_ticks, const float* sched_tgt, const uint32_t* sched_road,
+                          const uint8_t* sched_flag, const float* final_tgt, const uint32_t* final_road,
+                          int traj_carry, int adj_carry, const double* grad_pos, double* grad_theta,
+                          double* grad_alpha);
 
 #ifdef __cplusplus
 }

@@ -49,6 +49,11 @@ def _load():
                                              _u32, _vp, _vp, _vp, _vp]),
         "orc_diff_adjoint64": (ctypes.c_int, [_u32, _u32, _vp, _vp, _vp, _vp, _u32, ctypes.c_double,
                                               ctypes.c_double, _u32, _vp, _vp, _vp, _vp, _vp, _vp]),
+        "orc_diff_replay64_carry": (ctypes.c_int, [_u32, _u32, _vp, _vp, _vp, _vp, ctypes.c_double, ctypes.c_double,
+                                                   _u32, _vp, _vp, _vp, _vp, _vp, _vp]),
+        "orc_diff_adjoint64_ex": (ctypes.c_int, [_u32, _u32, _vp, _vp, _vp, _vp, _u32, ctypes.c_double,
+                                                 ctypes.c_double, _u32, _vp, _vp, _vp, _vp, _vp, ctypes.c_int,
+                                                 ctypes.c_int, _vp, _vp, _vp]),
     }
     for k, (r, a) in sig.items():
         f = getattr(lib, k)
@@ -226,13 +231,16 @@ def advance_f64(pos, tgt, speed=None):
 
 class DiffSchedule:
     """Decisions of a recorded differentiable run: per tick the target and
-    road each move used and a flag (1 arrived, 2 no move), caller order."""
+    road each move used and a flag (1 arrived, 2 no move), caller order;
+    final_tgt / final_road: the decisions in force after the last tick."""
 
-    def __init__(self, tgt, road, flag):
+    def __init__(self, tgt, road, flag, final_tgt=None, final_road=None):
         self.tgt = np.ascontiguousarray(tgt, np.float32)    # [T, n, 2]
         self.road = np.ascontiguousarray(road, np.uint32)   # [T, n]
         self.flag = np.ascontiguousarray(flag, np.uint8)    # [T, n]
         self.ticks = self.flag.shape[0]
+        self.final_tgt = None if final_tgt is None else np.ascontiguousarray(final_tgt, np.float32)  # [n, 2]
+        self.final_road = None if final_road is None else np.ascontiguousarray(final_road, np.uint32)  # [n]
 
 
 def diff_record(roads, conns, entities, speed, agent_off, theta, alpha, radius, ticks):
@@ -253,7 +261,7 @@ def diff_record(roads, conns, entities, speed, agent_off, theta, alpha, radius, 
                            no_move, _p(tg), _p(rd), _p(fl))
     if rc != 0:
         raise RuntimeError(f"orc_diff_record failed: {rc}")
-    return st.pos, DiffSchedule(tg, rd, fl)
+    return st.pos, DiffSchedule(tg, rd, fl, st.tgt.copy(), st.road.copy())
 
 
 def diff_replay64(pos0, speed, agent_off, theta, alpha, radius, sched):
@@ -270,8 +278,11 @@ def diff_replay64(pos0, speed, agent_off, theta, alpha, radius, sched):
     return out
 
 
-def diff_adjoint64(pos0, speed, agent_off, theta, alpha, radius, sched, grad_pos):
-    """(dL/dtheta, dL/dalpha) in binary64 for dL/d(final pos) = grad_pos."""
+def diff_adjoint64(pos0, speed, agent_off, theta, alpha, radius, sched, grad_pos, carry=False, carry_traj=False):
+    """(dL/dtheta, dL/dalpha) in binary64 for dL/d(final pos) = grad_pos.
+    carry: arrivals differentiated as carrying their leftover time onto the
+    next road (the surrogate); carry_traj: on the carried trajectory
+    (diff_replay64_carry's) instead of the snapped one."""
     p0 = np.ascontiguousarray(pos0, np.float32)
     n = p0.shape[0]
     off = np.ascontiguousarray(agent_off, np.uint32)
@@ -280,9 +291,29 @@ def diff_adjoint64(pos0, speed, agent_off, theta, alpha, radius, sched, grad_pos
     g = np.ascontiguousarray(grad_pos, np.float64)
     gt = np.zeros(th.shape[0], np.float64)
     ga = np.zeros(1, np.float64)
-    rc = L.orc_diff_adjoint64(n, off.shape[0] - 1, _p(off), _p(p0), _p(sp), _p(th), th.shape[0], float(alpha),
-                              float(radius), sched.ticks, _p(sched.tgt), _p(sched.road), _p(sched.flag), _p(g),
-                              _p(gt), _p(ga))
+    if (carry or carry_traj) and sched.final_tgt is None:
+        raise ValueError("the carry-over needs the schedule's final decisions")
+    rc = L.orc_diff_adjoint64_ex(n, off.shape[0] - 1, _p(off), _p(p0), _p(sp), _p(th), th.shape[0], float(alpha),
+                                 float(radius), sched.ticks, _p(sched.tgt), _p(sched.road), _p(sched.flag),
+                                 _p(sched.final_tgt), _p(sched.final_road), int(carry_traj), int(carry), _p(g),
+                                 _p(gt), _p(ga))
     if rc != 0:
         raise RuntimeError("orc_diff_adjoint64 failed")
     return gt, float(ga[0])
+
+
+def diff_replay64_carry(pos0, speed, agent_off, theta, alpha, radius, sched):
+    """binary64 replay in which every arrival carries its leftover time onto
+    the next road (the model the carry-over surrogate differentiates)."""
+    p0 = np.ascontiguousarray(pos0, np.float32)
+    n = p0.shape[0]
+    off = np.ascontiguousarray(agent_off, np.uint32)
+    th = np.ascontiguousarray(theta, np.float64)
+    sp = None if speed is None else np.ascontiguousarray(speed, np.float32)
+    out = np.zeros((n, 2), np.float64)
+    rc = L.orc_diff_replay64_carry(n, off.shape[0] - 1, _p(off), _p(p0), _p(sp), _p(th), float(alpha),
+                                   float(radius), sched.ticks, _p(sched.tgt), _p(sched.road), _p(sched.flag),
+                                   _p(sched.final_tgt), _p(sched.final_road), _p(out))
+    if rc != 0:
+        raise RuntimeError("orc_diff_replay64_carry failed")
+    return out

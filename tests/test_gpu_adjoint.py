@@ -15,7 +15,10 @@
   100 ticks, eps = 1e-3 differences are off by up to 1e-2 (they converge to
   the adjoint only as eps -> 0, see test_oracle_adjoint.py), and dL/dtheta is
   compared with the host adjoint alone;
-* the checkpoint interval does not change the gradient.
+* the checkpoint interval does not change the gradient;
+* the carry-over surrogate through arrivals agrees with the host's binary64
+  surrogate adjoint (the carry-over itself is pinned on the CPU against
+  differences of the carried run, test_oracle_adjoint.py).
 """
 import numpy as np
 import pytest
@@ -48,7 +51,9 @@ def schedule(batch, ticks):
     fl = np.zeros((ticks, batch.n), np.uint8)
     for t in range(ticks):
         tg[t], rd[t], fl[t] = batch.diff_schedule(t)
-    return orc.DiffSchedule(tg, rd, fl)
+    ft, fr, ff = batch.diff_schedule(ticks)  # the decisions after the run
+    assert not ff.any()
+    return orc.DiffSchedule(tg, rd, fl, ft, fr)
 
 
 def test_neutral_parameters_are_the_plain_tick():
@@ -101,11 +106,11 @@ def test_avoidance_run_close_to_oracle_and_replay_exact():
     np.testing.assert_array_equal(rep.view(np.uint32), pos.view(np.uint32))
 
 
-def gradients(ck, alpha=0.05, T=100, seed=5):
+def gradients(ck, alpha=0.05, T=100, seed=5, arrival_gradient="frozen"):
     scs, (roads, conns, ents, speed, _, _, off) = problem(seed=seed)
     theta = theta_for(roads.shape[0], seed=6)
     b = msim.Batch(scs)
-    run = DiffRun(b, T, checkpoint_every=ck, keep_schedule=True)
+    run = DiffRun(b, T, checkpoint_every=ck, keep_schedule=True, arrival_gradient=arrival_gradient)
     th = torch.from_numpy(theta).to(DEV).requires_grad_(True)
     al = torch.tensor(alpha, device=DEV, requires_grad=True)
     pos = run(th, al)
@@ -165,6 +170,27 @@ def test_dense_theta_gradient_matches_fp64_host_adjoint():
     rel = np.linalg.norm(r["gt"] - gt64) / np.linalg.norm(gt64)
     print(f"dense: dL/dtheta vs binary64 host adjoint: relative {rel:.2e} ({r['b'].n} agents)")
     assert rel < 1e-3
+
+
+def test_carry_over_gradient_matches_fp64_host():
+    """The arrival surrogate (each arrival differentiated as carrying its
+    leftover time onto the next road) against the binary64 host adjoint with
+    the same surrogate on the same snapped run; it also reaches the roads
+    agents left before the end, which the frozen gradient cannot."""
+    r = gradients(ck=30, seed=5, arrival_gradient="carry")
+    sched = schedule(r["b"], r["T"])
+    assert (sched.flag & 1).sum() > 20
+    gpos = 2.0 * (r["pos"].cpu().numpy().astype(np.float64) - r["target"].cpu().numpy())
+    p0, th64 = r["ents"]["pos"], r["theta"].astype(np.float64)
+    gt64, ga64 = orc.diff_adjoint64(p0, r["speed"], r["off"], th64, r["alpha"], 1.0, sched, gpos, carry=True)
+    rel = np.linalg.norm(r["gt"] - gt64) / np.linalg.norm(gt64)
+    print(f"carry-over: dL/dtheta vs binary64 host: relative {rel:.2e}; dL/dalpha {r['ga']:.6g} vs {ga64:.6g}")
+    assert rel < 1e-3
+    assert abs(r["ga"] - ga64) < 1e-3 * abs(ga64)
+    frozen, _ = orc.diff_adjoint64(p0, r["speed"], r["off"], th64, r["alpha"], 1.0, sched, gpos)
+    assert np.linalg.norm(gt64 - frozen) > 1e-3 * np.linalg.norm(frozen)
+    last = np.unique(np.concatenate([sched.road[-1], sched.final_road]))
+    assert np.count_nonzero(np.delete(r["gt"], last)) > np.count_nonzero(np.delete(frozen, last))
 
 
 def test_checkpoint_interval_does_not_change_the_gradient():

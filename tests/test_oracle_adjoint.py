@@ -73,3 +73,62 @@ def test_alpha_zero_gradient_is_the_pushes_first_order_effect():
     eps = 1e-4
     fda = (loss(th, eps) - loss(th, -eps)) / (2 * eps)
     assert abs(ga - fda) <= 1e-4 * max(1.0, abs(fda))
+
+
+def _carry_problem(alpha=0.05, ticks=80, seed=2):
+    scs, (roads, conns, ents, speed, _, _, off) = packed()
+    rng = np.random.default_rng(seed)
+    theta = np.exp(rng.normal(0.0, 0.2, roads.shape[0])).astype(np.float32)
+    fin, sched = orc.diff_record(roads, conns, ents, speed, off, theta, alpha, 1.0, ticks)
+    p0 = ents["pos"].copy()
+    th = theta.astype(np.float64)
+    p = orc.diff_replay64_carry(p0, speed, off, th, alpha, 1.0, sched)
+    target = p + rng.normal(0.0, 1.0, p.shape)
+
+    def loss(t, a):
+        q = orc.diff_replay64_carry(p0, speed, off, t, a, 1.0, sched)
+        return float(((q - target) ** 2).sum())
+
+    gt, ga = orc.diff_adjoint64(p0, speed, off, th, alpha, 1.0, sched, 2.0 * (p - target), carry=True,
+                                carry_traj=True)
+    return sched, th, alpha, loss, gt, ga, rng
+
+
+def test_carry_over_adjoint_matches_differences_of_the_carried_replay():
+    """The arrival surrogate differentiates the run in which arrivals carry
+    their leftover time onto the next road: on that run's own trajectory it
+    is its exact gradient.  That run follows the snapped run's decisions, so
+    a carried agent can pass its next target before the decisions say it
+    arrives, and the moves then turn sharply: the differences need a small
+    step (eps 1e-6; at 1e-4 some roads are off by 2x, at 1e-6 they agree to
+    1e-8 relative)."""
+    sched, th, alpha, loss, gt, ga, rng = _carry_problem()
+    assert (sched.flag & 1).sum() > 50  # arrivals happen
+    eps = 1e-6
+    for r in np.argsort(-np.abs(gt))[:8]:
+        e = np.zeros_like(th)
+        e[r] = eps
+        fd = (loss(th + e, alpha) - loss(th - e, alpha)) / (2 * eps)
+        assert abs(gt[r] - fd) <= 1e-5 * abs(fd), (r, gt[r], fd)
+    v = rng.normal(0.0, 1.0, th.shape)
+    fd = (loss(th + eps * v, alpha) - loss(th - eps * v, alpha)) / (2 * eps)
+    assert abs(gt @ v - fd) <= 1e-4 * abs(fd)
+    fda = (loss(th, alpha + eps) - loss(th, alpha - eps)) / (2 * eps)
+    assert abs(ga - fda) <= 1e-4 * abs(fda)
+    # the roads agents left before the end receive gradient too
+    last = np.unique(np.concatenate([sched.road[-1], sched.final_road]))
+    assert np.count_nonzero(np.delete(gt, last)) > 0
+
+
+def test_carry_over_without_arrivals_is_the_frozen_adjoint():
+    scs, (roads, conns, ents, speed, _, _, off) = packed()
+    theta = np.ones(roads.shape[0], np.float32)
+    _, rec = orc.diff_record(roads, conns, ents, speed, off, theta, 0.05, 1.0, 6)
+    # the same decisions with every arrival taken out: nothing to carry
+    sched = orc.DiffSchedule(rec.tgt, rec.road, rec.flag & 2, rec.final_tgt, rec.final_road)
+    p0 = ents["pos"].copy()
+    g = np.random.default_rng(4).normal(0.0, 1.0, (p0.shape[0], 2))
+    a = orc.diff_adjoint64(p0, speed, off, theta.astype(np.float64), 0.05, 1.0, sched, g)
+    b = orc.diff_adjoint64(p0, speed, off, theta.astype(np.float64), 0.05, 1.0, sched, g, carry=True)
+    np.testing.assert_array_equal(a[0], b[0])
+    assert a[1] == b[1]
