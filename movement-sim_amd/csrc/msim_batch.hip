@@ -65,7 +65,7 @@ int benqueue_move(msim_batch* b, bool write_dir) {
 
 int breorder(msim_batch* b) {
     const int nxt = 1 - b->cur;
-    launch_reorder(b->agents(b->cur), b->agents(nxt), b->bbid[b->bcur], b->n, b->stream);
+    launch_reorder(b->agents(b->cur), b->agents(nxt), b->bbid[b->bcur], b->n, b->stream, !b->capturing);
     int rc = bcheck(b, "k_reorder");
     if (rc != MSIM_OK) return rc;
     b->cur = nxt;
@@ -91,9 +91,11 @@ int bbuild_graph(msim_batch* b) {
     hipError_t e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return bfail(b, MSIM_ERR_HIP, "hipStreamBeginCapture: %s", hipGetErrorString(e));
     int rc = MSIM_OK;
+    b->capturing = true;  // run segments are always followed by the run's last tick
     for (uint32_t t = 0; t < b->seg_ticks && rc == MSIM_OK; t++) {
         rc = btick(b, false);
     }
+    b->capturing = false;
     e = hipStreamEndCapture(b->stream, &graph);
     msim_batch::Segment& g = b->gexec[cur0][bcur0];
     g.cur = b->cur;

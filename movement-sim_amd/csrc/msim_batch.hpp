@@ -319,6 +319,7 @@ struct msim_batch {
     unsigned int* faults = nullptr;
     uint32_t* qdummy = nullptr;
     bool first_tick_insert_only = false;
+    bool capturing = false;  // inside bbuild_graph: renumberings skip the output-only arrays
     int collide_mode = 0;  // 0: staged, 2 slots per lane; 1: per lane; 2: staged, 1 slot per lane
     int binning = 0;  // 0: k_bbin_count + k_bbin_place; 2: k_bbin_fused
     uint32_t bin_unroll = 4;  // window slots per rebinning thread and load round

@@ -832,6 +832,26 @@ static __global__ __launch_bounds__(kBlock) void k_reorder(AgentsDev src, Agents
     }
 }
 
+// k_reorder without the output-only arrays (dir, ncount): rows 0 pos,
+// 1 tgt, 2 road, 3 rng, 4 init, 5 perm, 6 spd when present.
+static __global__ __launch_bounds__(kBlock) void k_reorder_core(AgentsDev src, AgentsDev dst,
+                                                         const uint32_t* __restrict__ bid) {
+    const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    if (s >= src.n) {
+        return;
+    }
+    const uint32_t j = bid[s];
+    switch (blockIdx.y) {
+        case 0: dst.pos[s] = src.pos[j]; break;
+        case 1: dst.tgt[s] = src.tgt[j]; break;
+        case 2: dst.road[s] = src.road[j]; break;
+        case 3: dst.rng[s] = src.rng[j]; break;
+        case 4: dst.init[s] = src.init[j]; break;
+        case 5: dst.perm[s] = src.perm[j]; break;
+        default: dst.spd[s] = src.spd[j]; break;
+    }
+}
+
 // Agent index = slot after a renumbering.
 static __global__ __launch_bounds__(kBlock) void k_bid_identity(uint32_t* __restrict__ bid, uint32_t n) {
     const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
@@ -841,12 +861,20 @@ static __global__ __launch_bounds__(kBlock) void k_bid_identity(uint32_t* __rest
 }
 
 // Launches the renumbering of n agents from src into dst (bid: the binning's
-// slot -> agent map, reset to the identity).
+// slot -> agent map, reset to the identity).  live_outputs = false skips
+// dir and ncount: a renumbering inside a captured run segment is always
+// followed by the run's last tick, whose move rewrites every heading and
+// whose collision dispatch every neighbour count.
 inline void launch_reorder(const AgentsDev& src, const AgentsDev& dst, uint32_t* bid, uint32_t n,
-                           hipStream_t stream) {
+                           hipStream_t stream, bool live_outputs = true) {
     const uint32_t nb = std::max(1u, (n + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL(k_reorder, dim3(nb, src.spd ? 9u : 8u), dim3(kBlock), 0, stream, src, dst,
-                       static_cast<const uint32_t*>(bid));
+    if (live_outputs) {
+        hipLaunchKernelGGL(k_reorder, dim3(nb, src.spd ? 9u : 8u), dim3(kBlock), 0, stream, src, dst,
+                           static_cast<const uint32_t*>(bid));
+    } else {
+        hipLaunchKernelGGL(k_reorder_core, dim3(nb, src.spd ? 7u : 6u), dim3(kBlock), 0, stream, src, dst,
+                           static_cast<const uint32_t*>(bid));
+    }
     hipLaunchKernelGGL(k_bid_identity, dim3(nb), dim3(kBlock), 0, stream, bid, n);
 }
 

@@ -64,6 +64,7 @@ struct msim_sim {
     int binning = 0;  // 0: segment-owned (k_bin_count + k_bin_place), 1: atomic counting,
                       // 2: segment-owned in one pass (k_bin_fused, look-back)
     bool route_inline = false;  // arrivals routed inside k_advance instead of k_route
+    bool capturing = false;  // inside build_graph: renumberings skip the output-only arrays
     int collide_mode = 0;  // 0: staged, 2 slots per lane; 1: per lane; 2: staged, 1 slot per lane
     uint32_t bin_unroll = 4;  // window slots per rebinning thread and load round
     // msim_time_ticks probe: launch number probe_kernel of the tick is
@@ -466,7 +467,7 @@ void drop_graphs(msim_sim* s) {
 
 int reorder(msim_sim* s) {
     const int nxt = 1 - s->cur;
-    launch_reorder(s->agents(s->cur), s->agents(nxt), s->grid_cur().bid, s->n, s->stream);
+    launch_reorder(s->agents(s->cur), s->agents(nxt), s->grid_cur().bid, s->n, s->stream, !s->capturing);
     int rc = check_launch(s, "k_reorder");
     if (rc != MSIM_OK) {
         return rc;
@@ -508,11 +509,13 @@ int build_graph(msim_sim* s) {
     const uint64_t ticks0 = s->ticks_simulated;
     MSIM_TRY_HIP(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
     int rc = MSIM_OK;
+    s->capturing = true;  // run segments are always followed by the run's last tick
     for (uint32_t t = 0; t < s->seg_ticks && rc == MSIM_OK; t++) {
         rc = enqueue_move(s, kMoveAll, false);
         if (rc == MSIM_OK) rc = enqueue_collide(s, false);
         if (rc == MSIM_OK) rc = after_move(s);
     }
+    s->capturing = false;
     hipError_t e = hipStreamEndCapture(s->stream, &graph);
     msim_sim::Segment& g = s->gexec[cur0][bcur0][qpar0];
     g.cur = s->cur;
