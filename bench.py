@@ -77,7 +77,7 @@ def plain_line(args, sc):
 
     n = sc.entities.shape[0]
     opts = dict(autotune.DEFAULT) if args.untuned else autotune.tuned_options(sc)
-    for k in ("binning", "collide_mode", "bin_unroll"):
+    for k in ("binning", "collide_mode", "bin_unroll", "reorder_interval"):
         if getattr(args, k) is not None:
             opts[k] = getattr(args, k)
     sim = msim.Simulation(sc.roads, sc.conns, sc.entities, sc.pc, speed=sc.speed, **opts)
@@ -289,6 +289,8 @@ def main():
                                                                "1 atomic (plain only), 2 one pass)")
     ap.add_argument("--collide-mode", type=int, default=None, help="override the tuned neighbour walk (0 staged, "
                                                                     "2 slots per lane; 1 per lane; 2 staged, 1 slot)")
+    ap.add_argument("--reorder-interval", type=int, default=None, help="plain workloads: override the tuned "
+                    "renumbering interval (ticks)")
     ap.add_argument("--bin-unroll", type=int, default=None, help="override the tuned rebinning load batching (4, 8)")
     ap.add_argument("--arrival-gradient", choices=("frozen", "carry"), default="carry",
                     help="calibration (--config 5): how the adjoint treats arrivals")

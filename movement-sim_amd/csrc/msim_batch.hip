@@ -65,7 +65,8 @@ int benqueue_move(msim_batch* b, bool write_dir) {
 
 int breorder(msim_batch* b) {
     const int nxt = 1 - b->cur;
-    launch_reorder(b->agents(b->cur), b->agents(nxt), b->bbid[b->bcur], b->n, b->stream, !b->capturing);
+    launch_reorder(b->agents(b->cur), b->agents(nxt), b->bbid[b->bcur], b->bbpos[b->bcur], b->n, b->stream,
+                   !b->capturing);
     int rc = bcheck(b, "k_reorder");
     if (rc != MSIM_OK) return rc;
     b->cur = nxt;

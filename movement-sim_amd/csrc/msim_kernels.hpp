@@ -806,49 +806,65 @@ static __global__ __launch_bounds__(64) void k_collide_finish(unsigned long long
 // ---- spatial renumbering -------------------------------------------------
 
 // Gathers the per-agent arrays into bucket order so that agents that are
-// close in space are close in memory: workgroup row blockIdx.y gathers one
-// array (0 pos, 1 tgt, 2 dir, 3 road, 4 rng, 5 init, 6 ncount, 7 perm,
-// 8 spd when present).  Rows are dispatched one after the other, so the
-// resident workgroups mostly gather from the same array and share its cache
-// lines (a gather of all arrays per slot spread its reads over every
-// array's lines at once).  k_bid_identity then resets bid.
-static __global__ __launch_bounds__(kBlock) void k_reorder(AgentsDev src, AgentsDev dst,
-                                                    const uint32_t* __restrict__ bid) {
-    const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-    if (s >= src.n) {
-        return;
+// close in space are close in memory.  Workgroup row blockIdx.y handles one
+// array, named by the row's 4-bit code in `codes` (kRow*).  Rows are
+// dispatched one after the other, so the resident workgroups mostly gather
+// from the same array and share its cache lines.  Each thread handles
+// kReorderU slots kBlock apart and issues all of their slot-map loads, then
+// all of their gathers, before the first store: a thread with one slot spends
+// two dependent memory round trips on 4 to 16 bytes.  Positions need no
+// gather: the binning's bpos[slot] is pos[bid[slot]] already.
+// k_bid_identity then resets bid.
+constexpr uint32_t kReorderU = 8;
+constexpr uint32_t kRowPos = 0, kRowTgt = 1, kRowDir = 2, kRowRoad = 3, kRowRng = 4, kRowInit = 5,
+                   kRowNcount = 6, kRowPerm = 7, kRowSpd = 8;
+
+template <class E>
+__device__ __forceinline__ void reorder_gather(const E* __restrict__ src, E* __restrict__ dst,
+                                               const uint32_t* __restrict__ bid, uint32_t s0, uint32_t n) {
+    uint32_t j[kReorderU];
+    E v[kReorderU];
+#pragma unroll
+    for (uint32_t u = 0; u < kReorderU; u++) {
+        j[u] = bid[min(s0 + u * kBlock, n - 1u)];  // s0 < n
     }
-    const uint32_t j = bid[s];
-    switch (blockIdx.y) {
-        case 0: dst.pos[s] = src.pos[j]; break;
-        case 1: dst.tgt[s] = src.tgt[j]; break;
-        case 2: dst.dir[s] = src.dir[j]; break;
-        case 3: dst.road[s] = src.road[j]; break;
-        case 4: dst.rng[s] = src.rng[j]; break;
-        case 5: dst.init[s] = src.init[j]; break;
-        case 6: dst.ncount[s] = src.ncount[j]; break;
-        case 7: dst.perm[s] = src.perm[j]; break;
-        default: dst.spd[s] = src.spd[j]; break;
+#pragma unroll
+    for (uint32_t u = 0; u < kReorderU; u++) {
+        v[u] = src[j[u]];
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < kReorderU; u++) {
+        if (s0 + u * kBlock < n) {
+            dst[s0 + u * kBlock] = v[u];
+        }
     }
 }
 
-// k_reorder without the output-only arrays (dir, ncount): rows 0 pos,
-// 1 tgt, 2 road, 3 rng, 4 init, 5 perm, 6 spd when present.
-static __global__ __launch_bounds__(kBlock) void k_reorder_core(AgentsDev src, AgentsDev dst,
-                                                         const uint32_t* __restrict__ bid) {
-    const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-    if (s >= src.n) {
+static __global__ __launch_bounds__(kBlock) void k_reorder(AgentsDev src, AgentsDev dst,
+                                                    const uint32_t* __restrict__ bid,
+                                                    const float2* __restrict__ bpos, unsigned long long codes) {
+    const uint32_t s0 = xcd_block(blockIdx.x, gridDim.x) * (kBlock * kReorderU) + threadIdx.x;
+    const uint32_t n = src.n;
+    if (s0 >= n) {
         return;
     }
-    const uint32_t j = bid[s];
-    switch (blockIdx.y) {
-        case 0: dst.pos[s] = src.pos[j]; break;
-        case 1: dst.tgt[s] = src.tgt[j]; break;
-        case 2: dst.road[s] = src.road[j]; break;
-        case 3: dst.rng[s] = src.rng[j]; break;
-        case 4: dst.init[s] = src.init[j]; break;
-        case 5: dst.perm[s] = src.perm[j]; break;
-        default: dst.spd[s] = src.spd[j]; break;
+    switch (static_cast<uint32_t>(codes >> (4u * blockIdx.y)) & 15u) {
+        case kRowPos:
+#pragma unroll
+            for (uint32_t u = 0; u < kReorderU; u++) {
+                if (s0 + u * kBlock < n) {
+                    dst.pos[s0 + u * kBlock] = bpos[s0 + u * kBlock];
+                }
+            }
+            break;
+        case kRowTgt: reorder_gather(src.tgt, dst.tgt, bid, s0, n); break;
+        case kRowDir: reorder_gather(src.dir, dst.dir, bid, s0, n); break;
+        case kRowRoad: reorder_gather(src.road, dst.road, bid, s0, n); break;
+        case kRowRng: reorder_gather(src.rng, dst.rng, bid, s0, n); break;
+        case kRowInit: reorder_gather(src.init, dst.init, bid, s0, n); break;
+        case kRowNcount: reorder_gather(src.ncount, dst.ncount, bid, s0, n); break;
+        case kRowPerm: reorder_gather(src.perm, dst.perm, bid, s0, n); break;
+        default: reorder_gather(src.spd, dst.spd, bid, s0, n); break;
     }
 }
 
@@ -860,22 +876,33 @@ static __global__ __launch_bounds__(kBlock) void k_bid_identity(uint32_t* __rest
     }
 }
 
-// Launches the renumbering of n agents from src into dst (bid: the binning's
-// slot -> agent map, reset to the identity).  live_outputs = false skips
-// dir and ncount: a renumbering inside a captured run segment is always
-// followed by the run's last tick, whose move rewrites every heading and
-// whose collision dispatch every neighbour count.
-inline void launch_reorder(const AgentsDev& src, const AgentsDev& dst, uint32_t* bid, uint32_t n,
-                           hipStream_t stream, bool live_outputs = true) {
-    const uint32_t nb = std::max(1u, (n + kBlock - 1) / kBlock);
+// Launches the renumbering of n agents from src into dst (bid, bpos: the
+// current binning's slot -> agent map, reset to the identity, and its
+// positions).  live_outputs = false skips dir and ncount: a renumbering
+// inside a captured run segment is always followed by the run's last tick,
+// whose move rewrites every heading and whose collision dispatch every
+// neighbour count.
+inline void launch_reorder(const AgentsDev& src, const AgentsDev& dst, uint32_t* bid, const float2* bpos,
+                           uint32_t n, hipStream_t stream, bool live_outputs = true) {
+    const uint32_t nb = std::max(1u, (n + kBlock * kReorderU - 1) / (kBlock * kReorderU));
+    unsigned long long codes = 0;
+    uint32_t rows = 0;
+    auto row = [&](uint32_t code) { codes |= static_cast<unsigned long long>(code) << (4u * rows++); };
+    row(kRowPos);
+    row(kRowTgt);
+    row(kRowRoad);
+    row(kRowRng);
+    row(kRowInit);
+    row(kRowPerm);
+    if (src.spd) row(kRowSpd);
     if (live_outputs) {
-        hipLaunchKernelGGL(k_reorder, dim3(nb, src.spd ? 9u : 8u), dim3(kBlock), 0, stream, src, dst,
-                           static_cast<const uint32_t*>(bid));
-    } else {
-        hipLaunchKernelGGL(k_reorder_core, dim3(nb, src.spd ? 7u : 6u), dim3(kBlock), 0, stream, src, dst,
-                           static_cast<const uint32_t*>(bid));
+        row(kRowDir);
+        row(kRowNcount);
     }
-    hipLaunchKernelGGL(k_bid_identity, dim3(nb), dim3(kBlock), 0, stream, bid, n);
+    hipLaunchKernelGGL(k_reorder, dim3(nb, rows), dim3(kBlock), 0, stream, src, dst,
+                       static_cast<const uint32_t*>(bid), bpos, codes);
+    hipLaunchKernelGGL(k_bid_identity, dim3(std::max(1u, (n + kBlock - 1) / kBlock)), dim3(kBlock), 0, stream, bid,
+                       n);
 }
 
 // ---- Entity records (64 B, the layout of sim::Entity, Entity.hpp:33-46) ----

@@ -467,7 +467,8 @@ void drop_graphs(msim_sim* s) {
 
 int reorder(msim_sim* s) {
     const int nxt = 1 - s->cur;
-    launch_reorder(s->agents(s->cur), s->agents(nxt), s->grid_cur().bid, s->n, s->stream, !s->capturing);
+    launch_reorder(s->agents(s->cur), s->agents(nxt), s->grid_cur().bid, s->grid_cur().bpos, s->n, s->stream,
+                   !s->capturing);
     int rc = check_launch(s, "k_reorder");
     if (rc != MSIM_OK) {
         return rc;
