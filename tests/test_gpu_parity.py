@@ -112,6 +112,44 @@ def test_options_do_not_change_results(opts):
     assert g.pairs()[0] == o.pairs_total
 
 
+def test_route_choice_off_node_targets_and_shared_groups():
+    """Route choice away from the usual geometry (random_move.comp:778-819):
+    agents whose target is off their road's endpoints (the endpoint test
+    falls to the road's end, the new target to the next road's start), and
+    a corner endpoint sharing the centre's connection list (one entry drawn
+    from two positions).  Dispatch loop, fused ticks and a lattice map all
+    match the oracle."""
+    roads, conns, w, h = S.star_map()
+    roads = roads.copy()
+    roads["start_index"][0] = roads["end_index"][0]  # corner (5, 5) draws the centre's list
+    roads["start_count"][0] = roads["end_count"][0]
+    ents = S.reference_agents(roads, 96, 11)
+    rng = np.random.default_rng(4)
+    k = ents.shape[0] // 3
+    ents["target"][:k] += rng.uniform(-3.0, 3.0, (k, 2)).astype(np.float32)  # off the node
+    sc = S.Scenario("star-shared", roads, conns, ents, None, msim.push_consts(w, h, 10.0), 0, (w, h))
+    g, o = pair(sc)
+    for t in range(200):
+        g.dispatch()
+        o.dispatch()
+        if t % 20 == 0:
+            assert_same(g.state(), o.st, f"dispatch {t}")
+    assert_same(g.state(), o.st, "dispatch loop")
+    g2, o2 = pair(sc, graph_ticks=16)
+    g2.run_ticks(150)
+    o2.run_ticks(150)
+    assert_same(g2.state(), o2.st, "fused ticks")
+    big = S.config(1, scale=0.2)
+    e2 = big.entities.copy()
+    k = e2.shape[0] // 3
+    e2["target"][:k] += np.random.default_rng(5).uniform(-2.0, 2.0, (k, 2)).astype(np.float32)
+    sc3 = S.Scenario("lattice-off-node", big.roads, big.conns, e2, big.speed, big.pc, 0, big.world)
+    g3, o3 = pair(sc3)
+    g3.run_ticks(120)
+    o3.run_ticks(120)
+    assert_same(g3.state(), o3.st, "lattice, off-node targets")
+
+
 def test_mixed_initialized_flags_and_collision_first():
     """Agents with Entity.initialized == 0 only enter the index on their first
     dispatch and do not search in a collision dispatch
