@@ -48,13 +48,24 @@ def stats_table(d):
     return "\n".join(out)
 
 
+# Kernels whose first dispatch belongs to the handle's creation (the
+# renumbering out of the caller's agent order) and is reported on its own.
+CREATION_FIRST = ("k_reorder",)
+
+
 def counter_means(d, counter):
-    """{kernel base name: (mean KiB per dispatch, dispatches)} for one counter."""
+    """{kernel base name: (mean KiB per dispatch, dispatches)} for one counter;
+    the first dispatch of a CREATION_FIRST kernel is keyed '<name> (creation)'."""
     acc = {}
-    for r in csv.DictReader(open(_one(os.path.join(d, "*counter_collection.csv")))):
-        if r["Counter_Name"] != counter:
-            continue
+    rows = [r for r in csv.DictReader(open(_one(os.path.join(d, "*counter_collection.csv"))))
+            if r["Counter_Name"] == counter]
+    rows.sort(key=lambda r: int(r["Dispatch_Id"]))
+    seen = set()
+    for r in rows:
         k = base_name(r["Kernel_Name"])
+        if k in CREATION_FIRST and k not in seen:
+            seen.add(k)
+            k += " (creation)"
         s, c = acc.get(k, (0.0, 0))
         acc[k] = (s + float(r["Counter_Value"]), c + 1)
     return {k: (s / c, c) for k, (s, c) in acc.items()}
@@ -80,7 +91,7 @@ def traffic(a):
         fb = f.get(k, (0.0, 0))[0] * 1024.0 * fs
         wb = w.get(k, (0.0, 0))[0] * 1024.0 * ws
         kernels[k] = {"fetch_bytes": fb, "write_bytes": wb, "bytes_per_launch": fb + wb,
-                      "algorithmic_bytes": RL.kernel_bytes(k, a.uniform) * a.agents,
+                      "algorithmic_bytes": RL.kernel_bytes(k.split(" ")[0], a.uniform) * a.agents,
                       "dispatches": max(f.get(k, (0, 0))[1], w.get(k, (0, 0))[1])}
     out = {"workload": a.workload, "fetch_scale": fs, "write_scale": ws, "kernels": kernels}
     with open(a.json, "w") as fh:
