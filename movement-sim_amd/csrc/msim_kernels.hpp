@@ -532,8 +532,28 @@ __device__ __forceinline__ void wave_count_in_cells(const float2 (&me)[P], const
             hi[p][r] = start[rowb + ch + 1u];
         }
     }
-    // wave-uniform row blocks: first slot a (even), staged length n (even)
-    uint32_t a[3], n[3];
+    // The column across the table's edge (slots in its first or last column,
+    // when the table has more than one): bounds loaded with the others.
+    uint32_t elo[P][3], ehi[P][3];
+    bool ecol[P];
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+        ecol[p] = valid[p] && (col[p] == 0u || col[p] == wmask) && wmask != 0u;
+        const uint32_t wc = col[p] == 0u ? wmask : 0u;
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            elo[p][r] = 0u;
+            ehi[p][r] = 0u;
+            if (ecol[p]) {
+                const uint32_t rowb = ((row[p] + static_cast<uint32_t>(r - 1)) & hmask) << lw;
+                elo[p][r] = start[rowb + wc];
+                ehi[p][r] = start[rowb + wc + 1u];
+            }
+        }
+    }
+    // wave-uniform row blocks: first slot a (even), staged length n (even),
+    // last staged slot that is a candidate ub
+    uint32_t a[3], n[3], ub[3];
 #pragma unroll
     for (int r = 0; r < 3; r++) {
         bool edge = false;
@@ -543,6 +563,7 @@ __device__ __forceinline__ void wave_count_in_cells(const float2 (&me)[P], const
         }
         n[r] = 0u;
         a[r] = 0u;
+        ub[r] = 0u;
         if (__ballot(edge) != 0ull) {  // the wave's rows are not monotone here
 #pragma unroll
             for (int p = 0; p < P; p++) {
@@ -566,6 +587,7 @@ __device__ __forceinline__ void wave_count_in_cells(const float2 (&me)[P], const
                                                                               static_cast<int>(last_lane)));
         a[r] = u_lo & ~1u;
         n[r] = (u_hi - a[r] + 1u) & ~1u;
+        ub[r] = u_hi;
     }
     const uint32_t off[3] = {0u, n[0], n[0] + n[1]};
     const uint32_t U = off[2] + n[2];
@@ -580,6 +602,19 @@ __device__ __forceinline__ void wave_count_in_cells(const float2 (&me)[P], const
                     if (n[r]) {
                         cnt[p] += count_staged(stage, off[r] + lo[p][r] - a[r], off[r] + hi[p][r] - a[r], me[p], T);
                     }
+                }
+            }
+        }
+        // The edge column's candidates usually lie inside the staged rows
+        // (narrow tables: the wave's rows are staged whole); those are
+        // counted from LDS and marked done.
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+                if (ecol[p] && n[r] && elo[p][r] >= a[r] && ehi[p][r] <= ub[r]) {
+                    cnt[p] += count_staged(stage, off[r] + elo[p][r] - a[r], off[r] + ehi[p][r] - a[r], me[p], T);
+                    ehi[p][r] = elo[p][r];
                 }
             }
         }
@@ -608,13 +643,10 @@ __device__ __forceinline__ void wave_count_in_cells(const float2 (&me)[P], const
     }
 #pragma unroll
     for (int p = 0; p < P; p++) {
-        if (valid[p] && (col[p] == 0u || col[p] == wmask) && wmask != 0u) {  // the column across the table's edge
-            const uint32_t wc = col[p] == 0u ? wmask : 0u;
-#pragma unroll 1
+        if (ecol[p]) {  // the rest of the column across the table's edge, from global memory
+#pragma unroll
             for (int r = 0; r < 3; r++) {
-                const uint32_t rowb = ((row[p] + static_cast<uint32_t>(r - 1)) & hmask) << lw;
-                const uint32_t k1 = start[rowb + wc + 1u];
-                for (uint32_t k = start[rowb + wc]; k < k1; k++) {
+                for (uint32_t k = elo[p][r]; k < ehi[p][r]; k++) {
                     cnt[p] += in_range(bpos[k], me[p], T) ? 1u : 0u;
                 }
             }
