@@ -839,35 +839,47 @@ static __global__ __launch_bounds__(64) void k_collide_finish(unsigned long long
 
 // Gathers the per-agent arrays into bucket order so that agents that are
 // close in space are close in memory.  Workgroup row blockIdx.y handles one
-// array, named by the row's 4-bit code in `codes` (kRow*).  Rows are
-// dispatched one after the other, so the resident workgroups mostly gather
-// from the same array and share its cache lines.  Each thread handles
+// array or a group of small ones, named by the row's 4-bit code in `codes`
+// (kRow*); a row's arrays share its slot-map reads.  Rows are dispatched one
+// after the other, so the resident workgroups mostly gather from the same
+// arrays and share their cache lines.  Each thread handles
 // kReorderU slots kBlock apart and issues all of their slot-map loads, then
 // all of their gathers, before the first store: a thread with one slot spends
 // two dependent memory round trips on 4 to 16 bytes.  Positions need no
 // gather: the binning's bpos[slot] is pos[bid[slot]] already.
 // k_bid_identity then resets bid.
 constexpr uint32_t kReorderU = 8;
-constexpr uint32_t kRowPos = 0, kRowTgt = 1, kRowDir = 2, kRowRoad = 3, kRowRng = 4, kRowInit = 5,
-                   kRowNcount = 6, kRowPerm = 7, kRowSpd = 8;
+// Row codes: arrays that share a row share its slot-map reads.
+constexpr uint32_t kRowPos = 0, kRowTgtRoad = 1, kRowRng = 2, kRowInitPerm = 3, kRowInitPermSpd = 4,
+                   kRowDirNcount = 5;
 
-template <class E>
-__device__ __forceinline__ void reorder_gather(const E* __restrict__ src, E* __restrict__ dst,
-                                               const uint32_t* __restrict__ bid, uint32_t s0, uint32_t n) {
+// dst_a[s] = src_a[bid[s]] (and likewise for b, c when given) for this
+// thread's kReorderU slots s0 + u * kBlock.
+template <class A, class B = uint32_t, class C = uint32_t>
+__device__ __forceinline__ void reorder_gather(const uint32_t* __restrict__ bid, uint32_t s0, uint32_t n,
+                                               const A* __restrict__ sa, A* __restrict__ da,
+                                               const B* __restrict__ sb = nullptr, B* __restrict__ db = nullptr,
+                                               const C* __restrict__ sc = nullptr, C* __restrict__ dc = nullptr) {
     uint32_t j[kReorderU];
-    E v[kReorderU];
+    A va[kReorderU];
+    B vb[kReorderU];
+    C vc[kReorderU];
 #pragma unroll
     for (uint32_t u = 0; u < kReorderU; u++) {
         j[u] = bid[min(s0 + u * kBlock, n - 1u)];  // s0 < n
     }
 #pragma unroll
     for (uint32_t u = 0; u < kReorderU; u++) {
-        v[u] = src[j[u]];
+        va[u] = sa[j[u]];
+        if (sb) vb[u] = sb[j[u]];
+        if (sc) vc[u] = sc[j[u]];
     }
 #pragma unroll
     for (uint32_t u = 0; u < kReorderU; u++) {
         if (s0 + u * kBlock < n) {
-            dst[s0 + u * kBlock] = v[u];
+            da[s0 + u * kBlock] = va[u];
+            if (sb) db[s0 + u * kBlock] = vb[u];
+            if (sc) dc[s0 + u * kBlock] = vc[u];
         }
     }
 }
@@ -889,14 +901,13 @@ static __global__ __launch_bounds__(kBlock) void k_reorder(AgentsDev src, Agents
                 }
             }
             break;
-        case kRowTgt: reorder_gather(src.tgt, dst.tgt, bid, s0, n); break;
-        case kRowDir: reorder_gather(src.dir, dst.dir, bid, s0, n); break;
-        case kRowRoad: reorder_gather(src.road, dst.road, bid, s0, n); break;
-        case kRowRng: reorder_gather(src.rng, dst.rng, bid, s0, n); break;
-        case kRowInit: reorder_gather(src.init, dst.init, bid, s0, n); break;
-        case kRowNcount: reorder_gather(src.ncount, dst.ncount, bid, s0, n); break;
-        case kRowPerm: reorder_gather(src.perm, dst.perm, bid, s0, n); break;
-        default: reorder_gather(src.spd, dst.spd, bid, s0, n); break;
+        case kRowTgtRoad: reorder_gather(bid, s0, n, src.tgt, dst.tgt, src.road, dst.road); break;
+        case kRowRng: reorder_gather(bid, s0, n, src.rng, dst.rng); break;
+        case kRowInitPerm: reorder_gather(bid, s0, n, src.init, dst.init, src.perm, dst.perm); break;
+        case kRowInitPermSpd:
+            reorder_gather(bid, s0, n, src.init, dst.init, src.perm, dst.perm, src.spd, dst.spd);
+            break;
+        default: reorder_gather(bid, s0, n, src.dir, dst.dir, src.ncount, dst.ncount); break;
     }
 }
 
@@ -921,15 +932,11 @@ inline void launch_reorder(const AgentsDev& src, const AgentsDev& dst, uint32_t*
     uint32_t rows = 0;
     auto row = [&](uint32_t code) { codes |= static_cast<unsigned long long>(code) << (4u * rows++); };
     row(kRowPos);
-    row(kRowTgt);
-    row(kRowRoad);
+    row(kRowTgtRoad);
     row(kRowRng);
-    row(kRowInit);
-    row(kRowPerm);
-    if (src.spd) row(kRowSpd);
+    row(src.spd ? kRowInitPermSpd : kRowInitPerm);
     if (live_outputs) {
-        row(kRowDir);
-        row(kRowNcount);
+        row(kRowDirNcount);
     }
     hipLaunchKernelGGL(k_reorder, dim3(nb, rows), dim3(kBlock), 0, stream, src, dst,
                        static_cast<const uint32_t*>(bid), bpos, codes);
