@@ -116,6 +116,26 @@ def test_batch_options_do_not_change_results(opts):
     check_against_plain(scs, 64, **opts)
 
 
+def test_batch_run_pieces_match_plain_dispatch_loops():
+    """Batched renumberings inside captured segments skip the heading and
+    neighbour-count arrays; after every run_ticks call of uneven length each
+    scenario's state equals a plain dispatch loop's."""
+    scs = S.batch_config(3, scale=0.01, seed=31)
+    b = msim.Batch(scs, graph_ticks=8, reorder_interval=3)
+    plain = [msim.Simulation(sc.roads, sc.conns, sc.entities, sc.pc, speed=sc.speed) for sc in scs]
+    done = 0
+    for k in (1, 10, 2, 25, 9):
+        b.run_ticks(k)
+        for g in plain:
+            for _ in range(k):
+                g.sim_tick()
+        done += k
+        for s, g in enumerate(plain):
+            assert_state_equal(b.state(s), g.state(), f"scenario {s} after {done} ticks")
+    for g in plain:
+        g.close()
+
+
 def test_batch_initialised_agents_move_on_the_first_tick():
     scs = S.batch_config(3, scale=0.01, seed=9)
     for sc in scs:

@@ -90,6 +90,27 @@ def test_dispatch_loop_equals_fused_graph_path():
     np.testing.assert_array_equal(ha, hb)
 
 
+@pytest.mark.parametrize("reorder", [1, 4])
+def test_run_pieces_match_dispatch_loop_after_each_piece(reorder):
+    """Renumberings inside captured run segments skip the heading and
+    neighbour-count arrays (the run's last tick rewrites both).  After every
+    run_ticks call of uneven length (some replay captured segments, some end
+    in direct ticks) the whole state equals a dispatch loop's."""
+    sc = S.config(1, scale=0.3)
+    a = msim.Simulation(sc.roads, sc.conns, sc.entities, sc.pc, speed=sc.speed, graph_ticks=8,
+                        reorder_interval=reorder)
+    b = msim.Simulation(sc.roads, sc.conns, sc.entities, sc.pc, speed=sc.speed, reorder_interval=reorder)
+    done = 0
+    for k in (1, 9, 17, 2, 33, 16, 8):
+        a.run_ticks(k)
+        for _ in range(k):
+            b.sim_tick()
+        done += k
+        sa, sb = a.state(), b.state()
+        for key in sa:
+            np.testing.assert_array_equal(sa[key], sb[key], err_msg=f"{key} after {done} ticks")
+
+
 @pytest.mark.parametrize("opts", [dict(reorder_interval=1), dict(reorder_interval=13, graph_ticks=5),
                                   dict(graph_ticks=1), dict(table_log2=12), dict(table_log2=20),
                                   dict(binning=1), dict(binning=1, reorder_interval=5),
