@@ -77,7 +77,7 @@ def plain_line(args, sc):
 
     n = sc.entities.shape[0]
     opts = dict(autotune.DEFAULT) if args.untuned else autotune.tuned_options(sc)
-    for k in ("binning", "collide_mode", "bin_unroll", "reorder_interval"):
+    for k in ("binning", "collide_mode", "bin_unroll", "reorder_interval", "segment_log2"):
         if getattr(args, k) is not None:
             opts[k] = getattr(args, k)
     sim = msim.Simulation(sc.roads, sc.conns, sc.entities, sc.pc, speed=sc.speed, **opts)
@@ -291,6 +291,8 @@ def main():
                                                                     "2 slots per lane; 1 per lane; 2 staged, 1 slot)")
     ap.add_argument("--reorder-interval", type=int, default=None, help="plain workloads: override the tuned "
                     "renumbering interval (ticks)")
+    ap.add_argument("--segment-log2", type=int, default=None, help="plain workloads: override the tuned "
+                    "rebinning segment size (log2 buckets)")
     ap.add_argument("--bin-unroll", type=int, default=None, help="override the tuned rebinning load batching (4, 8)")
     ap.add_argument("--arrival-gradient", choices=("frozen", "carry"), default="carry",
                     help="calibration (--config 5): how the adjoint treats arrivals")
