@@ -56,8 +56,11 @@ typedef struct msim_options {
                                   per-segment offsets, count and place passes);
                                   1: global-atomic counting; 2: segment-owned
                                   in one pass (count, look-back over the
-                                  earlier segments' totals, place); batched:
-                                  0 or 2                                        */
+                                  earlier segments' totals, place); 3: sorted
+                                  state: agent state kept in bucket order and
+                                  advanced and rebinned by one launch per move
+                                  dispatch (reorder_interval is ignored);
+                                  batched: 0, 2 or 3                            */
     uint32_t route_queue;      /* 0: arrivals routed inside the advance kernel;
                                   1: queued and routed by a separate pass       */
     uint32_t collide_mode;     /* 0: each wave stages its candidates in LDS with
@@ -66,7 +69,8 @@ typedef struct msim_options {
                                   lane                                          */
     uint32_t bin_unroll;       /* window slots each rebinning thread loads per
                                   round: 4 or 8 (0: 8 for segments of up to
-                                  2^13 buckets, else 4)                         */
+                                  2^13 buckets, else 4); binning 3: 2, 4 or 8
+                                  (0: 4)                                        */
 } msim_options;
 
 typedef struct msim_info {
@@ -156,6 +160,17 @@ int msim_read_pairs(msim_sim* sim, uint64_t* total, uint32_t* per_tick, uint32_t
  * the integer cell (floor(x/cell), floor(y/cell)) of every agent as binned by
  * the last move dispatch (2 ints per agent).  Synchronous. */
 int msim_read_cells(msim_sim* sim, int32_t* cell_xy);
+
+/* Grid occupancy for the UI overlay, the counterpart of the quadtree nodes
+ * get_quad_tree_nodes hands the UI (Simulator.cpp:149-153, Node layout
+ * GpuQuadTree.hpp:14-41, drawn as rectangles by QuadTreeGridGlObject.cpp:
+ * 30-50): every occupied cell of the last binning, cell_xy = (cx, cy) (the
+ * world rectangle [cx, cx + 1) x [cy, cy + 1) times msim_info.cell_size) and
+ * counts = its agents, computed on the device.  *n_cells receives the number
+ * of occupied cells; min(*n_cells, capacity) entries are written, in no
+ * particular order.  Synchronous. */
+int msim_read_grid_occupancy(msim_sim* sim, int32_t* cell_xy, uint32_t* counts, uint32_t capacity,
+                             uint32_t* n_cells);
 
 int msim_get_info(msim_sim* sim, msim_info* out);
 
@@ -258,6 +273,11 @@ int msim_batch_diff_forward(msim_batch* b, const float* theta, float alpha, int 
  * double per packed road) and grad_alpha (device, 1 double) receive dL/dtheta
  * and dL/dalpha with the run's decisions and neighbour cells frozen. */
 int msim_batch_diff_backward(msim_batch* b, const float* grad_pos, double* grad_theta, double* grad_alpha);
+
+/* Number of recorded (non-replay) forward runs so far.  msim_batch_diff_backward
+ * differentiates the most recent one; a caller holding the result of an
+ * earlier run compares this number to tell.  No reference counterpart. */
+int msim_batch_diff_generation(msim_batch* b, uint64_t* out);
 
 /* One tick's recorded decisions (host, caller order, packed road indices):
  * target (2 floats per agent), road, flag (bit 0 arrived, bit 1 no move).

@@ -30,7 +30,71 @@ void badvance(msim_batch* b) {
                        b->part_stats, b->roads, b->conns);
 }
 
+constexpr uint32_t kBSortedThreads = 512;
+
+inline size_t bsorted_lds_bytes(uint32_t seg_log, uint32_t unroll) {
+    return (sizeof(uint32_t) << seg_log) + sizeof(uint2) * kBSortedThreads * unroll;
+}
+
+SortedBufs bsorted_bufs(const msim_batch* b, int k) {
+    const BBuffers& x = b->buf[k];
+    return SortedBufs{x.pos, x.tgt, b->uniform ? nullptr : x.spd, x.perm, b->nxt[k], x.dir};
+}
+
+template <uint32_t U, bool UN, bool W>
+void blaunch_sorted(msim_batch* b, const SortedMove& mv, const BatchBin& bb) {
+    hipLaunchKernelGGL((k_bmove_sorted<U, kBSortedThreads, UN, W>), dim3(b->n_seg), dim3(kBSortedThreads),
+                       bsorted_lds_bytes(b->max_seg_log, U), b->stream, mv, bb, 1u << b->max_seg_log);
+}
+
+template <uint32_t U>
+void blaunch_sorted_u(msim_batch* b, const SortedMove& mv, const BatchBin& bb, bool write_dir) {
+    if (b->uniform) {
+        if (write_dir) blaunch_sorted<U, true, true>(b, mv, bb);
+        else blaunch_sorted<U, true, false>(b, mv, bb);
+    } else {
+        if (write_dir) blaunch_sorted<U, false, true>(b, mv, bb);
+        else blaunch_sorted<U, false, false>(b, mv, bb);
+    }
+}
+
+// Sorted move of every scenario: buf[cur] / binning bcur -> buf[1-cur] /
+// binning 1-bcur in one launch.
+int benqueue_move_sorted(msim_batch* b, bool write_dir) {
+    b->launch_no = 0;
+    const int nb = 1 - b->cur;
+    SortedMove mv{};
+    mv.src = bsorted_bufs(b, b->cur);
+    mv.dst = bsorted_bufs(b, nb);
+    mv.road = b->cold_road;
+    mv.rng = b->cold_rng;
+    mv.roads = b->roads;
+    mv.conns = b->conns;
+    mv.spd_const = kShaderSpeed;
+    mv.part_stats = b->part_stats;
+    mv.faults = b->faults;
+    BatchBin bb{};
+    bb.scen = b->scen;
+    bb.seg_scen = b->seg_scen;
+    bb.old_start = b->bstart[b->bcur];
+    bb.new_start = b->bstart[1 - b->bcur];
+    bb.ticket = b->bin_ticket;
+    bb.status = b->seg_status;
+    bb.faults = b->faults;
+    bb.inv_cell = b->inv_cell;
+    bb.n_seg = b->n_seg;
+    bprobe_pre(b);
+    if (b->bin_unroll == 2) blaunch_sorted_u<2>(b, mv, bb, write_dir);
+    else if (b->bin_unroll == 8) blaunch_sorted_u<8>(b, mv, bb, write_dir);
+    else blaunch_sorted_u<4>(b, mv, bb, write_dir);
+    bprobe_post(b);
+    b->cur = nb;
+    b->bcur = 1 - b->bcur;
+    return bcheck(b, "batched sorted move dispatch");
+}
+
 int benqueue_move(msim_batch* b, bool write_dir) {
+    if (b->sorted) return benqueue_move_sorted(b, write_dir);
     b->launch_no = 0;
     bprobe_pre(b);
     if (b->uniform) {
@@ -79,9 +143,28 @@ int btick(msim_batch* b, bool write_dir) {
     if (rc == MSIM_OK) rc = benqueue_collide(b);
     b->tick += 2;
     b->since_reorder++;
-    if (rc == MSIM_OK && b->reorder_interval && b->since_reorder >= b->reorder_interval) rc = breorder(b);
+    if (rc == MSIM_OK && !b->sorted && b->reorder_interval && b->since_reorder >= b->reorder_interval) {
+        rc = breorder(b);
+    }
     return rc;
 }
+
+// Enters the sorted state right after the creation's renumbering (buf[cur]
+// in the order of binning bcur; road and PRNG state already uploaded in
+// caller order): the next bucket of every slot.
+int benter_sorted(msim_batch* b) {
+    const dim3 g(bblocks(b->n));
+    if (b->uniform) {
+        hipLaunchKernelGGL(k_bsorted_next<true>, g, dim3(kBlock), 0, b->stream, bsorted_bufs(b, b->cur), b->blk_scen,
+                           b->scen, b->agent_off, b->n, kShaderSpeed, b->inv_cell);
+    } else {
+        hipLaunchKernelGGL(k_bsorted_next<false>, g, dim3(kBlock), 0, b->stream, bsorted_bufs(b, b->cur), b->blk_scen,
+                           b->scen, b->agent_off, b->n, kShaderSpeed, b->inv_cell);
+    }
+    b->sorted = true;
+    return bcheck(b, "sorted state set-up");
+}
+
 
 // Captures seg_ticks ticks (whole reorder periods, each ending with its
 // reorder) from the current buffers, right after a reorder.
@@ -150,6 +233,37 @@ void host_bin(const std::vector<ScenDev>& sc, const msim_entity* ents, float inv
 
 }  // namespace
 
+namespace msim {
+
+// Leaves the sorted state for the classic layout (the differentiable tick
+// runs on it): road and PRNG state gathered into slot order, the binning's
+// slot map reset to the identity and its positions copied.
+int bleave_sorted(msim_batch* b) {
+    if (!b->sorted) return MSIM_OK;
+    const BBuffers& x = b->buf[b->cur];
+    const dim3 g(bblocks(b->n));
+    hipLaunchKernelGGL(k_gather_perm<uint32_t>, g, dim3(kBlock), 0, b->stream, x.perm, b->cold_road, x.road, b->n);
+    hipLaunchKernelGGL(k_gather_perm<uint4>, g, dim3(kBlock), 0, b->stream, x.perm, b->cold_rng, x.rng, b->n);
+    hipLaunchKernelGGL(k_bid_identity, g, dim3(kBlock), 0, b->stream, b->bbid[b->bcur], b->n);
+    int rc = bcheck(b, "sorted state hand-over");
+    if (rc != MSIM_OK) return rc;
+    if (b->n) {
+        hipError_t e = hipMemcpyAsync(b->bbpos[b->bcur], x.pos, sizeof(float2) * b->n, hipMemcpyDeviceToDevice,
+                                      b->stream);
+        if (e != hipSuccess) return bfail(b, MSIM_ERR_HIP, "position copy: %s", hipGetErrorString(e));
+    }
+    b->sorted = false;
+    for (auto& row : b->gexec)
+        for (auto& gx : row)
+            if (gx.exec) {
+                (void)hipGraphExecDestroy(gx.exec);
+                gx.exec = nullptr;
+            }
+    return MSIM_OK;
+}
+
+}  // namespace msim
+
 extern "C" int msim_batch_create(uint32_t n_scen, const uint32_t* road_off, const msim_road* roads,
                                  const uint32_t* conn_off, const uint32_t* connections, const uint32_t* agent_off,
                                  const msim_entity* entities, const float* speed, const msim_push_consts* pc,
@@ -191,11 +305,12 @@ extern "C" int msim_batch_create(uint32_t n_scen, const uint32_t* road_off, cons
     if (o.collide_mode > 2) {
         return bfail(nullptr, MSIM_ERR_INVALID, "collide_mode must be 0, 1 or 2");
     }
-    if (o.bin_unroll != 0 && o.bin_unroll != 4 && o.bin_unroll != 8) {
-        return bfail(nullptr, MSIM_ERR_INVALID, "bin_unroll must be 0, 4 or 8");
+    if (o.binning != 0 && o.binning != 2 && o.binning != 3) {
+        return bfail(nullptr, MSIM_ERR_INVALID,
+                     "batched binning must be 0 (two passes), 2 (one pass) or 3 (sorted state)");
     }
-    if (o.binning != 0 && o.binning != 2) {
-        return bfail(nullptr, MSIM_ERR_INVALID, "batched binning must be 0 (two passes) or 2 (one pass)");
+    if (o.bin_unroll != 0 && o.bin_unroll != 4 && o.bin_unroll != 8 && !(o.binning == 3 && o.bin_unroll == 2)) {
+        return bfail(nullptr, MSIM_ERR_INVALID, "bin_unroll must be 0, 4 or 8 (or 2 with binning 3)");
     }
     const float cell = o.cell_size ? o.cell_size : pow2_cell(radius);
     if (!is_pow2_float(cell) || cell < radius) {
@@ -235,7 +350,8 @@ extern "C" int msim_batch_create(uint32_t n_scen, const uint32_t* road_off, cons
     b->first_tick_insert_only = (init0 == 0);
     b->tick = pc->tick;
     b->graph_ticks = o.graph_ticks ? o.graph_ticks : 100u;
-    b->reorder_interval = opt ? o.reorder_interval : 20u;
+    // the sorted state is in bucket order after every move: no renumbering
+    b->reorder_interval = o.binning == 3 ? 0u : (opt ? o.reorder_interval : 20u);
     b->seg_ticks = segment_ticks(b->graph_ticks, b->reorder_interval);
     b->collide_mode = static_cast<int>(o.collide_mode);
     b->binning = static_cast<int>(o.binning);
@@ -275,7 +391,11 @@ extern "C" int msim_batch_create(uint32_t n_scen, const uint32_t* road_off, cons
         d.lw = auto_width_log2(tlog, rows_log);
         uint32_t seg_log = auto_segment_log2(tlog, d.lw);
         if (o.segment_log2) seg_log = std::min(std::max(o.segment_log2, 9u), tlog);
-        while (seg_log > 9 && (static_cast<size_t>(4) << seg_log) + 1024 > static_cast<size_t>(lds_max)) seg_log--;
+        const uint32_t su = o.bin_unroll ? o.bin_unroll : 4u;
+        auto lds_need = [&](uint32_t l) {
+            return (o.binning == 3 ? bsorted_lds_bytes(l, su) : (static_cast<size_t>(4) << l)) + 1024;
+        };
+        while (seg_log > 9 && lds_need(seg_log) > static_cast<size_t>(lds_max)) seg_log--;
         d.seg_log = seg_log;
         d.m = 1u << tlog;
         d.wmask = (1u << d.lw) - 1u;
@@ -289,7 +409,7 @@ extern "C" int msim_batch_create(uint32_t n_scen, const uint32_t* road_off, cons
         b->max_seg_log = std::max(b->max_seg_log, seg_log);
     }
     b->n_seg = static_cast<uint32_t>(seg_scen.size());
-    b->bin_unroll = o.bin_unroll ? o.bin_unroll : auto_bin_unroll(b->max_seg_log);
+    b->bin_unroll = o.bin_unroll ? o.bin_unroll : (o.binning == 3 ? 4u : auto_bin_unroll(b->max_seg_log));
     b->start_len = start_len;
 
     int rc = MSIM_OK;
@@ -300,7 +420,9 @@ extern "C" int msim_batch_create(uint32_t n_scen, const uint32_t* road_off, cons
     BALLOC(b->conns, n_conns);
     for (int k = 0; k < 2; k++) {
         BBuffers& bb = b->buf[k];
-        BALLOC(bb.pos, n_adv);  // k_advance4 moves kAdv agents per thread
+        // k_advance4 moves kAdv agents per thread; the neighbour pass's
+        // 16-byte staged reads of the sorted state may touch slots n, n + 1
+        BALLOC(bb.pos, n_adv + 4);
         BALLOC(bb.tgt, n_adv);
         BALLOC(bb.dir, n_adv);
         if (!b->uniform) BALLOC(bb.spd, n_adv);
@@ -312,6 +434,12 @@ extern "C" int msim_batch_create(uint32_t n_scen, const uint32_t* road_off, cons
         BALLOC(b->bstart[k], start_len);
         BALLOC(b->bbpos[k], n + 2);  // 16-byte staged reads may touch one slot past n
         BALLOC(b->bbid[k], n);
+    }
+    if (b->binning == 3) {
+        BALLOC(b->nxt[0], n);
+        BALLOC(b->nxt[1], n);
+        BALLOC(b->cold_road, n);
+        BALLOC(b->cold_rng, n);
     }
     BALLOC(b->scen, n_scen);
     BALLOC(b->agent_off, n_scen + 1);
@@ -379,6 +507,10 @@ extern "C" int msim_batch_create(uint32_t n_scen, const uint32_t* road_off, cons
     up(b0.init, init.data(), sizeof(uint32_t) * n);
     up(b0.ncount, zeros.data(), sizeof(uint32_t) * n);
     up(b0.perm, perm.data(), sizeof(uint32_t) * n);
+    if (b->binning == 3) {  // caller order: the sorted state's road and PRNG state
+        up(b->cold_road, road.data(), sizeof(uint32_t) * n);
+        up(b->cold_rng, rng.data(), sizeof(uint4) * n);
+    }
     up(b->scen, sc.data(), sizeof(ScenDev) * n_scen);
     up(b->agent_off, agent_off, sizeof(uint32_t) * (n_scen + 1));
     std::vector<BlockScen> blk(bblocks(n));
@@ -413,6 +545,7 @@ extern "C" int msim_batch_create(uint32_t n_scen, const uint32_t* road_off, cons
         if (e != hipSuccess) rc = bfail(b, MSIM_ERR_HIP, "upload failed: %s", hipGetErrorString(e));
     }
     if (rc == MSIM_OK) rc = breorder(b);
+    if (rc == MSIM_OK && b->binning == 3) rc = benter_sorted(b);
     if (rc == MSIM_OK) rc = bwait(b);
     if (rc != MSIM_OK) {
         std::string e = b->err;
@@ -470,9 +603,10 @@ extern "C" int msim_batch_sync(msim_batch* b) {
     int rc = bwait(b);
     if (rc != MSIM_OK) return rc;
     if (f) {
-        return bfail(b, MSIM_ERR_STATE, "device-side check failed (fault bits 0x%x%s%s)", f,
+        return bfail(b, MSIM_ERR_STATE, "device-side check failed (fault bits 0x%x%s%s%s)", f,
                      (f & kFaultBinCount) ? ": binned agent count != agent count" : "",
-                     (f & kFaultLookback) ? ": rebinning look-back timed out" : "");
+                     (f & kFaultLookback) ? ": rebinning look-back timed out" : "",
+                     (f & kFaultNext) ? ": an agent's position left its counted bucket" : "");
     }
     return MSIM_OK;
 }
@@ -497,8 +631,9 @@ extern "C" int msim_batch_read_state(msim_batch* b, uint32_t scen, float* pos, f
     dn(p.data(), bb.pos + off, sizeof(float2) * na);
     dn(t.data(), bb.tgt + off, sizeof(float2) * na);
     dn(d.data(), bb.dir + off, sizeof(float2) * na);
-    dn(r.data(), bb.road + off, sizeof(uint32_t) * na);
-    dn(g.data(), bb.rng + off, sizeof(uint4) * na);
+    // sorted state: road and PRNG state are kept in caller order
+    dn(r.data(), (b->sorted ? b->cold_road : bb.road) + off, sizeof(uint32_t) * na);
+    dn(g.data(), (b->sorted ? b->cold_rng : bb.rng) + off, sizeof(uint4) * na);
     dn(c.data(), bb.ncount + off, sizeof(uint32_t) * na);
     dn(pm.data(), bb.perm + off, sizeof(uint32_t) * na);
     if (rc != MSIM_OK) return rc;
@@ -517,12 +652,13 @@ extern "C" int msim_batch_read_state(msim_batch* b, uint32_t scen, float* pos, f
             direction[2 * i] = d[k].x;
             direction[2 * i + 1] = d[k].y;
         }
-        if (road_index) road_index[i] = r[k] - road_base;
+        const uint32_t c_i = b->sorted ? i : k;  // caller-order arrays are indexed by agent
+        if (road_index) road_index[i] = r[c_i] - road_base;
         if (rand_state) {
-            rand_state[4 * i] = g[k].x;
-            rand_state[4 * i + 1] = g[k].y;
-            rand_state[4 * i + 2] = g[k].z;
-            rand_state[4 * i + 3] = g[k].w;
+            rand_state[4 * i] = g[c_i].x;
+            rand_state[4 * i + 1] = g[c_i].y;
+            rand_state[4 * i + 2] = g[c_i].z;
+            rand_state[4 * i + 3] = g[c_i].w;
         }
         if (neighbour_count) neighbour_count[i] = c[k];
     }

@@ -17,6 +17,7 @@
 #include "msim_bin.hpp"
 #include "msim_host.hpp"
 #include "msim_kernels.hpp"
+#include "msim_sorted.hpp"
 
 namespace msim {
 
@@ -111,6 +112,32 @@ static __global__ __launch_bounds__(T, 8) void k_bbin_fused(const float2* __rest
     }
 }
 
+// Sorted move of every scenario's table (see msim_sorted.hpp): segments are
+// handed out by ticket, each scenario's look-back chain starts at its first
+// segment (prefix = the scenario's first slot).  Dynamic LDS: counters for
+// the largest segment (list_off words), then the per-wave arrival lists.
+template <uint32_t U, uint32_t T, bool UNIFORM, bool WRITE_DIR>
+static __global__ __launch_bounds__(T) void k_bmove_sorted(SortedMove mv, BatchBin b, uint32_t list_off) {
+    extern __shared__ uint32_t hist[];
+    __shared__ uint32_t lds[T / 64];
+    __shared__ uint32_t sh[3];
+    take_segment(b.ticket, b.n_seg, sh);
+    const uint32_t g = sh[0];
+    const ScenDev sc = b.scen[b.seg_scen[g]];
+    uint2* alist = reinterpret_cast<uint2*>(hist + list_off);
+    uint32_t base, total;
+    sorted_segment<U, T, UNIFORM, WRITE_DIR>(mv, scen_view(sc, b), (g - sc.seg_first) << sc.seg_log, hist, alist,
+                                             lds, &sh[2], b.new_start + sc.start_base, b.status, g, sc.seg_first,
+                                             sc.agent_off, sh[1], &base, &total);
+    if (g == sc.seg_first + sc.n_seg - 1 && threadIdx.x == 0) {
+        const uint32_t end = base + total;
+        b.new_start[sc.start_base + sc.m] = end;
+        if (end != sc.agent_off + sc.n) {
+            atomicOr(b.faults, kFaultBinCount);
+        }
+    }
+}
+
 // Table parameters of the scenario holding a block's first slot, and the
 // slot where that scenario ends (read with scalar loads: block-uniform).
 struct BlockScen {
@@ -130,6 +157,28 @@ struct BatchCollide {
     float d2_limit;
     unsigned long long* part;  // [n_scen][kScenSlots * kScenSlotStride]
 };
+
+// nxt for every slot of every scenario: the bucket of the agent's next
+// position in its own scenario's table.
+template <bool UNIFORM>
+static __global__ __launch_bounds__(kBlock) void k_bsorted_next(SortedBufs st, const BlockScen* __restrict__ blk,
+                                                                const ScenDev* __restrict__ scen,
+                                                                const uint32_t* __restrict__ agent_off, uint32_t n,
+                                                                float spd_const, float inv_cell) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    const BlockScen bs = blk[blockIdx.x];
+    uint32_t sid = bs.sid;
+    if (i >= bs.end) {
+        sid++;
+        while (i >= agent_off[sid + 1]) sid++;
+    }
+    const ScenDev sc = scen[sid];
+    const float2 np = next_position(st.pos[i], st.tgt[i], UNIFORM ? spd_const : st.spd[i]);
+    st.nxt[i] = bucket_of(cell_coord(np.x, inv_cell), cell_coord(np.y, inv_cell), sc.lw, sc.wmask, sc.hmask);
+}
 
 // Per-scenario partial sums of a wave's counts: one atomic per wave when
 // all its valid slots lie in one scenario, one per slot otherwise.
@@ -183,7 +232,7 @@ static __global__ __launch_bounds__(kBlock) void k_bcollide(BatchCollide c) {
         const ScenDev& sc = c.scen[sid];
         cnt = count_in_cells(me, s, cx, cy, c.start + sc.start_base, sc.lw, sc.wmask, sc.hmask, c.bpos,
                              c.d2_limit) - 1u;  // own slot
-        c.ncount[c.bid[s]] = cnt;
+        c.ncount[c.bid ? c.bid[s] : s] = cnt;  // no slot map: the state is in slot order
     }
     bpart_add(c, (blk * kBlock + threadIdx.x) >> 6, valid, sid, cnt);
 }
@@ -223,7 +272,7 @@ static __global__ __launch_bounds__(kBlock, P == 1 ? 8 : 6) void k_bcollide_stag
         const uint32_t s = w0 + 64u * p + lane;
         valid[p] = s < c.n;
         me[p] = valid[p] ? c.bpos[s] : make_float2(0.0f, 0.0f);
-        id[p] = valid[p] ? c.bid[s] : 0u;
+        id[p] = valid[p] ? (c.bid ? c.bid[s] : s) : 0u;  // no slot map: the state is in slot order
         cx[p] = cell_coord(me[p].x, c.inv_cell);
         cy[p] = cell_coord(me[p].y, c.inv_cell);
         later = later || (valid[p] && s >= bs.end);
@@ -290,6 +339,9 @@ struct BBuffers {
 struct DiffState;  // msim_diff.hip: parameters, checkpoints and tape
 struct msim_batch;
 void diff_release(msim_batch* b);  // msim_diff.hip
+namespace msim {
+int bleave_sorted(msim_batch* b);  // msim_batch.hip
+}  // namespace msim
 
 struct msim_batch {
     hipStream_t stream = nullptr;
@@ -321,7 +373,14 @@ struct msim_batch {
     bool first_tick_insert_only = false;
     bool capturing = false;  // inside bbuild_graph: renumberings skip the output-only arrays
     int collide_mode = 0;  // 0: staged, 2 slots per lane; 1: per lane; 2: staged, 1 slot per lane
-    int binning = 0;  // 0: k_bbin_count + k_bbin_place; 2: k_bbin_fused
+    int binning = 0;  // 0: k_bbin_count + k_bbin_place; 2: k_bbin_fused; 3: sorted state (k_bmove_sorted)
+    // Sorted state (binning 3, from creation on): pos, tgt, spd, perm, dir and
+    // ncount of buf[cur] follow the slot order of binning bcur, which flips
+    // with cur on every move; road and PRNG state stay in packed caller order.
+    bool sorted = false;
+    uint32_t* nxt[2] = {nullptr, nullptr};  // per buffer: bucket of each slot's next position
+    uint32_t* cold_road = nullptr;          // [n] packed caller order
+    uint4* cold_rng = nullptr;              // [n] packed caller order
     uint32_t bin_unroll = 4;  // window slots per rebinning thread and load round
     uint32_t tick = 1;
     uint32_t graph_ticks = 100, seg_ticks = 100;
@@ -393,8 +452,12 @@ inline int bcheck(msim_batch* b, const char* what) {
 // Launches of a batched tick, in order, by kernel name (template arguments
 // dropped).
 constexpr int kBatchKernelMax = 5;
-inline int batch_kernel_count(const msim_batch* b) { return b->binning == 2 ? 4 : 5; }
+inline int batch_kernel_count(const msim_batch* b) { return b->sorted ? 3 : (b->binning == 2 ? 4 : 5); }
 inline const char* batch_kernel_name(const msim_batch* b, int k) {
+    if (b->sorted) {
+        if (k == 0) return "k_bmove_sorted";
+        return k == 1 ? (b->collide_mode == 1 ? "k_bcollide" : "k_bcollide_staged") : "k_bfinish";
+    }
     if (k == 0) return "k_advance4";
     if (b->binning == 2) {
         if (k == 1) return "k_bbin_fused";
@@ -474,8 +537,9 @@ inline int benqueue_collide(msim_batch* b) {
     c.agent_off = b->agent_off;
     c.n = b->n;
     c.start = b->bstart[b->bcur];
-    c.bpos = b->bbpos[b->bcur];
-    c.bid = b->bbid[b->bcur];
+    // sorted state: positions read in place, counts stored by slot
+    c.bpos = b->sorted ? b->buf[b->cur].pos : b->bbpos[b->bcur];
+    c.bid = b->sorted ? nullptr : b->bbid[b->bcur];
     c.ncount = b->buf[b->cur].ncount;
     c.inv_cell = b->inv_cell;
     c.d2_limit = b->d2_limit;

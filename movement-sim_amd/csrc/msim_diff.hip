@@ -106,10 +106,14 @@ struct DiffCells {
     float d2_limit;
     float inv_r2;
     double inv_r2_d;
+    float fix_scale;   // forward avoidance: terms summed as integers in units of 1/fix_scale
+    float push_allow;  // largest push per axis the rebinning windows were sized for
 };
 
 // Calls f(slot) for every other slot within the d2 limit of slot s, over the
-// 3x3 cells of s's scenario table, in a fixed order.
+// 3x3 cells of s's scenario table.  The order of the slots inside a bucket
+// follows the rebinning's LDS atomics and is not reproducible: callers sum in
+// an order-independent way.
 template <class F>
 __device__ __forceinline__ void for_neighbours(const DiffCells& c, uint32_t s, float2 me, F&& f) {
     const BlockScen bs = c.blk[s / kBlock];
@@ -148,22 +152,36 @@ __device__ __forceinline__ void for_neighbours(const DiffCells& c, uint32_t s, f
 }
 
 // Forward avoidance: pos[agent] = p~ + alpha * A over the tick's binning.
-static __global__ __launch_bounds__(kBlock) void k_davoid(DiffCells c, float alpha, float2* __restrict__ pos) {
+// Each binary32 term w x is rounded to a multiple of 1/fix_scale (a power of
+// two with |w x| * fix_scale < 2^40, at most 2^-40 R) and the terms are
+// summed as 64-bit integers, so A does not depend on the order in which the
+// neighbours are visited and two runs with the same parameters are
+// bit-identical.  A push beyond the windows' allowance raises kFaultBinCount
+// (the run then fails instead of rebinning an agent outside its window).
+static __global__ __launch_bounds__(kBlock) void k_davoid(DiffCells c, float alpha, float2* __restrict__ pos,
+                                                   unsigned int* __restrict__ faults) {
     const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
     if (s >= c.n) {
         return;
     }
     const float2 me = c.bpos[s];
-    float ax = 0.0f, ay = 0.0f;
+    long long ix = 0, iy = 0;
+    const double sc = c.fix_scale;
     for_neighbours(c, s, me, [&](uint32_t k) {
         const float2 q = c.bpos[k];
         const float x = me.x - q.x;
         const float y = me.y - q.y;
         const float w = avoid_w(x * x + y * y, c.inv_r2);
-        ax += w * x;
-        ay += w * y;
+        ix += __double2ll_rn(static_cast<double>(w * x) * sc);
+        iy += __double2ll_rn(static_cast<double>(w * y) * sc);
     });
-    pos[c.bid[s]] = make_float2(me.x + alpha * ax, me.y + alpha * ay);
+    const float ax = static_cast<float>(static_cast<double>(ix) / sc);
+    const float ay = static_cast<float>(static_cast<double>(iy) / sc);
+    const float px = alpha * ax, py = alpha * ay;
+    if (!(fabsf(px) <= c.push_allow) || !(fabsf(py) <= c.push_allow)) {
+        atomicOr(faults, kFaultBinCount);
+    }
+    pos[c.bid[s]] = make_float2(me.x + px, me.y + py);
 }
 
 // Avoidance adjoint over the tick's binning, in binary64 from the run's
@@ -245,12 +263,13 @@ static __global__ __launch_bounds__(kBlock) void k_dmove_adj(uint32_t n, const f
             const double ex = static_cast<double>(T2.x) - T.x;
             const double ey = static_cast<double>(T2.y) - T.y;
             const double E = sqrt(ex * ex + ey * ey);
-            if (E > 0.0) {
-                const uint32_t r = tp.road[k];
-                const uint32_t r2 = nxt_road[k];
-                const float v = UNIFORM ? spd_const : spd[k];
-                const double s = static_cast<double>(v * theta[r]);
-                const double s2 = static_cast<double>(v * theta[r2]);
+            const uint32_t r = tp.road[k];
+            const uint32_t r2 = nxt_road[k];
+            const float v = UNIFORM ? spd_const : spd[k];
+            const double s = static_cast<double>(v * theta[r]);
+            const double s2 = static_cast<double>(v * theta[r2]);
+            // a zero speed (theta[r] == 0) carries nothing: the frozen snap
+            if (E > 0.0 && s > 0.0) {
                 const double dx = static_cast<double>(T.x) - p.x;
                 const double dy = static_cast<double>(T.y) - p.y;
                 const double L = sqrt(dx * dx + dy * dy);
@@ -343,6 +362,9 @@ struct DiffState {
     ScenDev* scen = nullptr;  // scenario tables with the differentiable tick's margins
     std::vector<ScenDev> scen_h;
     float smax = 0.0f;  // largest base speed
+    uint32_t max_block = 1;  // most agents in one 3x3 cell block of the initial binning
+    float push_allow = 0.0f;  // avoidance push per axis the window margins allow for
+    uint64_t generation = 0;  // recorded (non-replay) forward runs so far
 };
 
 namespace {
@@ -399,11 +421,18 @@ void load_initial(msim_batch* b, const DiffSnapshot& s) {
 }
 
 // Margins of the rebinning windows for the largest displacement of a
-// differentiable tick: speed * theta_max plus 2R for the avoidance push (the
-// device check kFaultBinCount reports a larger one).
-int diff_margins(msim_batch* b, DiffState& d) {
+// differentiable tick: speed * theta_max plus the avoidance push.  Per axis
+// the push is |alpha| sum_j w x_j <= |alpha| k max_u (1 - u^2)^3 u R
+// (u = |x| / R; the maximum, at u^2 = 1/7, is below 0.24) for k neighbours;
+// the margin allows for k = the most agents found in one 3x3 block of cells
+// at set-up, and at least 2R.  A larger push (agents bunched tighter later)
+// is caught on the device (kFaultBinCount, the run fails).
+int diff_margins(msim_batch* b, DiffState& d, float alpha) {
     const float cell = 1.0f / b->inv_cell;
-    const double disp = static_cast<double>(d.smax) * d.theta_max + 2.0 * b->radius;
+    const double push = std::max(2.0 * b->radius, 0.24 * std::fabs(static_cast<double>(alpha)) * d.max_block *
+                                                      static_cast<double>(b->radius));
+    d.push_allow = static_cast<float>(push);
+    const double disp = static_cast<double>(d.smax) * d.theta_max + push;
     const uint32_t D = static_cast<uint32_t>(std::floor(disp * (1.0 + 1.0 / 1048576.0) / cell)) + 1u;
     for (ScenDev& s : d.scen_h) s.margin = (D + 1u) * (s.wmask + 1u) + D;
     hipError_t e = hipMemcpy(d.scen, d.scen_h.data(), sizeof(ScenDev) * d.scen_h.size(), hipMemcpyHostToDevice);
@@ -444,6 +473,9 @@ DiffCells dcells(msim_batch* b, const uint32_t* start, const float2* bpos, const
     c.d2_limit = b->d2_limit;
     c.inv_r2 = 1.0f / (b->radius * b->radius);
     c.inv_r2_d = 1.0 / (static_cast<double>(b->radius) * b->radius);
+    // |w x| < R: terms below 2^40 units, sums of up to 2^22 terms in range
+    c.fix_scale = std::ldexp(1.0f, 40 - std::max(0, static_cast<int>(std::ceil(std::log2(b->radius)))));
+    c.push_allow = b->diff ? b->diff->push_allow : 0.0f;
     return c;
 }
 
@@ -476,7 +508,7 @@ void dtick(msim_batch* b, DiffState& d, const float* theta, float alpha, uint32_
     dbins(b, d, old_start, old_bid, new_start, new_bpos, new_bid);
     if (alpha != 0.0f) {
         hipLaunchKernelGGL(k_davoid, g, dim3(kBlock), 0, b->stream, dcells(b, new_start, new_bpos, new_bid), alpha,
-                           a.pos);
+                           a.pos, b->faults);
     }
 }
 
@@ -507,7 +539,8 @@ extern "C" int msim_batch_diff_setup(msim_batch* b, uint32_t n_ticks, uint32_t c
     if (!b) return bfail(b, MSIM_ERR_INVALID, "NULL handle");
     if (b->diff) return bfail(b, MSIM_ERR_STATE, "the differentiable tick is already set up on this handle");
     if (n_ticks == 0 || checkpoint_every == 0) return bfail(b, MSIM_ERR_INVALID, "n_ticks and checkpoint_every must be > 0");
-    int rc = msim_batch_sync(b);
+    int rc = bleave_sorted(b);  // the differentiable tick runs on the classic layout
+    if (rc == MSIM_OK) rc = msim_batch_sync(b);
     if (rc != MSIM_OK) return rc;
     DiffState* d = new DiffState();
     b->diff = d;
@@ -546,6 +579,20 @@ extern "C" int msim_batch_diff_setup(msim_batch* b, uint32_t n_ticks, uint32_t c
         d->scen_h.resize(b->n_scen);
         hipError_t e = hipMemcpy(d->scen_h.data(), b->scen, sizeof(ScenDev) * b->n_scen, hipMemcpyDeviceToHost);
         if (e != hipSuccess) rc = bfail(b, MSIM_ERR_HIP, "scenario table readback: %s", hipGetErrorString(e));
+    }
+    if (rc == MSIM_OK && b->start_len) {
+        // the push allowance: at most 9 x the fullest bucket in a 3x3 block
+        std::vector<uint32_t> st(b->start_len);
+        hipError_t e = hipMemcpy(st.data(), b->bstart[b->bcur], sizeof(uint32_t) * b->start_len,
+                                 hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = bfail(b, MSIM_ERR_HIP, "binning readback: %s", hipGetErrorString(e));
+        uint32_t most = 1;
+        for (const ScenDev& sc : d->scen_h) {
+            for (uint32_t j = 0; j < sc.m && rc == MSIM_OK; j++) {
+                most = std::max(most, st[sc.start_base + j + 1] - st[sc.start_base + j]);
+            }
+        }
+        d->max_block = 9u * most;
     }
     if (rc == MSIM_OK) {
         d->smax = kShaderSpeed;
@@ -592,8 +639,7 @@ int set_params(msim_batch* b, DiffState& d, const float* theta, float alpha, flo
                               : hipSuccess;
     if (e != hipSuccess) return bfail(b, MSIM_ERR_HIP, "theta upload: %s", hipGetErrorString(e));
     d.theta_max = mx;
-    (void)alpha;
-    int rc = diff_margins(b, d);
+    int rc = diff_margins(b, d, alpha);
     // the uploads ran on the null stream, which the handle's stream does not wait for
     if (rc == MSIM_OK && hipStreamSynchronize(nullptr) != hipSuccess) {
         rc = bfail(b, MSIM_ERR_HIP, "parameter upload failed");
@@ -651,8 +697,17 @@ extern "C" int msim_batch_diff_forward(msim_batch* b, const float* theta, float 
         rc = bcheck(b, "k_dscatter");
     }
     if (rc == MSIM_OK) rc = check_diff_faults(b);
-    if (rc == MSIM_OK && !replay) d.have_forward = true;
+    if (rc == MSIM_OK && !replay) {
+        d.have_forward = true;
+        d.generation++;
+    }
     return rc;
+}
+
+extern "C" int msim_batch_diff_generation(msim_batch* b, uint64_t* out) {
+    if (!b || !b->diff || !out) return bfail(b, MSIM_ERR_STATE, "msim_batch_diff_setup first (or NULL argument)");
+    *out = b->diff->generation;
+    return MSIM_OK;
 }
 
 extern "C" int msim_batch_diff_backward(msim_batch* b, const float* grad_pos, double* grad_theta,

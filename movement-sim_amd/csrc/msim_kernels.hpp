@@ -675,7 +675,7 @@ static __global__ __launch_bounds__(kBlock, 8) void k_collide_staged(AgentsDev a
         const uint32_t s = w0 + 64u * p + __lane_id();
         valid[p] = s < a.n;
         me[p] = valid[p] ? bpos[s] : make_float2(0.0f, 0.0f);
-        id[p] = valid[p] ? g.bid[s] : 0u;
+        id[p] = valid[p] ? (g.bid ? g.bid[s] : s) : 0u;  // no slot map: the state is in slot order
         cx[p] = cell_coord(me[p].x, g.inv_cell);
         cy[p] = cell_coord(me[p].y, g.inv_cell);
         cnt[p] = 0u;
@@ -765,7 +765,7 @@ static __global__ __launch_bounds__(kBlock) void k_collide(AgentsDev a, GridDev 
         cnt -= 1u;  // the agent's own slot passed the predicate
     }
     if (valid) {
-        a.ncount[g.bid[s]] = cnt;
+        a.ncount[(SLOW || g.bid) ? g.bid[s] : s] = cnt;  // no slot map: the state is in slot order
     }
     const uint32_t bs = block_sum<kBlock>(SLOW ? own : cnt, lds);
     if (threadIdx.x == 0) {
@@ -1007,6 +1007,45 @@ static __global__ __launch_bounds__(kBlock) void k_export_cells(AgentsDev a, flo
     }
     const float2 p = a.pos[i];
     out[a.perm[i]] = make_int2(cell_coord(p.x, inv_cell), cell_coord(p.y, inv_cell));
+}
+
+// Occupied cells of the current binning, the grid's counterpart of the
+// quadtree nodes the UI overlay draws (src/sim/GpuQuadTree.hpp:14-41 via
+// Simulator::get_quad_tree_nodes, Simulator.cpp:149-153): one entry per
+// distinct cell (cx, cy) holding agents, with its agent count.  A bucket can
+// hold several world cells (the table is a torus), so each slot looks through
+// its own bucket: the bucket's first slot of a cell writes the cell, counted
+// over the bucket.  Entries are appended in no particular order; *total
+// receives the number of occupied cells, entries past cap are not written.
+static __global__ __launch_bounds__(kBlock) void k_occupancy(const float2* __restrict__ bpos,
+                                                             const uint32_t* __restrict__ start, uint32_t n,
+                                                             float inv_cell, uint32_t lw, uint32_t wmask,
+                                                             uint32_t hmask, int2* __restrict__ out_cell,
+                                                             uint32_t* __restrict__ out_count,
+                                                             uint32_t* __restrict__ total, uint32_t cap) {
+    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= n) {
+        return;
+    }
+    const float2 p = bpos[s];
+    const int32_t cx = cell_coord(p.x, inv_cell), cy = cell_coord(p.y, inv_cell);
+    const uint32_t b = bucket_of(cx, cy, lw, wmask, hmask);
+    const uint32_t lo = start[b], hi = start[b + 1];
+    uint32_t cnt = 0;
+    for (uint32_t k = lo; k < hi; k++) {
+        const float2 q = bpos[k];
+        if (cell_coord(q.x, inv_cell) == cx && cell_coord(q.y, inv_cell) == cy) {
+            if (k < s) {
+                return;  // an earlier slot writes this cell
+            }
+            cnt++;
+        }
+    }
+    const uint32_t idx = atomicAdd(total, 1u);
+    if (idx < cap) {
+        out_cell[idx] = make_int2(cx, cy);
+        out_count[idx] = cnt;
+    }
 }
 
 }  // namespace msim

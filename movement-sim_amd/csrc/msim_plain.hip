@@ -20,6 +20,7 @@
 #include "msim_bin.hpp"
 #include "msim_host.hpp"
 #include "msim_kernels.hpp"
+#include "msim_sorted.hpp"
 
 using namespace msim;
 
@@ -62,7 +63,18 @@ struct msim_sim {
     int bcur = 0;
     bool stats_pending = false;
     int binning = 0;  // 0: segment-owned (k_bin_count + k_bin_place), 1: atomic counting,
-                      // 2: segment-owned in one pass (k_bin_fused, look-back)
+                      // 2: segment-owned in one pass (k_bin_fused, look-back), 3: sorted state
+                      // (k_move_sorted: advance and rebinning in one pass, no slot map)
+    // Sorted state (binning 3), entered at the first move dispatch of
+    // initialised agents: pos, tgt, spd, perm (and dir, ncount) of buf[cur]
+    // follow the slot order of binning bcur, which flips with cur on every
+    // move; road and PRNG state stay in caller order in cold_road / cold_rng.
+    bool sorted = false;
+    uint32_t* nxt[2] = {nullptr, nullptr};  // per buffer: bucket of each slot's next position
+    uint32_t* cold_road = nullptr;          // [n] caller order
+    uint4* cold_rng = nullptr;              // [n] caller order
+    uint32_t* ncount_caller = nullptr;      // [n] neighbour counts kept in caller order
+    int ncount_at = 0;  // buffer whose slot order ncount follows (-1: ncount_caller)
     bool route_inline = false;  // arrivals routed inside k_advance instead of k_route
     bool capturing = false;  // inside build_graph: renumberings skip the output-only arrays
     int collide_mode = 0;  // 0: staged, 2 slots per lane; 1: per lane; 2: staged, 1 slot per lane
@@ -130,10 +142,16 @@ const char* const kSegKernels[] = {"k_bin_count", "k_bin_place"};
 const char* const kAtomicKernels[] = {"k_scan_reduce", "k_scan_partials", "k_scan_down", "k_scatter"};
 const char* const kFusedKernels[] = {"k_bin_fused"};
 uint32_t bin_kernels(const msim_sim* s) { return s->binning == 1 ? 4u : (s->binning == 2 ? 1u : 2u); }
-uint32_t kernels_per_tick(const msim_sim* s) { return (s->route_inline ? 1u : 2u) + bin_kernels(s) + 2u; }
+uint32_t kernels_per_tick(const msim_sim* s) {
+    return s->binning == 3 ? 3u : (s->route_inline ? 1u : 2u) + bin_kernels(s) + 2u;
+}
 // Name of launch k of a fused tick (the kernel's name, template arguments
 // dropped).
 const char* tick_kernel_name(const msim_sim* s, uint32_t k) {
+    if (s->binning == 3) {
+        if (k == 0) return "k_move_sorted";
+        return k == 1 ? (s->collide_mode == 1 ? "k_collide" : "k_collide_staged") : "k_collide_finish";
+    }
     if (k == 0) return (s->route_inline && s->binning != 1) ? "k_advance4" : "k_advance";
     if (!s->route_inline) {
         if (k == 1) return "k_route";
@@ -382,9 +400,17 @@ void flush_stats(msim_sim* s) {
 
 // Move dispatch: advance + bucket table rebuild.  ev (optional) receives an
 // event record after each of the 5 launches of the atomic path.
+int enqueue_move_sorted(msim_sim* s, bool write_dir);
+
 int enqueue_move(msim_sim* s, int mode, bool write_dir, hipEvent_t* ev = nullptr) {
     flush_stats(s);
     if (mode == kMoveAll) {
+        if (s->sorted) {
+            return enqueue_move_sorted(s, write_dir);
+        }
+        if (s->binning == 3) {
+            return fail(s, MSIM_ERR_STATE, "internal: sorted-state move dispatch before the state was sorted");
+        }
         return enqueue_move_fast(s, write_dir, ev);
     }
     switch (mode) {
@@ -420,6 +446,30 @@ int enqueue_collide(msim_sim* s, bool slow, hipEvent_t* ev = nullptr) {
     const dim3 g(agent_blocks(s->n));
     const uint32_t fold = s->stats_pending ? 1u : 0u;
     s->stats_pending = false;
+    if (s->sorted) {
+        // the state itself is in slot order: positions are read in place and
+        // the counts stored by slot
+        GridDev gd = s->grid_cur();
+        gd.bid = nullptr;
+        const float2* bpos = s->buf[s->cur].pos;
+        probe_pre(s);
+        if (s->collide_mode == 1) {
+            hipLaunchKernelGGL((k_collide<false>), g, dim3(kBlock), 0, s->stream, a, gd, bpos, s->part_pairs);
+        } else if (s->collide_mode == 2) {
+            hipLaunchKernelGGL(k_collide_staged<1>, g, dim3(kBlock), 0, s->stream, a, gd, bpos, s->part_pairs);
+        } else {
+            const uint32_t per = kBlock * 2u;
+            hipLaunchKernelGGL(k_collide_staged<2>, dim3(std::max(1u, (s->n + per - 1) / per)), dim3(kBlock), 0,
+                               s->stream, a, gd, bpos, s->part_pairs);
+        }
+        probe_post(s);
+        probe_pre(s);
+        hipLaunchKernelGGL(k_collide_finish<false>, dim3(1), dim3(64), 0, s->stream, s->part_pairs, s->ctr,
+                           s->hist, s->hist_cap - 1, s->part_stats, fold);
+        probe_post(s);
+        s->ncount_at = s->cur;
+        return check_launch(s, "collision dispatch");
+    }
     if (slow) {
         probe_pre(s);
         hipLaunchKernelGGL((k_collide<true>), g, dim3(kBlock), 0, s->stream, a, s->grid_cur(),
@@ -478,6 +528,135 @@ int reorder(msim_sim* s) {
     return MSIM_OK;
 }
 
+SortedBufs sorted_bufs(const msim_sim* s, int k) {
+    const Buffers& b = s->buf[k];
+    return SortedBufs{b.pos, b.tgt, s->uniform ? nullptr : b.spd, b.perm, s->nxt[k], b.dir};
+}
+
+constexpr uint32_t kSortedThreads = 512;
+
+size_t sorted_lds_bytes(uint32_t seg_log, uint32_t unroll) {
+    return (sizeof(uint32_t) << seg_log) + sizeof(uint2) * kSortedThreads * unroll;
+}
+
+// Enters the sorted state from the classic layout: renumbers the agents into
+// the current binning's order (slot map = identity), moves road and PRNG
+// state to caller order and computes every slot's next bucket.  Not inside a
+// graph capture.
+int enter_sorted(msim_sim* s) {
+    int rc = reorder(s);
+    if (rc != MSIM_OK) return rc;
+    const Buffers& b = s->buf[s->cur];
+    const dim3 g(agent_blocks(s->n));
+    hipLaunchKernelGGL(k_scatter_perm<uint32_t>, g, dim3(kBlock), 0, s->stream, b.perm, b.road, s->cold_road, s->n);
+    hipLaunchKernelGGL(k_scatter_perm<uint4>, g, dim3(kBlock), 0, s->stream, b.perm, b.rng, s->cold_rng, s->n);
+    const GridDev& gd = s->grid;
+    if (s->uniform) {
+        hipLaunchKernelGGL(k_sorted_next<true>, g, dim3(kBlock), 0, s->stream, sorted_bufs(s, s->cur), s->n,
+                           s->spd_const, gd.inv_cell, gd.lw, gd.wmask, gd.hmask);
+    } else {
+        hipLaunchKernelGGL(k_sorted_next<false>, g, dim3(kBlock), 0, s->stream, sorted_bufs(s, s->cur), s->n,
+                           s->spd_const, gd.inv_cell, gd.lw, gd.wmask, gd.hmask);
+    }
+    s->ncount_at = s->cur;
+    s->sorted = true;
+    return check_launch(s, "sorted state set-up");
+}
+
+int ensure_sorted(msim_sim* s) {
+    if (s->binning == 3 && !s->sorted && s->n_uninit == 0) {
+        return enter_sorted(s);
+    }
+    return MSIM_OK;
+}
+
+template <uint32_t U, bool UN, bool W>
+void launch_sorted(msim_sim* s, const SortedMove& mv, const BinDev& b) {
+    hipLaunchKernelGGL((k_move_sorted<U, kSortedThreads, UN, W>), dim3(s->n_seg), dim3(kSortedThreads),
+                       sorted_lds_bytes(s->seg_log, U), s->stream, mv, b);
+}
+
+template <uint32_t U>
+void launch_sorted_u(msim_sim* s, const SortedMove& mv, const BinDev& b, bool write_dir) {
+    if (s->uniform) {
+        if (write_dir) launch_sorted<U, true, true>(s, mv, b);
+        else launch_sorted<U, true, false>(s, mv, b);
+    } else {
+        if (write_dir) launch_sorted<U, false, true>(s, mv, b);
+        else launch_sorted<U, false, false>(s, mv, b);
+    }
+}
+
+// Sorted move dispatch: buf[cur] / binning bcur -> buf[1-cur] / binning
+// 1-bcur in one launch.
+int enqueue_move_sorted(msim_sim* s, bool write_dir) {
+    const int nb = 1 - s->cur;
+    const dim3 g(agent_blocks(s->n));
+    if (s->ncount_at == nb) {  // the move overwrites the slot order the last counts follow
+        hipLaunchKernelGGL(k_scatter_perm<uint32_t>, g, dim3(kBlock), 0, s->stream, s->buf[nb].perm,
+                           s->buf[nb].ncount, s->ncount_caller, s->n);
+        s->ncount_at = -1;
+    }
+    SortedMove mv{};
+    mv.src = sorted_bufs(s, s->cur);
+    mv.dst = sorted_bufs(s, nb);
+    mv.road = s->cold_road;
+    mv.rng = s->cold_rng;
+    mv.roads = s->roads;
+    mv.conns = s->conns;
+    mv.spd_const = s->spd_const;
+    mv.part_stats = s->part_stats;
+    mv.faults = &s->ctr->faults;
+    BinDev b{};
+    b.old_start = s->bstart[s->bcur];
+    b.new_start = s->bstart[1 - s->bcur];
+    b.ticket = s->bin_ticket;
+    b.status = s->seg_status;
+    b.ctr = s->ctr;
+    b.inv_cell = s->grid.inv_cell;
+    b.lw = s->grid.lw;
+    b.wmask = s->grid.wmask;
+    b.hmask = s->grid.hmask;
+    b.m = s->grid.m;
+    b.seg_log = s->seg_log;
+    b.n_seg = s->n_seg;
+    b.margin = s->margin;
+    b.n = s->n;
+    s->launch_no = 0;
+    probe_pre(s);
+    if (s->bin_unroll == 2) launch_sorted_u<2>(s, mv, b, write_dir);
+    else if (s->bin_unroll == 8) launch_sorted_u<8>(s, mv, b, write_dir);
+    else launch_sorted_u<4>(s, mv, b, write_dir);
+    probe_post(s);
+    s->cur = nb;
+    s->bcur = 1 - s->bcur;
+    s->stats_pending = true;  // folded by the next collision dispatch's finish pass
+    return check_launch(s, "sorted move dispatch");
+}
+
+// Makes the classic per-slot arrays of buf[cur] (road, rng, init, ncount)
+// readable for the readbacks; the sorted state stays in force.
+int publish_sorted(msim_sim* s) {
+    if (!s->sorted) return MSIM_OK;
+    const Buffers& b = s->buf[s->cur];
+    const dim3 g(agent_blocks(s->n));
+    if (s->ncount_at == 1 - s->cur) {
+        const Buffers& o = s->buf[1 - s->cur];
+        hipLaunchKernelGGL(k_scatter_perm<uint32_t>, g, dim3(kBlock), 0, s->stream, o.perm, o.ncount,
+                           s->ncount_caller, s->n);
+        s->ncount_at = -1;
+    }
+    if (s->ncount_at == -1) {
+        hipLaunchKernelGGL(k_gather_perm<uint32_t>, g, dim3(kBlock), 0, s->stream, b.perm, s->ncount_caller,
+                           b.ncount, s->n);
+        s->ncount_at = s->cur;
+    }
+    hipLaunchKernelGGL(k_gather_perm<uint32_t>, g, dim3(kBlock), 0, s->stream, b.perm, s->cold_road, b.road, s->n);
+    hipLaunchKernelGGL(k_gather_perm<uint4>, g, dim3(kBlock), 0, s->stream, b.perm, s->cold_rng, b.rng, s->n);
+    hipLaunchKernelGGL(k_set_init, g, dim3(kBlock), 0, s->stream, b.init, s->n);
+    return check_launch(s, "sorted state readback");
+}
+
 int wait_stream(msim_sim* s) {
     const auto t0 = std::chrono::steady_clock::now();
     for (;;) {
@@ -505,7 +684,7 @@ int after_move(msim_sim* s);
 // ending with its reorder, so every replay starts on freshly ordered agents.
 int build_graph(msim_sim* s) {
     hipGraph_t graph = nullptr;
-    const int cur0 = s->cur, bcur0 = s->bcur, qpar0 = s->qpar;
+    const int cur0 = s->cur, bcur0 = s->bcur, qpar0 = s->qpar, ncount_at0 = s->ncount_at;
     const uint32_t since0 = s->since_reorder;
     const uint64_t ticks0 = s->ticks_simulated;
     MSIM_TRY_HIP(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
@@ -525,6 +704,7 @@ int build_graph(msim_sim* s) {
     s->cur = cur0;  // capture only recorded the ticks; nothing ran
     s->bcur = bcur0;
     s->qpar = qpar0;
+    s->ncount_at = ncount_at0;
     s->since_reorder = since0;
     s->ticks_simulated = ticks0;
     if (rc != MSIM_OK) {
@@ -546,7 +726,7 @@ int build_graph(msim_sim* s) {
 int after_move(msim_sim* s) {
     s->ticks_simulated++;
     s->since_reorder++;
-    if (s->reorder_interval && s->since_reorder >= s->reorder_interval) {
+    if (!s->sorted && s->reorder_interval && s->since_reorder >= s->reorder_interval) {
         return reorder(s);
     }
     return MSIM_OK;
@@ -578,6 +758,8 @@ int single_dispatch(msim_sim* s, uint32_t tick) {
     s->pc.tick = tick;
     int rc;
     if ((tick & 1u) == 0u) {
+        rc = ensure_sorted(s);
+        if (rc != MSIM_OK) return rc;
         rc = enqueue_move(s, s->n_uninit ? kMoveInit : kMoveAll, true);
         s->n_uninit = 0;
         if (rc == MSIM_OK) rc = after_move(s);
@@ -666,11 +848,12 @@ extern "C" int msim_create(const msim_road* roads, uint32_t n_roads, const uint3
     if (o.collide_mode > 2) {
         return fail(nullptr, MSIM_ERR_INVALID, "collide_mode must be 0, 1 or 2");
     }
-    if (o.bin_unroll != 0 && o.bin_unroll != 4 && o.bin_unroll != 8) {
-        return fail(nullptr, MSIM_ERR_INVALID, "bin_unroll must be 0, 4 or 8");
+    if (o.binning > 3) {
+        return fail(nullptr, MSIM_ERR_INVALID,
+                    "binning must be 0 (segment), 1 (atomic), 2 (segment, one pass) or 3 (sorted state)");
     }
-    if (o.binning > 2) {
-        return fail(nullptr, MSIM_ERR_INVALID, "binning must be 0 (segment), 1 (atomic) or 2 (segment, one pass)");
+    if (o.bin_unroll != 0 && o.bin_unroll != 4 && o.bin_unroll != 8 && !(o.binning == 3 && o.bin_unroll == 2)) {
+        return fail(nullptr, MSIM_ERR_INVALID, "bin_unroll must be 0, 4 or 8 (or 2 with binning 3)");
     }
     if (!validate_extent(roads, n_roads, entities, n_entities, 1.0f / cell, &why)) {
         return fail(nullptr, MSIM_ERR_INVALID, "%s", why.c_str());
@@ -681,7 +864,8 @@ extern "C" int msim_create(const msim_road* roads, uint32_t n_roads, const uint3
     s->n_roads = n_roads;
     s->n_conns = n_connections;
     s->pc = *pc;
-    s->reorder_interval = o.reorder_interval;
+    // the sorted state is in bucket order after every move: no renumbering
+    s->reorder_interval = o.binning == 3 ? 0u : o.reorder_interval;
     s->graph_ticks = o.graph_ticks ? o.graph_ticks : 32u;
     s->seg_ticks = segment_ticks(s->graph_ticks, s->reorder_interval);
     s->watchdog_ms = o.watchdog_ms ? o.watchdog_ms : 600000;
@@ -708,8 +892,12 @@ extern "C" int msim_create(const msim_road* roads, uint32_t n_roads, const uint3
     // LDS per workgroup bounds the segment size (4 bytes per bucket counter).
     int lds_max = 65536;
     (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, s->device);
+    const uint32_t sorted_unroll = o.bin_unroll ? o.bin_unroll : 4u;
+    auto lds_need = [&](uint32_t l) {
+        return (o.binning == 3 ? sorted_lds_bytes(l, sorted_unroll) : (static_cast<size_t>(4) << l)) + 1024;
+    };
     uint32_t seg_use = seg_log;
-    while (seg_use > 9 && (static_cast<size_t>(4) << seg_use) + 1024 > static_cast<size_t>(lds_max)) seg_use--;
+    while (seg_use > 9 && lds_need(seg_use) > static_cast<size_t>(lds_max)) seg_use--;
     if (o.segment_log2 && seg_use != seg_log) {
         msim_destroy(s);
         return fail(nullptr, MSIM_ERR_INVALID, "segment_log2 %u needs more LDS than the device's %d bytes",
@@ -730,7 +918,7 @@ extern "C" int msim_create(const msim_road* roads, uint32_t n_roads, const uint3
     s->collide_mode = static_cast<int>(o.collide_mode);
     s->seg_log = seg_use;
     s->n_seg = g.m >> seg_use;
-    s->bin_unroll = o.bin_unroll ? o.bin_unroll : auto_bin_unroll(seg_use);
+    s->bin_unroll = o.binning == 3 ? sorted_unroll : (o.bin_unroll ? o.bin_unroll : auto_bin_unroll(seg_use));
     // An agent moves at most max_speed per axis per tick (the heading has
     // |dir| = speed up to rounding), i.e. at most D cells; in linear bucket
     // order that is at most (D + 1) rows plus D columns.
@@ -749,7 +937,9 @@ extern "C" int msim_create(const msim_road* roads, uint32_t n_roads, const uint3
     ALLOC(s->conns, n_connections);
     for (int k = 0; k < 2; k++) {
         Buffers& b = s->buf[k];
-        ALLOC(b.pos, n_adv);  // k_advance4 moves kAdv agents per thread
+        // k_advance4 moves kAdv agents per thread; the neighbour pass's
+        // 16-byte staged reads of the sorted state may touch slots n, n + 1
+        ALLOC(b.pos, n_adv + 4);
         ALLOC(b.tgt, n_adv);
         ALLOC(b.dir, n_adv);
         if (!s->uniform) ALLOC(b.spd, n_adv);
@@ -766,6 +956,13 @@ extern "C" int msim_create(const msim_road* roads, uint32_t n_roads, const uint3
         ALLOC(s->bstart[k], g.m + 1);
         ALLOC(s->bbpos[k], n + 2);  // 16-byte staged reads may touch one slot past n
         ALLOC(s->bbid[k], n);
+    }
+    if (s->binning == 3) {
+        ALLOC(s->nxt[0], n);
+        ALLOC(s->nxt[1], n);
+        ALLOC(s->cold_road, n);
+        ALLOC(s->cold_rng, n);
+        ALLOC(s->ncount_caller, n);
     }
     ALLOC(s->seg_total, s->n_seg);
     ALLOC(s->bin_ticket, 1);
@@ -859,8 +1056,11 @@ extern "C" int msim_run_ticks(msim_sim* s, uint32_t n_ticks) {
             done++;
             continue;
         }
+        rc = ensure_sorted(s);
+        if (rc != MSIM_OK) break;
         const uint32_t left = n_ticks - done;
-        if (left - 1 >= s->seg_ticks && (s->reorder_interval == 0 || s->since_reorder == 0)) {
+        if (left - 1 >= s->seg_ticks && (s->reorder_interval == 0 || s->since_reorder == 0) &&
+            (!s->sorted || s->ncount_at == s->cur)) {
             msim_sim::Segment& g = s->gexec[s->cur][s->bcur][s->qpar];
             if (!g.exec) {
                 rc = build_graph(s);
@@ -871,6 +1071,7 @@ extern "C" int msim_run_ticks(msim_sim* s, uint32_t n_ticks) {
             s->cur = g.cur;
             s->bcur = g.bcur;
             s->qpar = g.qpar;
+            if (s->sorted) s->ncount_at = s->cur;  // every captured tick ends with its collision dispatch
             s->pc.tick += 2 * s->seg_ticks;
             s->ticks_simulated += s->seg_ticks;
             s->any_collide = true;
@@ -894,9 +1095,10 @@ int check_faults(msim_sim* s) {
     int rc = wait_stream(s);
     if (rc != MSIM_OK) return rc;
     if (c.faults) {
-        return fail(s, MSIM_ERR_STATE, "device-side check failed (fault bits 0x%x%s%s)", c.faults,
+        return fail(s, MSIM_ERR_STATE, "device-side check failed (fault bits 0x%x%s%s%s)", c.faults,
                     (c.faults & kFaultBinCount) ? ": binned agent count != agent count" : "",
-                    (c.faults & kFaultLookback) ? ": rebinning look-back timed out" : "");
+                    (c.faults & kFaultLookback) ? ": rebinning look-back timed out" : "",
+                    (c.faults & kFaultNext) ? ": an agent's position left its counted bucket" : "");
     }
     return MSIM_OK;
 }
@@ -916,7 +1118,7 @@ struct HostState {
 
 int fetch_state(msim_sim* s, HostState& h, bool need_dir, bool need_rng) {
     const Buffers& b = s->buf[s->cur];
-    int rc = MSIM_OK;
+    int rc = publish_sorted(s);
     if (rc == MSIM_OK) rc = download(s, h.pos, b.pos, s->n);
     if (rc == MSIM_OK) rc = download(s, h.tgt, b.tgt, s->n);
     if (rc == MSIM_OK && need_dir) rc = download(s, h.dir, b.dir, s->n);
@@ -938,6 +1140,8 @@ extern "C" int msim_read_entities(msim_sim* s, msim_entity* out, uint32_t first,
     }
     if (count == 0) return msim_sync(s);
     flush_stats(s);
+    int rc0 = publish_sorted(s);
+    if (rc0 != MSIM_OK) return rc0;
     hipLaunchKernelGGL(k_pack_entities, dim3(agent_blocks(s->n)), dim3(kBlock), 0, s->stream, s->agents(s->cur),
                        s->color0, s->any_collide ? 1u : 0u, first, count, reinterpret_cast<uint4*>(s->ent_stage));
     int rc = check_launch(s, "k_pack_entities");
@@ -1031,6 +1235,47 @@ extern "C" int msim_read_cells(msim_sim* s, int32_t* cell_xy) {
     return rc;
 }
 
+extern "C" int msim_read_grid_occupancy(msim_sim* s, int32_t* cell_xy, uint32_t* counts, uint32_t capacity,
+                                        uint32_t* n_cells) {
+    if (!s || !n_cells || (capacity && (!cell_xy || !counts))) return fail(s, MSIM_ERR_INVALID, "NULL argument");
+    *n_cells = 0;
+    if (s->n == 0) return MSIM_OK;
+    const uint32_t cap = std::min(capacity, s->n);
+    const size_t cell_bytes = sizeof(int2) * std::max(cap, 1u), cnt_bytes = sizeof(uint32_t) * std::max(cap, 1u);
+    char* d = nullptr;
+    MSIM_TRY_HIP(s, hipMallocAsync(reinterpret_cast<void**>(&d), cell_bytes + cnt_bytes + 256, s->stream));
+    int2* d_cell = reinterpret_cast<int2*>(d);
+    uint32_t* d_cnt = reinterpret_cast<uint32_t*>(d + cell_bytes);
+    uint32_t* d_total = reinterpret_cast<uint32_t*>(d + cell_bytes + cnt_bytes);
+    int rc = MSIM_OK;
+    hipError_t e = hipMemsetAsync(d_total, 0, sizeof(uint32_t), s->stream);
+    if (e != hipSuccess) rc = fail(s, MSIM_ERR_HIP, "hipMemsetAsync: %s", hipGetErrorString(e));
+    if (rc == MSIM_OK) {
+        // the current binning: the sorted state itself, or the binned copy
+        const float2* bpos = s->sorted ? s->buf[s->cur].pos : s->bbpos[s->bcur];
+        hipLaunchKernelGGL(k_occupancy, dim3(agent_blocks(s->n)), dim3(kBlock), 0, s->stream, bpos, s->bstart[s->bcur],
+                           s->n, s->grid.inv_cell, s->grid.lw, s->grid.wmask, s->grid.hmask, d_cell, d_cnt, d_total,
+                           cap);
+        rc = check_launch(s, "k_occupancy");
+    }
+    uint32_t total = 0;
+    if (rc == MSIM_OK) {
+        e = hipMemcpyAsync(&total, d_total, sizeof(uint32_t), hipMemcpyDeviceToHost, s->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+        if (e != hipSuccess) rc = fail(s, MSIM_ERR_HIP, "occupancy readback: %s", hipGetErrorString(e));
+    }
+    const uint32_t k = std::min(total, cap);
+    if (rc == MSIM_OK && k) {
+        e = hipMemcpyAsync(cell_xy, d_cell, sizeof(int2) * k, hipMemcpyDeviceToHost, s->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(counts, d_cnt, sizeof(uint32_t) * k, hipMemcpyDeviceToHost, s->stream);
+        if (e != hipSuccess) rc = fail(s, MSIM_ERR_HIP, "occupancy readback: %s", hipGetErrorString(e));
+    }
+    (void)hipFreeAsync(d, s->stream);
+    if (rc == MSIM_OK) rc = wait_stream(s);
+    if (rc == MSIM_OK) *n_cells = total;
+    return rc;
+}
+
 extern "C" int msim_get_info(msim_sim* s, msim_info* out) {
     if (!s || !out) return fail(s, MSIM_ERR_INVALID, "NULL argument");
     out->n_agents = s->n;
@@ -1060,6 +1305,10 @@ extern "C" int msim_time_ticks(msim_sim* s, uint32_t n_ticks, float* ms_total, f
     if (!s) return fail(s, MSIM_ERR_INVALID, "NULL handle");
     if (s->n_uninit != 0) return fail(s, MSIM_ERR_STATE, "agents are not initialised yet; dispatch first");
     if ((s->pc.tick + 1) & 1u) return fail(s, MSIM_ERR_STATE, "next dispatch is not a move dispatch");
+    {
+        const int rc0 = ensure_sorted(s);
+        if (rc0 != MSIM_OK) return rc0;
+    }
     constexpr int kMaxK = 8;
     const int nk = static_cast<int>(kernels_per_tick(s));
     hipEvent_t ev[kMaxK + 1];

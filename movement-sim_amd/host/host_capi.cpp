@@ -205,12 +205,12 @@ int msim_host_sim_wait_ticks(msim_host_sim* s, uint64_t n, uint32_t timeout_ms) 
 int msim_host_sim_stats(const msim_host_sim* s, uint64_t* ticks, double* tps, int64_t* tick_ns,
                         int64_t* update_ns, int64_t* collision_ns) {
     if (!s) return fail(MSIM_ERR_INVALID, "null simulator");
-    const auto& x = *s->sim;
-    if (ticks) *ticks = x.ticks_done();
-    if (tps) *tps = x.get_tps().get_ticks();
-    if (tick_ns) *tick_ns = x.get_tps_history().get_avg_time().count();
-    if (update_ns) *update_ns = x.get_update_tick_history().get_avg_time().count();
-    if (collision_ns) *collision_ns = x.get_collision_detection_tick_history().get_avg_time().count();
+    const auto st = s->sim->stats();  // taken under the worker's lock
+    if (ticks) *ticks = s->sim->ticks_done();
+    if (tps) *tps = st.tps;
+    if (tick_ns) *tick_ns = st.tick.count();
+    if (update_ns) *update_ns = st.update.count();
+    if (collision_ns) *collision_ns = st.collision.count();
     return MSIM_OK;
 }
 

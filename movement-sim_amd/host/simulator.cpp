@@ -189,7 +189,10 @@ bool Simulator::sim_tick() {
         return false;
     }
     const std::chrono::nanoseconds upd = clock::now() - t0;
-    update_history_.add_time(upd);
+    {
+        std::lock_guard<std::mutex> lk(stats_mutex_);
+        update_history_.add_time(upd);
+    }
     pc_.tick++;
     t0 = clock::now();
     if (msim_dispatch(sim_, &pc_) != MSIM_OK || msim_sync(sim_) != MSIM_OK) {
@@ -197,7 +200,10 @@ bool Simulator::sim_tick() {
         return false;
     }
     const std::chrono::nanoseconds col = clock::now() - t0;
-    collision_history_.add_time(col);
+    {
+        std::lock_guard<std::mutex> lk(stats_mutex_);
+        collision_history_.add_time(col);
+    }
     write_log_csv(pc_.tick, upd, col, upd + col);
 
     bool want_entities, want_cells;
@@ -229,10 +235,19 @@ bool Simulator::sim_tick() {
         set_error(msim_last_error(sim_));
         return false;
     }
-    tps_history_.add_time(clock::now() - tick_start);
-    tps_.tick();
+    {
+        std::lock_guard<std::mutex> lk(stats_mutex_);
+        tps_history_.add_time(clock::now() - tick_start);
+        tps_.tick();
+    }
     ticks_++;
     return true;
+}
+
+Simulator::StatsSnapshot Simulator::stats() const {
+    std::lock_guard<std::mutex> lk(stats_mutex_);
+    return StatsSnapshot{tps_.get_ticks(), tps_history_.get_avg_time(), update_history_.get_avg_time(),
+                         collision_history_.get_avg_time()};
 }
 
 void Simulator::write_log_csv(uint32_t tick, std::chrono::nanoseconds upd, std::chrono::nanoseconds col,

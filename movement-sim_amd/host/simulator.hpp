@@ -71,6 +71,14 @@ class Simulator {
     void continue_simulation();
     void pause_simulation();
     bool is_simulating() const { return simulating_.load(); }
+    // The statistics are updated by the worker thread; the reference-style
+    // getters below are for a stopped or paused simulator, stats() reads
+    // them under the worker's lock at any time.
+    struct StatsSnapshot {
+        double tps;
+        std::chrono::nanoseconds tick, update, collision;
+    };
+    StatsSnapshot stats() const;
     const TickRate& get_tps() const { return tps_; }
     const TickDurationHistory& get_tps_history() const { return tps_history_; }
     const TickDurationHistory& get_update_tick_history() const { return update_history_; }
@@ -106,6 +114,7 @@ class Simulator {
     std::atomic<bool> failed_{false};
     std::condition_variable progress_cv_;  // signalled (under wait_mutex_) per tick and on pause
 
+    mutable std::mutex stats_mutex_;  // guards the histories and the tick rate
     TickDurationHistory tps_history_, update_history_, collision_history_;
     TickRate tps_;
 

@@ -82,6 +82,7 @@ SIGNATURES = {
     "msim_read_state": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP, _VP, _VP]),
     "msim_read_pairs": (ctypes.c_int, [_VP, _VP, _VP, _U32, _VP]),
     "msim_read_cells": (ctypes.c_int, [_VP, _VP]),
+    "msim_read_grid_occupancy": (ctypes.c_int, [_VP, _VP, _VP, _U32, _VP]),
     "msim_get_info": (ctypes.c_int, [_VP, _VP]),
     "msim_time_ticks": (ctypes.c_int, [_VP, _U32, _VP, _VP, _U32]),
     "msim_kernel_name": (ctypes.c_char_p, [_VP, _U32]),
@@ -105,6 +106,7 @@ SIGNATURES = {
     "msim_batch_diff_backward": (ctypes.c_int, [_VP, _VP, _VP, _VP]),
     "msim_batch_diff_read_schedule": (ctypes.c_int, [_VP, _U32, _VP, _VP, _VP]),
     "msim_batch_diff_set_arrival_gradient": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "msim_batch_diff_generation": (ctypes.c_int, [_VP, _VP]),
     # host driver (msim_host.h)
     "msim_host_map_load": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_VP)]),
     "msim_host_map_info": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP, _VP]),
@@ -278,6 +280,17 @@ class Simulation:
         self._check(self.lib.msim_read_cells(self.h, _ptr(out)))
         return out
 
+    def grid_occupancy(self, capacity=None):
+        """(cells [k, 2] int32, counts [k] uint32, number of occupied cells):
+        the occupied cells of the last binning, in no particular order."""
+        cap = self.n if capacity is None else capacity
+        cells = np.zeros((max(cap, 1), 2), np.int32)
+        counts = np.zeros(max(cap, 1), np.uint32)
+        total = ctypes.c_uint32()
+        self._check(self.lib.msim_read_grid_occupancy(self.h, _ptr(cells), _ptr(counts), cap, ctypes.byref(total)))
+        k = min(total.value, cap)
+        return cells[:k].copy(), counts[:k].copy(), total.value
+
     def info(self):
         i = Info()
         self._check(self.lib.msim_get_info(self.h, ctypes.byref(i)))
@@ -427,6 +440,13 @@ class Batch:
         self._check(self.lib.msim_batch_diff_backward(self.h, ctypes.c_void_p(grad_pos_ptr),
                                                       ctypes.c_void_p(grad_theta_ptr),
                                                       ctypes.c_void_p(grad_alpha_ptr)))
+
+    def diff_generation(self):
+        """Number of recorded (non-replay) forward runs; diff_backward
+        differentiates the latest."""
+        g = np.zeros(1, np.uint64)
+        self._check(self.lib.msim_batch_diff_generation(self.h, _ptr(g)))
+        return int(g[0])
 
     def diff_set_arrival_gradient(self, mode):
         """0: arrivals cut the gradient (frozen run); 1: carry-over surrogate."""

@@ -24,11 +24,17 @@ class _DiffFn(torch.autograd.Function):
         torch.cuda.synchronize(th.device)
         batch.diff_forward(th.data_ptr(), float(alpha.detach().item()), out.data_ptr())
         ctx.batch = batch
+        ctx.generation = batch.diff_generation()
         return out
 
     @staticmethod
     def backward(ctx, grad_pos):
         b = ctx.batch
+        if b.diff_generation() != ctx.generation:
+            # the handle keeps one run's checkpoints: the adjoint would be
+            # that of a later forward call, not of this one
+            raise RuntimeError("DiffRun: another forward run was made on this handle after the one being "
+                               "differentiated; call backward before the next forward")
         g = grad_pos.detach().to(torch.float32).contiguous()
         gt = torch.empty(max(1, b.n_roads), device=g.device, dtype=torch.float64)
         ga = torch.empty(1, device=g.device, dtype=torch.float64)

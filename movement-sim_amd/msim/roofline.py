@@ -37,9 +37,13 @@ def kernel_bytes(name, uniform_speed=False):
     in the one-pass segment path; k_collide / k_collide_staged 16.  The
     batched kernels k_bbin_count, k_bbin_place, k_bbin_fused, k_bcollide and
     k_bcollide_staged do the same work per agent as their plain
-    counterparts.  k_route (arrival path), the
-    single-workgroup scan/finish passes, k_bfinish and k_reorder: 0."""
+    counterparts.  k_move_sorted / k_bmove_sorted (advance and rebinning in
+    one pass over the bucket-sorted state) 48 (44 with the constant speed).
+    k_route (arrival path), the single-workgroup scan/finish passes,
+    k_bfinish and k_reorder: 0."""
     name = name.split("<")[0]
+    if name in ("k_move_sorted", "k_bmove_sorted"):
+        return (B_ADVANCE_UNIFORM_SPEED if uniform_speed else B_ADVANCE) + B_SCAN + B_SCATTER
     if name in ("k_advance", "k_advance4", "k_advance_bin"):
         return B_ADVANCE_UNIFORM_SPEED if uniform_speed else B_ADVANCE
     if name in ("k_bin_count", "k_bbin_count", "k_scan_down"):

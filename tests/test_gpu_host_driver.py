@@ -72,6 +72,67 @@ def test_worker_ticks_equal_a_plain_run_and_hand_over_refreshes(tmp_path):
         assert upd >= 0 and col >= 0 and tot >= 0
 
 
+def _records_of(o):
+    """Entity records the oracle's state reads back as
+    (random_move.comp:545-548,876 colours: blue in a pair, else green)."""
+    st = o.st
+    rec = np.zeros(st.pos.shape[0], msim.ENTITY_DTYPE)
+    rec["color"] = np.where((st.ncount > 0)[:, None], [0, 0, 1, 1], [0, 1, 0, 1])
+    rec["randomState"] = st.rng
+    rec["pos"] = st.pos
+    rec["target"] = st.tgt
+    rec["direction"] = st.dir
+    rec["roadIndex"] = st.road
+    rec["initialized"] = 1
+    return rec
+
+
+def test_worker_hand_over_matches_the_oracle(tmp_path):
+    """The worker thread's handed-over entity buffer (Simulator.cpp:159-295:
+    read after the tick that follows a take) and its pair counts against the
+    CPU oracle driven through the same protocol, after ticks 1 and T + 1."""
+    import ctypes
+
+    import orc
+
+    n, seed, T = 3000, 9, 40
+    m = host.HostMap(GOLDEN)
+    m.add_guard()
+    roads, conns = m.arrays()
+    ents = m.make_entities(n, seed)
+    i = m.info()
+    o = orc.OracleSim(roads, conns, ents, msim.push_consts(i["width"], i["height"], 10.0))
+    s = host.HostSimulator(GOLDEN, n, 10.0, seed, csv_path=tmp_path / "o.csv", tick_limit=T)
+    try:
+        first = s.take_entities()
+        np.testing.assert_array_equal(first.view(np.uint8), ents.view(np.uint8))
+        s.start_worker()
+        s.continue_simulation()
+        s.wait_ticks(T)
+        o.run_ticks(1)
+        got = s.take_entities()  # read after tick 1, the first tick after the take
+        np.testing.assert_array_equal(got.view(np.uint8), _records_of(o).view(np.uint8))
+        s.continue_simulation()
+        s.wait_ticks(T + 1)
+        o.run_ticks(T)
+        got = s.take_entities()  # read after tick T + 1
+        np.testing.assert_array_equal(got.view(np.uint8), _records_of(o).view(np.uint8))
+        lib = msim.load_library()
+        total = np.zeros(1, np.uint64)
+        per = np.zeros(T + 1, np.uint32)
+        k = ctypes.c_uint32()
+        assert lib.msim_read_pairs(ctypes.c_void_p(s.handle()), total.ctypes.data, per.ctypes.data, T + 1,
+                                   ctypes.byref(k)) == 0
+        assert int(total[0]) == o.pairs_total
+        assert per[: k.value].tolist() == o.pair_hist[-k.value:]
+        dbg = np.zeros(10, np.uint32)
+        assert lib.msim_read_debug(ctypes.c_void_p(s.handle()), dbg.ctypes.data) == 0
+        assert int(dbg[1]) == o.pairs_total % 2**32
+        s.stop_worker()
+    finally:
+        s.close()
+
+
 def test_pause_stops_the_worker(tmp_path):
     s = host.HostSimulator(GOLDEN, 500, 10.0, 2, csv_path=tmp_path / "p.csv")
     try:
