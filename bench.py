@@ -36,37 +36,76 @@ WORKLOADS = {
 }
 
 
+def lavapipe_probe():
+    """Is the reference's CPU Vulkan path (Mesa lavapipe) available here?"""
+    import ctypes.util
+    import glob
+    import shutil
+
+    icds = sorted(set(glob.glob("/usr/share/vulkan/icd.d/*.json") + glob.glob("/etc/vulkan/icd.d/*.json")))
+    found = {"vulkaninfo": shutil.which("vulkaninfo"), "libvulkan": ctypes.util.find_library("vulkan"),
+             "icd_files": icds, "glslc": shutil.which("glslc") or shutil.which("glslangValidator")}
+    found["lavapipe"] = any("lvp" in os.path.basename(x) for x in icds)
+    return found
+
+
 def cpu_baseline(scs, seconds, threads):
-    """The CPU oracle (restatement; the reference itself needs a Vulkan ICD)
-    on a bounded sample of the workload: scenarios in order, each from its own
-    initial state (one untimed insert-only tick), ticked until its share of
-    `seconds` elapses."""
+    """The CPU oracle (restatement; the reference itself needs a Vulkan ICD,
+    see lavapipe_probe) on a bounded sample of the workload: the first
+    scenarios, each from its own initial state (one untimed insert-only
+    tick).  A warm-up run sizes each scenario's tick count to its share of
+    seconds / 3; then three timed runs of exactly those ticks from the
+    initial states, and the median rate is reported."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import orc
 
     k = min(len(scs), 8)
-    share = seconds / k
-    agent_ticks = 0
-    el_total = 0.0
-    parts = []
-    for sc in scs[:k]:
+    share = seconds / 3 / k
+
+    def run(sc, ticks=None):
         o = orc.OracleSim(sc.roads, sc.conns, sc.entities, sc.pc, speed=sc.speed, threads=threads)
         o.sim_tick()  # insert-only move dispatch + first collision dispatch
-        ticks = 0
+        done = 0
         t0 = time.perf_counter()
-        while True:
+        while (ticks is None and time.perf_counter() - t0 < share) or (ticks is not None and done < ticks):
             o.sim_tick()
-            ticks += 1
-            el = time.perf_counter() - t0
-            if el >= share:
-                break
-        n = sc.entities.shape[0]
-        agent_ticks += n * ticks
-        el_total += el
-        parts.append(f"{n}x{ticks}")
-    return {"value": agent_ticks / el_total, "unit": "agent-ticks/s", "cores": threads, "kind": "port",
-            "sample": f"first {k} scenario(s) of the same workload, agents x ticks = {', '.join(parts)} "
-                      f"({el_total:.1f} s), oracle/msim_oracle.c with {threads} POSIX threads"}
+            done += 1
+        return done, time.perf_counter() - t0
+
+    plan = [max(1, run(sc)[0]) for sc in scs[:k]]  # warm-up, sizes the sample
+    rates, secs = [], []
+    for _ in range(3):
+        at = el = 0.0
+        for sc, ticks in zip(scs[:k], plan):
+            _, e = run(sc, ticks)
+            at += sc.entities.shape[0] * ticks
+            el += e
+        rates.append(at / el)
+        secs.append(el)
+    parts = [f"{sc.entities.shape[0]}x{t}" for sc, t in zip(scs[:k], plan)]
+    return {"value": sorted(rates)[1], "unit": "agent-ticks/s", "cores": threads, "kind": "port",
+            "runs": rates,
+            "sample": f"first {k} scenario(s) of the same workload, agents x ticks = {', '.join(parts)}; median "
+                      f"of 3 runs ({', '.join(f'{x:.1f}' for x in secs)} s) after one warm-up; "
+                      f"oracle/msim_oracle.c with {threads} POSIX threads",
+            "reference_cpu_path": lavapipe_probe()}
+
+
+def prime_graphs(h, steps, opts):
+    """Captures the graph segments a run of `steps` ticks will replay, from
+    the parity the timed run starts with (a segment of an even number of
+    ticks ends on the buffers it started from), so no capture falls inside
+    the timed region."""
+    h.run_ticks(steps if steps % 2 == 0 else steps + 1)
+    h.sync()
+
+
+def graph_ticks_for(steps, tuned):
+    """Segment length for a timed run of `steps` ticks: the tuned length, cut
+    to an even number below steps so that the run replays captured segments
+    (its last tick always runs direct)."""
+    g = min(int(tuned or 100), steps - 1)
+    return max(2, g - (g % 2))
 
 
 def plain_line(args, sc):
@@ -201,12 +240,14 @@ def batched_line(args, scs):
     for k in ("binning", "collide_mode", "bin_unroll"):
         if getattr(args, k) is not None:
             opts[k] = getattr(args, k)
+    opts["graph_ticks"] = graph_ticks_for(args.steps, opts.get("graph_ticks"))
     b = msim.Batch(scs, **opts)
     extra = {}
     if args.compare_ticks > 0:
         extra["batched_vs_plain"] = compare_plain(args, scs, b, n)
     b.run_ticks(max(1, args.warmup))
     b.sync()
+    prime_graphs(b, args.steps, opts)
     t0 = time.perf_counter()
     b.run_ticks(args.steps)
     b.sync()

@@ -32,9 +32,7 @@ void badvance(msim_batch* b) {
 
 constexpr uint32_t kBSortedThreads = 512;
 
-inline size_t bsorted_lds_bytes(uint32_t seg_log, uint32_t unroll) {
-    return (sizeof(uint32_t) << seg_log) + sizeof(uint2) * kBSortedThreads * unroll;
-}
+inline size_t bsorted_lds_bytes(uint32_t seg_log) { return sorted_lds_bytes(seg_log); }
 
 SortedBufs bsorted_bufs(const msim_batch* b, int k) {
     const BBuffers& x = b->buf[k];
@@ -43,8 +41,14 @@ SortedBufs bsorted_bufs(const msim_batch* b, int k) {
 
 template <uint32_t U, bool UN, bool W>
 void blaunch_sorted(msim_batch* b, const SortedMove& mv, const BatchBin& bb) {
+    bprobe_pre(b);
     hipLaunchKernelGGL((k_bmove_sorted<U, kBSortedThreads, UN, W>), dim3(b->n_seg), dim3(kBSortedThreads),
-                       bsorted_lds_bytes(b->max_seg_log, U), b->stream, mv, bb, 1u << b->max_seg_log);
+                       bsorted_lds_bytes(b->max_seg_log), b->stream, mv, bb);
+    bprobe_post(b);
+    if (b->mark_move) (void)hipEventRecord(b->ev_move, b->stream);
+    bprobe_pre(b);
+    hipLaunchKernelGGL((k_broute_sorted<UN, W>), dim3(b->n_seg), dim3(kBlock), 0, b->stream, mv, bb);
+    bprobe_post(b);
 }
 
 template <uint32_t U>
@@ -73,6 +77,9 @@ int benqueue_move_sorted(msim_batch* b, bool write_dir) {
     mv.spd_const = kShaderSpeed;
     mv.part_stats = b->part_stats;
     mv.faults = b->faults;
+    mv.arr_slot = b->arr_slot;
+    mv.seg_base = b->seg_base;
+    mv.seg_arr = b->seg_arr;
     BatchBin bb{};
     bb.scen = b->scen;
     bb.seg_scen = b->seg_scen;
@@ -83,11 +90,9 @@ int benqueue_move_sorted(msim_batch* b, bool write_dir) {
     bb.faults = b->faults;
     bb.inv_cell = b->inv_cell;
     bb.n_seg = b->n_seg;
-    bprobe_pre(b);
     if (b->bin_unroll == 2) blaunch_sorted_u<2>(b, mv, bb, write_dir);
     else if (b->bin_unroll == 8) blaunch_sorted_u<8>(b, mv, bb, write_dir);
     else blaunch_sorted_u<4>(b, mv, bb, write_dir);
-    bprobe_post(b);
     b->cur = nb;
     b->bcur = 1 - b->bcur;
     return bcheck(b, "batched sorted move dispatch");
@@ -166,6 +171,41 @@ int benter_sorted(msim_batch* b) {
 }
 
 
+#define BTRY_HIP(call)                                                                        \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess) return bfail(b, MSIM_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+// The captured ticks of the sorted state on two streams: tick t's collision
+// dispatch (stream2) only reads the buffers tick t + 1's move (stream) reads
+// too, so they run side by side; tick t + 2's move overwrites what tick t's
+// collision dispatch read and waits for it.  Collision dispatches stay in
+// order on stream2.  Ends with stream2 joined.
+int bcapture_sorted_ticks(msim_batch* b) {
+    BTRY_HIP(hipEventRecord(b->ev_fork, b->stream));
+    BTRY_HIP(hipStreamWaitEvent(b->stream2, b->ev_fork, 0));
+    int rc = MSIM_OK;
+    for (uint32_t t = 0; t < b->seg_ticks && rc == MSIM_OK; t++) {
+        if (t >= 2) BTRY_HIP(hipStreamWaitEvent(b->stream, b->ev_coll[t & 1], 0));
+        b->mark_move = true;  // ev_move right after the move kernel: the routing runs beside the collision
+        rc = benqueue_move(b, false);
+        b->mark_move = false;
+        if (rc != MSIM_OK) break;
+        BTRY_HIP(hipStreamWaitEvent(b->stream2, b->ev_move, 0));
+        std::swap(b->stream, b->stream2);  // the collision dispatch's launches go to stream2
+        rc = benqueue_collide(b);
+        std::swap(b->stream, b->stream2);
+        if (rc != MSIM_OK) break;
+        BTRY_HIP(hipEventRecord(b->ev_coll[t & 1], b->stream2));
+        b->tick += 2;
+        b->since_reorder++;
+    }
+    BTRY_HIP(hipEventRecord(b->ev_join, b->stream2));
+    BTRY_HIP(hipStreamWaitEvent(b->stream, b->ev_join, 0));
+    return rc;
+}
+
 // Captures seg_ticks ticks (whole reorder periods, each ending with its
 // reorder) from the current buffers, right after a reorder.
 int bbuild_graph(msim_batch* b) {
@@ -176,8 +216,12 @@ int bbuild_graph(msim_batch* b) {
     if (e != hipSuccess) return bfail(b, MSIM_ERR_HIP, "hipStreamBeginCapture: %s", hipGetErrorString(e));
     int rc = MSIM_OK;
     b->capturing = true;  // run segments are always followed by the run's last tick
-    for (uint32_t t = 0; t < b->seg_ticks && rc == MSIM_OK; t++) {
-        rc = btick(b, false);
+    if (b->sorted) {
+        rc = bcapture_sorted_ticks(b);
+    } else {
+        for (uint32_t t = 0; t < b->seg_ticks && rc == MSIM_OK; t++) {
+            rc = btick(b, false);
+        }
     }
     b->capturing = false;
     e = hipStreamEndCapture(b->stream, &graph);
@@ -366,6 +410,16 @@ extern "C" int msim_batch_create(uint32_t n_scen, const uint32_t* road_off, cons
         delete b;
         return bfail(nullptr, MSIM_ERR_HIP, "hipStreamCreate failed");
     }
+    {
+        bool ok = hipStreamCreateWithFlags(&b->stream2, hipStreamNonBlocking) == hipSuccess;
+        for (hipEvent_t* e : {&b->ev_fork, &b->ev_move, &b->ev_join, &b->ev_coll[0], &b->ev_coll[1]}) {
+            ok = ok && hipEventCreateWithFlags(e, hipEventDisableTiming) == hipSuccess;
+        }
+        if (!ok) {
+            msim_batch_destroy(b);
+            return bfail(nullptr, MSIM_ERR_HIP, "stream / event creation failed");
+        }
+    }
     int dev = 0;
     (void)hipGetDevice(&dev);
     int lds_max = 65536;
@@ -391,9 +445,8 @@ extern "C" int msim_batch_create(uint32_t n_scen, const uint32_t* road_off, cons
         d.lw = auto_width_log2(tlog, rows_log);
         uint32_t seg_log = auto_segment_log2(tlog, d.lw);
         if (o.segment_log2) seg_log = std::min(std::max(o.segment_log2, 9u), tlog);
-        const uint32_t su = o.bin_unroll ? o.bin_unroll : 4u;
         auto lds_need = [&](uint32_t l) {
-            return (o.binning == 3 ? bsorted_lds_bytes(l, su) : (static_cast<size_t>(4) << l)) + 1024;
+            return (o.binning == 3 ? sorted_lds_bytes(l) : (static_cast<size_t>(4) << l)) + 1024;
         };
         while (seg_log > 9 && lds_need(seg_log) > static_cast<size_t>(lds_max)) seg_log--;
         d.seg_log = seg_log;
@@ -440,6 +493,7 @@ extern "C" int msim_batch_create(uint32_t n_scen, const uint32_t* road_off, cons
         BALLOC(b->nxt[1], n);
         BALLOC(b->cold_road, n);
         BALLOC(b->cold_rng, n);
+        BALLOC(b->arr_slot, n);
     }
     BALLOC(b->scen, n_scen);
     BALLOC(b->agent_off, n_scen + 1);
@@ -448,6 +502,10 @@ extern "C" int msim_batch_create(uint32_t n_scen, const uint32_t* road_off, cons
     BALLOC(b->seg_total, b->n_seg);
     BALLOC(b->bin_ticket, 1);
     BALLOC(b->seg_status, b->n_seg);
+    if (b->binning == 3) {
+        BALLOC(b->seg_base, b->n_seg);
+        BALLOC(b->seg_arr, b->n_seg);
+    }
     BALLOC(b->part, static_cast<size_t>(n_scen) * kScenSlots * kScenSlotStride);
     BALLOC(b->part_stats, kSlots * kSlotStride);
     BALLOC(b->totals, n_scen);
@@ -742,6 +800,10 @@ extern "C" int msim_batch_destroy(msim_batch* b) {
         for (auto& g : row)
             if (g.exec) (void)hipGraphExecDestroy(g.exec);
     for (void* p : b->allocs) (void)hipFree(p);
+    for (hipEvent_t e : {b->ev_fork, b->ev_move, b->ev_join, b->ev_coll[0], b->ev_coll[1]}) {
+        if (e) (void)hipEventDestroy(e);
+    }
+    if (b->stream2) (void)hipStreamDestroy(b->stream2);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return MSIM_OK;

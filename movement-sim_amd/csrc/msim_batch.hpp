@@ -115,19 +115,18 @@ static __global__ __launch_bounds__(T, 8) void k_bbin_fused(const float2* __rest
 // Sorted move of every scenario's table (see msim_sorted.hpp): segments are
 // handed out by ticket, each scenario's look-back chain starts at its first
 // segment (prefix = the scenario's first slot).  Dynamic LDS: counters for
-// the largest segment (list_off words), then the per-wave arrival lists.
+// the largest segment.
 template <uint32_t U, uint32_t T, bool UNIFORM, bool WRITE_DIR>
-static __global__ __launch_bounds__(T) void k_bmove_sorted(SortedMove mv, BatchBin b, uint32_t list_off) {
+static __global__ __launch_bounds__(T) void k_bmove_sorted(SortedMove mv, BatchBin b) {
     extern __shared__ uint32_t hist[];
     __shared__ uint32_t lds[T / 64];
-    __shared__ uint32_t sh[3];
+    __shared__ uint32_t sh[4];
     take_segment(b.ticket, b.n_seg, sh);
     const uint32_t g = sh[0];
     const ScenDev sc = b.scen[b.seg_scen[g]];
-    uint2* alist = reinterpret_cast<uint2*>(hist + list_off);
     uint32_t base, total;
-    sorted_segment<U, T, UNIFORM, WRITE_DIR>(mv, scen_view(sc, b), (g - sc.seg_first) << sc.seg_log, hist, alist,
-                                             lds, &sh[2], b.new_start + sc.start_base, b.status, g, sc.seg_first,
+    sorted_segment<U, T, UNIFORM, WRITE_DIR>(mv, scen_view(sc, b), (g - sc.seg_first) << sc.seg_log, hist, lds,
+                                             &sh[2], b.new_start + sc.start_base, b.status, g, sc.seg_first,
                                              sc.agent_off, sh[1], &base, &total);
     if (g == sc.seg_first + sc.n_seg - 1 && threadIdx.x == 0) {
         const uint32_t end = base + total;
@@ -157,6 +156,15 @@ struct BatchCollide {
     float d2_limit;
     unsigned long long* part;  // [n_scen][kScenSlots * kScenSlotStride]
 };
+
+// Route choice of the arrivals the batched sorted move listed, one
+// workgroup per segment, on the segment's own scenario table.
+template <bool UNIFORM, bool WRITE_DIR>
+static __global__ __launch_bounds__(kBlock) void k_broute_sorted(SortedMove mv, BatchBin b) {
+    const uint32_t g = blockIdx.x;
+    const ScenDev sc = b.scen[b.seg_scen[g]];
+    route_segment_list<UNIFORM, WRITE_DIR>(mv, scen_view(sc, b), g);
+}
 
 // nxt for every slot of every scenario: the bucket of the agent's next
 // position in its own scenario's table.
@@ -381,6 +389,9 @@ struct msim_batch {
     uint32_t* nxt[2] = {nullptr, nullptr};  // per buffer: bucket of each slot's next position
     uint32_t* cold_road = nullptr;          // [n] packed caller order
     uint4* cold_rng = nullptr;              // [n] packed caller order
+    uint32_t* arr_slot = nullptr;           // [n] arrivals' new slots, listed per segment
+    uint32_t* seg_base = nullptr;           // [n_seg] first slot of each segment
+    uint32_t* seg_arr = nullptr;            // [n_seg] arrivals of each segment
     uint32_t bin_unroll = 4;  // window slots per rebinning thread and load round
     uint32_t tick = 1;
     uint32_t graph_ticks = 100, seg_ticks = 100;
@@ -390,6 +401,12 @@ struct msim_batch {
         int cur = 0, bcur = 0;
     };
     Segment gexec[2][2] = {};  // [cur][bcur] at the segment's start
+    // captured segments of the sorted state: collision dispatches on stream2
+    // beside the next tick's move (default priority, like stream)
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_move = nullptr, ev_join = nullptr;
+    hipEvent_t ev_coll[2] = {nullptr, nullptr};
+    bool mark_move = false;  // capture: record ev_move right after the move kernel
     uint32_t watchdog_ms = 600000;
     // msim_batch_time_ticks probe: launch number probe_kernel of a tick is
     // bracketed by probe_ev[0] / probe_ev[1]; -1 = no probe
@@ -451,12 +468,13 @@ inline int bcheck(msim_batch* b, const char* what) {
 // Launch order of a batched tick, for msim_batch_time_ticks.
 // Launches of a batched tick, in order, by kernel name (template arguments
 // dropped).
-constexpr int kBatchKernelMax = 5;
-inline int batch_kernel_count(const msim_batch* b) { return b->sorted ? 3 : (b->binning == 2 ? 4 : 5); }
+constexpr int kBatchKernelMax = 5;  // largest batch_kernel_count
+inline int batch_kernel_count(const msim_batch* b) { return b->sorted ? 4 : (b->binning == 2 ? 4 : 5); }
 inline const char* batch_kernel_name(const msim_batch* b, int k) {
     if (b->sorted) {
         if (k == 0) return "k_bmove_sorted";
-        return k == 1 ? (b->collide_mode == 1 ? "k_bcollide" : "k_bcollide_staged") : "k_bfinish";
+        if (k == 1) return "k_broute_sorted";
+        return k == 2 ? (b->collide_mode == 1 ? "k_bcollide" : "k_bcollide_staged") : "k_bfinish";
     }
     if (k == 0) return "k_advance4";
     if (b->binning == 2) {

@@ -30,13 +30,22 @@ __device__ __forceinline__ uint32_t xs128_next(uint4& s) {
     return s.x;
 }
 
+// The value of x, opaque to the optimiser: no operation is folded across
+// it, so a product passed through it stays a separately rounded product in
+// every build.  With -ffp-contract=fast the backend fuses a*b+c into an fma
+// whatever a scoped "fp contract(off)" says; arithmetic that decides integer
+// state therefore passes its products through here.
+__device__ __forceinline__ float opaque(float x) {
+    asm("" : "+v"(x));
+    return x;
+}
+
 // random_move.comp:738-746: offset in {lo..hi}; the product and the sum are
 // two separate roundings (no fused multiply-add, in every build: the result
 // decides integer state).
 __device__ __forceinline__ uint32_t xs128_range(uint4& s, uint32_t lo, uint32_t hi) {
-#pragma clang fp contract(off)
     const float f = static_cast<float>(xs128_next(s)) * 2.3283064365386963e-10f;
-    const float prod = f * static_cast<float>(hi - lo + 1u);
+    const float prod = opaque(f * static_cast<float>(hi - lo + 1u));
     const float sum = static_cast<float>(lo) + prod;
     return static_cast<uint32_t>(ceilf(sum)) - 1u;
 }
@@ -44,9 +53,8 @@ __device__ __forceinline__ uint32_t xs128_range(uint4& s, uint32_t lo, uint32_t 
 // GLSL length(v) = sqrt(x*x + y*y), one rounding per operation in every
 // build (the move compares it with the speed: an integer-deciding branch).
 __device__ __forceinline__ float vlen(float x, float y) {
-#pragma clang fp contract(off)
-    const float xx = x * x;
-    const float yy = y * y;
+    const float xx = opaque(x * x);
+    const float yy = opaque(y * y);
     return sqrtf(xx + yy);
 }
 
@@ -132,10 +140,10 @@ __device__ __forceinline__ uint32_t bucket_of(int32_t cx, int32_t cy, uint32_t l
 // tests are implied: |dx| > R gives fl(dx*dx) >= fl(R*R), so d2 >= fl(R*R),
 // and the host checks fl(R*R) >= d2_limit when it picks d2_limit.
 __device__ __forceinline__ bool in_range(float2 pj, float2 pi, float d2_limit) {
-#pragma clang fp contract(off)  // decides neighbour counts: one rounding per operation in every build
+    // decides neighbour counts: one rounding per operation in every build
     const float ddx = pj.x - pi.x;
     const float ddy = pj.y - pi.y;
-    const float d2 = ddx * ddx + ddy * ddy;
+    const float d2 = opaque(ddx * ddx) + opaque(ddy * ddy);
     return d2 < d2_limit;
 }
 

@@ -74,6 +74,10 @@ struct msim_sim {
     uint32_t* cold_road = nullptr;          // [n] caller order
     uint4* cold_rng = nullptr;              // [n] caller order
     uint32_t* ncount_caller = nullptr;      // [n] neighbour counts kept in caller order
+    uint32_t* arr_slot = nullptr;           // [n] arrivals' new slots, listed per segment
+    uint32_t* seg_base = nullptr;           // [n_seg] first slot of each segment
+    uint32_t* seg_arr = nullptr;            // [n_seg] arrivals of each segment
+    bool mark_move = false;  // capture: record ev_move right after the move kernel
     int ncount_at = 0;  // buffer whose slot order ncount follows (-1: ncount_caller)
     bool route_inline = false;  // arrivals routed inside k_advance instead of k_route
     bool capturing = false;  // inside build_graph: renumberings skip the output-only arrays
@@ -96,7 +100,14 @@ struct msim_sim {
     uint32_t q_blocks = 1;  // k_route workgroups per queue
     int qpar = 0;
     Counters* ctr = nullptr;
-    unsigned long long* part_stats = nullptr;  // [kSlots * kSlotStride] advance statistics
+    unsigned long long* part_stats = nullptr;  // [2][kSlots * kSlotStride] advance statistics
+    int stats_par = 0;  // half of part_stats the last move dispatch added to (sorted state: alternates)
+    // Captured segments of the sorted state run each tick's collision dispatch
+    // on stream2, beside the next tick's move on stream (both only read the
+    // buffers the move wrote); default priority, like the handle's stream.
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_move = nullptr, ev_join = nullptr;
+    hipEvent_t ev_coll[2] = {nullptr, nullptr};
     unsigned long long* part_pairs = nullptr;  // [kSlots * kSlotStride] neighbour sums
     uint32_t* hist = nullptr;
     uint32_t hist_cap = 0;
@@ -138,19 +149,21 @@ struct msim_sim {
 };
 
 namespace {
+unsigned long long* stats_buf(const msim_sim* s) { return s->part_stats + s->stats_par * (kSlots * kSlotStride); }
 const char* const kSegKernels[] = {"k_bin_count", "k_bin_place"};
 const char* const kAtomicKernels[] = {"k_scan_reduce", "k_scan_partials", "k_scan_down", "k_scatter"};
 const char* const kFusedKernels[] = {"k_bin_fused"};
 uint32_t bin_kernels(const msim_sim* s) { return s->binning == 1 ? 4u : (s->binning == 2 ? 1u : 2u); }
 uint32_t kernels_per_tick(const msim_sim* s) {
-    return s->binning == 3 ? 3u : (s->route_inline ? 1u : 2u) + bin_kernels(s) + 2u;
+    return s->binning == 3 ? 4u : (s->route_inline ? 1u : 2u) + bin_kernels(s) + 2u;
 }
 // Name of launch k of a fused tick (the kernel's name, template arguments
 // dropped).
 const char* tick_kernel_name(const msim_sim* s, uint32_t k) {
     if (s->binning == 3) {
         if (k == 0) return "k_move_sorted";
-        return k == 1 ? (s->collide_mode == 1 ? "k_collide" : "k_collide_staged") : "k_collide_finish";
+        if (k == 1) return "k_route_sorted";
+        return k == 2 ? (s->collide_mode == 1 ? "k_collide" : "k_collide_staged") : "k_collide_finish";
     }
     if (k == 0) return (s->route_inline && s->binning != 1) ? "k_advance4" : "k_advance";
     if (!s->route_inline) {
@@ -393,7 +406,7 @@ int enqueue_move_fast(msim_sim* s, bool write_dir, hipEvent_t* ev) {
 // Folds advance statistics that no collision dispatch has folded yet.
 void flush_stats(msim_sim* s) {
     if (s->stats_pending) {
-        hipLaunchKernelGGL(k_fold_stats, dim3(1), dim3(64), 0, s->stream, s->part_stats, s->ctr);
+        hipLaunchKernelGGL(k_fold_stats, dim3(1), dim3(64), 0, s->stream, stats_buf(s), s->ctr);
         s->stats_pending = false;
     }
 }
@@ -465,7 +478,7 @@ int enqueue_collide(msim_sim* s, bool slow, hipEvent_t* ev = nullptr) {
         probe_post(s);
         probe_pre(s);
         hipLaunchKernelGGL(k_collide_finish<false>, dim3(1), dim3(64), 0, s->stream, s->part_pairs, s->ctr,
-                           s->hist, s->hist_cap - 1, s->part_stats, fold);
+                           s->hist, s->hist_cap - 1, stats_buf(s), fold);
         probe_post(s);
         s->ncount_at = s->cur;
         return check_launch(s, "collision dispatch");
@@ -477,7 +490,7 @@ int enqueue_collide(msim_sim* s, bool slow, hipEvent_t* ev = nullptr) {
         probe_post(s);
         probe_pre(s);
         hipLaunchKernelGGL(k_collide_finish<true>, dim3(1), dim3(64), 0, s->stream, s->part_pairs, s->ctr,
-                           s->hist, s->hist_cap - 1, s->part_stats, fold);
+                           s->hist, s->hist_cap - 1, stats_buf(s), fold);
         hipLaunchKernelGGL(k_set_init, g, dim3(kBlock), 0, s->stream, a.init, s->n);
     } else {
         probe_pre(s);
@@ -496,7 +509,7 @@ int enqueue_collide(msim_sim* s, bool slow, hipEvent_t* ev = nullptr) {
         probe_post(s);
         probe_pre(s);
         hipLaunchKernelGGL(k_collide_finish<false>, dim3(1), dim3(64), 0, s->stream, s->part_pairs, s->ctr,
-                           s->hist, s->hist_cap - 1, s->part_stats, fold);
+                           s->hist, s->hist_cap - 1, stats_buf(s), fold);
     }
     probe_post(s);
     return check_launch(s, "collision dispatch");
@@ -535,9 +548,6 @@ SortedBufs sorted_bufs(const msim_sim* s, int k) {
 
 constexpr uint32_t kSortedThreads = 512;
 
-size_t sorted_lds_bytes(uint32_t seg_log, uint32_t unroll) {
-    return (sizeof(uint32_t) << seg_log) + sizeof(uint2) * kSortedThreads * unroll;
-}
 
 // Enters the sorted state from the classic layout: renumbers the agents into
 // the current binning's order (slot map = identity), moves road and PRNG
@@ -572,8 +582,14 @@ int ensure_sorted(msim_sim* s) {
 
 template <uint32_t U, bool UN, bool W>
 void launch_sorted(msim_sim* s, const SortedMove& mv, const BinDev& b) {
+    probe_pre(s);
     hipLaunchKernelGGL((k_move_sorted<U, kSortedThreads, UN, W>), dim3(s->n_seg), dim3(kSortedThreads),
-                       sorted_lds_bytes(s->seg_log, U), s->stream, mv, b);
+                       sorted_lds_bytes(s->seg_log), s->stream, mv, b);
+    probe_post(s);
+    if (s->mark_move) (void)hipEventRecord(s->ev_move, s->stream);
+    probe_pre(s);
+    hipLaunchKernelGGL((k_route_sorted<UN, W>), dim3(s->n_seg), dim3(kBlock), 0, s->stream, mv, b);
+    probe_post(s);
 }
 
 template <uint32_t U>
@@ -605,8 +621,12 @@ int enqueue_move_sorted(msim_sim* s, bool write_dir) {
     mv.roads = s->roads;
     mv.conns = s->conns;
     mv.spd_const = s->spd_const;
-    mv.part_stats = s->part_stats;
+    s->stats_par ^= 1;  // the next tick's collision dispatch folds this half
+    mv.part_stats = stats_buf(s);
     mv.faults = &s->ctr->faults;
+    mv.arr_slot = s->arr_slot;
+    mv.seg_base = s->seg_base;
+    mv.seg_arr = s->seg_arr;
     BinDev b{};
     b.old_start = s->bstart[s->bcur];
     b.new_start = s->bstart[1 - s->bcur];
@@ -623,11 +643,9 @@ int enqueue_move_sorted(msim_sim* s, bool write_dir) {
     b.margin = s->margin;
     b.n = s->n;
     s->launch_no = 0;
-    probe_pre(s);
     if (s->bin_unroll == 2) launch_sorted_u<2>(s, mv, b, write_dir);
     else if (s->bin_unroll == 8) launch_sorted_u<8>(s, mv, b, write_dir);
     else launch_sorted_u<4>(s, mv, b, write_dir);
-    probe_post(s);
     s->cur = nb;
     s->bcur = 1 - s->bcur;
     s->stats_pending = true;  // folded by the next collision dispatch's finish pass
@@ -679,21 +697,57 @@ int wait_stream(msim_sim* s) {
 
 int after_move(msim_sim* s);
 
+// The captured ticks of the sorted state, two streams: tick t's collision
+// dispatch (stream2) only reads positions of buffer q_t, which neither tick
+// t's route choice (targets, next buckets, road and PRNG state) nor tick
+// t + 1's move (reads q_t) write, so they run side by side; tick t + 2's
+// move writes
+// buffer q_t and waits for tick t's collision dispatch and finish.  The
+// finish of tick t folds the statistics half move t wrote (the halves
+// alternate), so it never races move t + 1.  Collision dispatches stay in
+// order on stream2 (pair slots, history).  Ends with stream2 joined.
+int capture_sorted_ticks(msim_sim* s) {
+    MSIM_TRY_HIP(s, hipEventRecord(s->ev_fork, s->stream));
+    MSIM_TRY_HIP(s, hipStreamWaitEvent(s->stream2, s->ev_fork, 0));
+    int rc = MSIM_OK;
+    for (uint32_t t = 0; t < s->seg_ticks && rc == MSIM_OK; t++) {
+        if (t >= 2) MSIM_TRY_HIP(s, hipStreamWaitEvent(s->stream, s->ev_coll[t & 1], 0));
+        s->mark_move = true;  // ev_move right after the move kernel: the routing runs beside the collision
+        rc = enqueue_move(s, kMoveAll, false);
+        s->mark_move = false;
+        if (rc != MSIM_OK) break;
+        MSIM_TRY_HIP(s, hipStreamWaitEvent(s->stream2, s->ev_move, 0));
+        std::swap(s->stream, s->stream2);  // the collision dispatch's launches go to stream2
+        rc = enqueue_collide(s, false);
+        std::swap(s->stream, s->stream2);
+        if (rc != MSIM_OK) break;
+        MSIM_TRY_HIP(s, hipEventRecord(s->ev_coll[t & 1], s->stream2));
+        rc = after_move(s);
+    }
+    MSIM_TRY_HIP(s, hipEventRecord(s->ev_join, s->stream2));
+    MSIM_TRY_HIP(s, hipStreamWaitEvent(s->stream, s->ev_join, 0));
+    return rc;
+}
+
 // Captures seg_ticks fused ticks starting from the current (cur, bcur, qpar)
 // buffers, right after a reorder: the segment is whole reorder periods, each
 // ending with its reorder, so every replay starts on freshly ordered agents.
 int build_graph(msim_sim* s) {
     hipGraph_t graph = nullptr;
-    const int cur0 = s->cur, bcur0 = s->bcur, qpar0 = s->qpar, ncount_at0 = s->ncount_at;
+    const int cur0 = s->cur, bcur0 = s->bcur, qpar0 = s->qpar, ncount_at0 = s->ncount_at, spar0 = s->stats_par;
     const uint32_t since0 = s->since_reorder;
     const uint64_t ticks0 = s->ticks_simulated;
     MSIM_TRY_HIP(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
     int rc = MSIM_OK;
     s->capturing = true;  // run segments are always followed by the run's last tick
-    for (uint32_t t = 0; t < s->seg_ticks && rc == MSIM_OK; t++) {
-        rc = enqueue_move(s, kMoveAll, false);
-        if (rc == MSIM_OK) rc = enqueue_collide(s, false);
-        if (rc == MSIM_OK) rc = after_move(s);
+    if (s->sorted) {
+        rc = capture_sorted_ticks(s);
+    } else {
+        for (uint32_t t = 0; t < s->seg_ticks && rc == MSIM_OK; t++) {
+            rc = enqueue_move(s, kMoveAll, false);
+            if (rc == MSIM_OK) rc = enqueue_collide(s, false);
+            if (rc == MSIM_OK) rc = after_move(s);
+        }
     }
     s->capturing = false;
     hipError_t e = hipStreamEndCapture(s->stream, &graph);
@@ -705,6 +759,7 @@ int build_graph(msim_sim* s) {
     s->bcur = bcur0;
     s->qpar = qpar0;
     s->ncount_at = ncount_at0;
+    s->stats_par = spar0;
     s->since_reorder = since0;
     s->ticks_simulated = ticks0;
     if (rc != MSIM_OK) {
@@ -888,13 +943,23 @@ extern "C" int msim_create(const msim_road* roads, uint32_t n_roads, const uint3
         delete s;
         return fail(nullptr, MSIM_ERR_HIP, "hipStreamCreate failed");
     }
+    {
+        bool ok = hipStreamCreateWithFlags(&s->stream2, hipStreamNonBlocking) == hipSuccess;
+        for (hipEvent_t* e : {&s->ev_fork, &s->ev_move, &s->ev_join, &s->ev_coll[0], &s->ev_coll[1]}) {
+            ok = ok && hipEventCreateWithFlags(e, hipEventDisableTiming) == hipSuccess;
+        }
+        if (!ok) {
+            msim_destroy(s);
+            return fail(nullptr, MSIM_ERR_HIP, "stream / event creation failed");
+        }
+    }
 
     // LDS per workgroup bounds the segment size (4 bytes per bucket counter).
     int lds_max = 65536;
     (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, s->device);
     const uint32_t sorted_unroll = o.bin_unroll ? o.bin_unroll : 4u;
     auto lds_need = [&](uint32_t l) {
-        return (o.binning == 3 ? sorted_lds_bytes(l, sorted_unroll) : (static_cast<size_t>(4) << l)) + 1024;
+        return (o.binning == 3 ? sorted_lds_bytes(l) : (static_cast<size_t>(4) << l)) + 1024;
     };
     uint32_t seg_use = seg_log;
     while (seg_use > 9 && lds_need(seg_use) > static_cast<size_t>(lds_max)) seg_use--;
@@ -963,6 +1028,9 @@ extern "C" int msim_create(const msim_road* roads, uint32_t n_roads, const uint3
         ALLOC(s->cold_road, n);
         ALLOC(s->cold_rng, n);
         ALLOC(s->ncount_caller, n);
+        ALLOC(s->arr_slot, n);
+        ALLOC(s->seg_base, s->n_seg);
+        ALLOC(s->seg_arr, s->n_seg);
     }
     ALLOC(s->seg_total, s->n_seg);
     ALLOC(s->bin_ticket, 1);
@@ -972,7 +1040,7 @@ extern "C" int msim_create(const msim_road* roads, uint32_t n_roads, const uint3
     ALLOC(s->q_items, static_cast<size_t>(kQueues) * s->q_cap);
     ALLOC(s->q_count, 2 * kQueues * kQueueStride);
     ALLOC(s->ctr, 1);
-    ALLOC(s->part_stats, kSlots * kSlotStride);
+    ALLOC(s->part_stats, 2 * kSlots * kSlotStride);
     ALLOC(s->part_pairs, kSlots * kSlotStride);
     ALLOC(s->hist, s->hist_cap);
     ALLOC(s->color0, n);
@@ -1003,7 +1071,7 @@ extern "C" int msim_create(const msim_road* roads, uint32_t n_roads, const uint3
         if (e == hipSuccess) e = hipMemset(s->bin_ticket, 0, sizeof(unsigned long long));
         if (e == hipSuccess) e = hipMemset(s->seg_status, 0, sizeof(unsigned long long) * s->n_seg);
         if (e == hipSuccess) e = hipMemset(s->q_count, 0, sizeof(uint32_t) * 2 * kQueues * kQueueStride);
-        if (e == hipSuccess) e = hipMemset(s->part_stats, 0, sizeof(unsigned long long) * kSlots * kSlotStride);
+        if (e == hipSuccess) e = hipMemset(s->part_stats, 0, sizeof(unsigned long long) * 2 * kSlots * kSlotStride);
         if (e == hipSuccess) e = hipMemset(s->part_pairs, 0, sizeof(unsigned long long) * kSlots * kSlotStride);
         // every binned id slot always holds a valid agent index
         for (int k = 0; k < 2 && e == hipSuccess; k++) e = hipMemset(s->bbid[k], 0, sizeof(uint32_t) * n);
@@ -1071,7 +1139,10 @@ extern "C" int msim_run_ticks(msim_sim* s, uint32_t n_ticks) {
             s->cur = g.cur;
             s->bcur = g.bcur;
             s->qpar = g.qpar;
-            if (s->sorted) s->ncount_at = s->cur;  // every captured tick ends with its collision dispatch
+            if (s->sorted) {
+                s->ncount_at = s->cur;  // every captured tick ends with its collision dispatch
+                s->stats_par ^= static_cast<int>(s->seg_ticks & 1u);  // the halves the segment's moves used
+            }
             s->pc.tick += 2 * s->seg_ticks;
             s->ticks_simulated += s->seg_ticks;
             s->any_collide = true;
@@ -1362,6 +1433,10 @@ extern "C" int msim_destroy(msim_sim* s) {
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     drop_graphs(s);
     for (void* p : s->allocs) (void)hipFree(p);
+    for (hipEvent_t e : {s->ev_fork, s->ev_move, s->ev_join, s->ev_coll[0], s->ev_coll[1]}) {
+        if (e) (void)hipEventDestroy(e);
+    }
+    if (s->stream2) (void)hipStreamDestroy(s->stream2);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
     return MSIM_OK;

@@ -19,10 +19,14 @@
 //     slot;
 //   * place pass: the window slots whose `nxt` lies in the segment load
 //     their state (coalesced: the window is a contiguous slot range), move,
-//     take arrivals through the route choice (random_move.comp:778-828;
-//     road and PRNG state live in caller order and are touched only here),
 //     and are stored at their new slots with the bucket of their following
-//     move as the new `nxt`.
+//     move as the new `nxt`;
+//   * arrivals are stored at once but for what the route choice decides
+//     (target, next bucket, heading): their new slots are listed in the
+//     segment's own slice [base, base + total) of the arrival list, and
+//     k_route_sorted, launched next, routes every segment's list
+//     (random_move.comp:778-828; road and PRNG state live in caller order and
+//     are touched only there).
 //
 // Each agent's arithmetic is the plain tick's exactly; only where the state
 // is stored changes.  The agents' order inside a bucket follows the place
@@ -74,6 +78,9 @@ struct SortedBufs {
 
 struct SortedMove {
     SortedBufs src, dst;
+    uint32_t* arr_slot;  // [n] arrivals' new slots, segment g's at [seg_base[g], + seg_arr[g])
+    uint32_t* seg_base;  // [n_seg] first slot of each segment
+    uint32_t* seg_arr;   // [n_seg] arrivals of each segment
     uint32_t* road;  // [n] caller order
     uint4* rng;      // [n] caller order
     const msim_road* roads;
@@ -105,161 +112,148 @@ __device__ __forceinline__ void sorted_count(const uint32_t* __restrict__ nxt, u
     }
 }
 
-// Routes the arrivals listed in this wave's LDS list (entries: new slot,
-// old slot): route choice (random_move.comp:778-828) on the caller-order
-// road and PRNG state, then the new target, the bucket of the next move and
-// (last tick) the heading.  The wave's lanes take the entries together.
+// Moves the agent of old slot k to new slot q: loads its hot state, steps
+// it (random_move.comp:841-852) and stores it; for an arrival, what the route
+// choice decides (target, next bucket, heading) is left to k_route_sorted.
+// Returns whether the agent arrived; *ins counts a cell change.
 template <bool UNIFORM, bool WRITE_DIR>
-__device__ __forceinline__ void sorted_route_list(const SortedMove& mv, const TableView& t, const uint2* alist,
-                                                  uint32_t na) {
+__device__ __forceinline__ bool move_agent(const SortedMove& mv, const TableView& t, uint32_t k, uint32_t q,
+                                           uint32_t& n_ins) {
     const SortedBufs& src = mv.src;
     const SortedBufs& dst = mv.dst;
-    wave_lds_sync();
-    for (uint32_t j = __lane_id(); j < na; j += 64) {
-        const uint2 e = alist[j];
-        const float2 at = src.tgt[e.y];  // the old target, where the agent now stands
-        const uint32_t hh = src.perm[e.y];
-        const float s = UNIFORM ? mv.spd_const : src.spd[e.y];
-        float2 nt = at;
-        uint32_t r = mv.road[hh];
-        uint4 st = mv.rng[hh];
-        choose_route(nt, r, st, mv.roads, mv.conns);
-        mv.road[hh] = r;
-        mv.rng[hh] = st;
-        dst.tgt[e.x] = nt;
-        dst.nxt[e.x] = bucket_of_pos(next_position(at, nt, s), t);
-        if (WRITE_DIR) dst.dir[e.x] = heading(nt, at, s);
+    const uint32_t b = src.nxt[k];
+    const float2 p = src.pos[k];
+    const float2 tg = src.tgt[k];
+    const float sp = UNIFORM ? mv.spd_const : src.spd[k];
+    const uint32_t h = src.perm[k];
+    const float dx = tg.x - p.x;
+    const float dy = tg.y - p.y;
+    const float len = vlen(dx, dy);
+    float2 np = tg, d = make_float2(0.0f, 0.0f);
+    const bool arr = !(len > sp);
+    if (!arr) {
+        const float4 w = walk_step(p.x, p.y, dx, dy, len, sp);
+        np = make_float2(w.x, w.y);
+        d = make_float2(w.z, w.w);
     }
-    wave_lds_sync();
+    const int32_t cx = cell_coord(np.x, t.inv_cell), cy = cell_coord(np.y, t.inv_cell);
+    if (bucket_of(cx, cy, t.lw, t.wmask, t.hmask) != b) {
+        atomicOr(mv.faults, kFaultNext);
+    }
+    n_ins += (cx != cell_coord(p.x, t.inv_cell) || cy != cell_coord(p.y, t.inv_cell)) ? 1u : 0u;
+    dst.pos[q] = np;
+    if (!UNIFORM) dst.spd[q] = sp;
+    dst.perm[q] = h;
+    if (!arr) {
+        dst.tgt[q] = tg;
+        dst.nxt[q] = bucket_of_pos(next_position(np, tg, sp), t);
+        if (WRITE_DIR) dst.dir[q] = d;
+    }
+    return arr;
 }
 
-// Place pass over window slots [lo, hi): the segment's agents move and are
-// stored at cursor[new bucket - s0]++.  Only slots whose next bucket lies in
-// the segment load their state (the margins of the window mostly hold agents
-// of the neighbouring segments, and a lane that loads nothing costs no
-// traffic); the next round's buckets are loaded before this round's state, so
-// each round waits for one memory round trip.  Arrivals are stored at once
-// except for what the route choice decides (target, next bucket, heading);
-// those are listed in the wave's LDS list and routed when the list could
-// overflow and at the end of the segment.  Waves run the rounds
-// independently (the rank helper needs whole waves, no workgroup barrier).
+// The segment's window: old slots [a0, a1) then [c0, c1), numbered from 0.
+struct Window {
+    uint32_t a0, a1, c0, c1;
+    __device__ uint32_t slot(uint32_t r) const { return r < a1 - a0 ? a0 + r : c0 + (r - (a1 - a0)); }
+    __device__ uint32_t size() const { return (a1 - a0) + (c1 - c0); }
+};
+
+// Rank pass over window slots [lo, hi) (window-relative index rel0 for lo):
+// each agent whose next bucket lies in the segment takes cursor[bucket]++,
+// its new slot.  The segment's output position j = slot - base records the
+// agent's window index in map[j] when j < cap; past cap (a segment more
+// crowded than the map holds) the agent is moved and stored at once.  Reads
+// only the next buckets; the next round's are loaded before this round is
+// ranked.  Waves run the rounds independently (the rank helper needs whole
+// waves, no workgroup barrier).
 template <uint32_t U, uint32_t T, bool UNIFORM, bool WRITE_DIR>
-__device__ __forceinline__ void sorted_place(const SortedMove& mv, const TableView& t, uint32_t s0, uint32_t lo,
-                                             uint32_t hi, uint32_t* cursor, uint2* alist, uint32_t& na,
-                                             uint32_t& n_ins, uint32_t& n_arr) {
+__device__ __forceinline__ void sorted_rank(const SortedMove& mv, const TableView& t, uint32_t s0, uint32_t lo,
+                                            uint32_t hi, uint32_t rel0, uint32_t* cursor, uint16_t* map,
+                                            uint32_t base, uint32_t cap, uint32_t* arr_count, uint32_t arr_base,
+                                            uint32_t& n_ins) {
     if (lo >= hi) {
         return;
     }
     const uint32_t S = 1u << t.seg_log;
-    const unsigned long long lt = (1ull << __lane_id()) - 1ull;
-    const SortedBufs& src = mv.src;
-    const SortedBufs& dst = mv.dst;
     uint32_t bn[U];
 #pragma unroll
     for (uint32_t u = 0; u < U; u++) {
-        bn[u] = src.nxt[min(lo + u * T + threadIdx.x, hi - 1u)];
+        bn[u] = mv.src.nxt[min(lo + u * T + threadIdx.x, hi - 1u)];
     }
-    for (uint32_t base = lo; base < hi; base += U * T) {
+    for (uint32_t r0 = lo; r0 < hi; r0 += U * T) {
         uint32_t b[U];
-        bool mine[U];
-        bool any = false;
 #pragma unroll
         for (uint32_t u = 0; u < U; u++) {
             b[u] = bn[u];
-            mine[u] = base + u * T + threadIdx.x < hi && b[u] - s0 < S;
-            any = any || mine[u];
         }
-        if (base + U * T < hi) {
+        if (r0 + U * T < hi) {
 #pragma unroll
             for (uint32_t u = 0; u < U; u++) {
-                bn[u] = src.nxt[min(base + U * T + u * T + threadIdx.x, hi - 1u)];
+                bn[u] = mv.src.nxt[min(r0 + U * T + u * T + threadIdx.x, hi - 1u)];
             }
         }
-        if (__ballot(any) == 0ull) {  // wave-uniform: none of the wave's slots is the segment's
-            continue;
-        }
-        uint32_t h[U];
-        float2 p[U], tg[U];
-        float sp[U];
 #pragma unroll
         for (uint32_t u = 0; u < U; u++) {
-            const uint32_t k = base + u * T + threadIdx.x;
-            p[u] = make_float2(0.0f, 0.0f);
-            tg[u] = make_float2(1.0f, 1.0f);
-            sp[u] = mv.spd_const;
-            h[u] = 0u;
-            if (mine[u]) {
-                p[u] = src.pos[k];
-                tg[u] = src.tgt[k];
-                if (!UNIFORM) sp[u] = src.spd[k];
-                h[u] = src.perm[k];
-            }
-        }
-        bool arr[U];
-        float2 np[U], d[U];
-#pragma unroll
-        for (uint32_t u = 0; u < U; u++) {
-            const float dx = tg[u].x - p[u].x;
-            const float dy = tg[u].y - p[u].y;
-            const float len = vlen(dx, dy);
-            if (len > sp[u]) {
-                const float4 w = walk_step(p[u].x, p[u].y, dx, dy, len, sp[u]);
-                np[u] = make_float2(w.x, w.y);
-                d[u] = make_float2(w.z, w.w);
-                arr[u] = false;
-            } else {
-                np[u] = tg[u];
-                d[u] = make_float2(0.0f, 0.0f);
-                arr[u] = mine[u];
-            }
-        }
-        uint32_t slot[U];
-#pragma unroll
-        for (uint32_t u = 0; u < U; u++) {
-            slot[u] = wave_rank_add(cursor, b[u] - s0, mine[u]);
-        }
-        uint32_t round_arr = 0;
-#pragma unroll
-        for (uint32_t u = 0; u < U; u++) {
-            round_arr += static_cast<uint32_t>(__popcll(__ballot(arr[u])));
-        }
-        if (na + round_arr > 64u * U) {  // wave-uniform: route the list before it overflows
-            sorted_route_list<UNIFORM, WRITE_DIR>(mv, t, alist, na);
-            n_arr += __lane_id() == 0 ? na : 0u;
-            na = 0;
-        }
-#pragma unroll
-        for (uint32_t u = 0; u < U; u++) {
-            if (mine[u]) {
-                const int32_t cx = cell_coord(np[u].x, t.inv_cell), cy = cell_coord(np[u].y, t.inv_cell);
-                if (bucket_of(cx, cy, t.lw, t.wmask, t.hmask) != b[u]) {
-                    atomicOr(mv.faults, kFaultNext);
-                }
-                n_ins += (cx != cell_coord(p[u].x, t.inv_cell) || cy != cell_coord(p[u].y, t.inv_cell)) ? 1u : 0u;
-                const uint32_t q = slot[u];
-                dst.pos[q] = np[u];
-                if (!UNIFORM) dst.spd[q] = sp[u];
-                dst.perm[q] = h[u];
-                if (!arr[u]) {
-                    dst.tgt[q] = tg[u];
-                    dst.nxt[q] = bucket_of_pos(next_position(np[u], tg[u], sp[u]), t);
-                    if (WRITE_DIR) dst.dir[q] = d[u];
+            const uint32_t k = r0 + u * T + threadIdx.x;
+            const bool mine = k < hi && b[u] - s0 < S;
+            const uint32_t j = wave_rank_add(cursor, b[u] - s0, mine) - base;
+            if (mine) {
+                if (j < cap) {
+                    map[j] = static_cast<uint16_t>(rel0 + (k - lo));
+                } else if (move_agent<UNIFORM, WRITE_DIR>(mv, t, k, base + j, n_ins)) {
+                    mv.arr_slot[arr_base + atomicAdd(arr_count, 1u)] = base + j;
                 }
             }
-            const unsigned long long m = __ballot(arr[u]);
-            if (arr[u]) {
-                alist[na + static_cast<uint32_t>(__popcll(m & lt))] = make_uint2(slot[u], base + u * T + threadIdx.x);
-            }
-            na += static_cast<uint32_t>(__popcll(m));
         }
     }
 }
 
-// LDS the sorted move needs besides the segment's counters.
-template <uint32_t U, uint32_t T>
-constexpr size_t sorted_list_bytes() {
-    return sizeof(uint2) * (T / 64) * 64 * U;
+// Store pass over the segment's first min(total, cap) new slots, in slot
+// order: each agent's state is gathered from its old slot (the window index
+// in map), moved and stored, so every wave writes consecutive slots and each
+// output line is written whole.  Arrivals are listed for k_route_sorted (one
+// LDS atomic per wave and round).
+template <uint32_t U, uint32_t T, bool UNIFORM, bool WRITE_DIR>
+__device__ __forceinline__ void sorted_store(const SortedMove& mv, const TableView& t, const Window& win,
+                                             const uint16_t* map, uint32_t base, uint32_t count,
+                                             uint32_t* arr_count, uint32_t arr_base, uint32_t& n_ins) {
+    const unsigned long long lt = (1ull << __lane_id()) - 1ull;
+    for (uint32_t j0 = 0; j0 < count; j0 += U * T) {
+        bool arr[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) {
+            const uint32_t j = j0 + u * T + threadIdx.x;
+            arr[u] = false;
+            if (j < count) {
+                arr[u] = move_agent<UNIFORM, WRITE_DIR>(mv, t, win.slot(map[j]), base + j, n_ins);
+            }
+        }
+        unsigned long long am[U];
+        uint32_t round_arr = 0;
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) {
+            am[u] = __ballot(arr[u]);
+            round_arr += static_cast<uint32_t>(__popcll(am[u]));
+        }
+        if (round_arr) {  // wave-uniform
+            uint32_t at = 0;
+            if (__lane_id() == 0) at = atomicAdd(arr_count, round_arr);
+            at = arr_base + static_cast<uint32_t>(__shfl(static_cast<int>(at), 0, 64));
+#pragma unroll
+            for (uint32_t u = 0; u < U; u++) {
+                if (arr[u]) {
+                    mv.arr_slot[at + static_cast<uint32_t>(__popcll(am[u] & lt))] = base + j0 + u * T + threadIdx.x;
+                }
+                at += static_cast<uint32_t>(__popcll(am[u]));
+            }
+        }
+    }
 }
+
+// LDS of the sorted move: the segment's counters (4 bytes per bucket) and the
+// new-slot -> window-index map (2 bytes per slot, 2 slots per bucket).
+__host__ __device__ constexpr size_t sorted_lds_bytes(uint32_t seg_log) { return size_t{8} << seg_log; }
 
 // Count, look back, place.  Writes the segment's absolute bucket offsets to
 // start_out[s0 + j]; *base_out / *total_out receive the segment's first slot
@@ -267,7 +261,7 @@ constexpr size_t sorted_list_bytes() {
 // to the statistics slots.
 template <uint32_t U, uint32_t T, bool UNIFORM, bool WRITE_DIR>
 __device__ __forceinline__ void sorted_segment(const SortedMove& mv, const TableView& t, uint32_t s0, uint32_t* hist,
-                                               uint2* alist_all, uint32_t* lds, uint32_t* sh_base,
+                                               uint32_t* lds, uint32_t* sh_base,
                                                uint32_t* __restrict__ start_out, unsigned long long* status,
                                                uint32_t g, uint32_t g_first, uint32_t first, uint32_t epoch,
                                                uint32_t* base_out, uint32_t* total_out) {
@@ -305,45 +299,85 @@ __device__ __forceinline__ void sorted_segment(const SortedMove& mv, const Table
         start_out[s0 + j] = v;
     }
     __syncthreads();
-    uint32_t n_ins = 0, n_arr = 0, na = 0;
-    uint2* alist = alist_all + (tid >> 6) * (64 * U);
-    sorted_place<U, T, UNIFORM, WRITE_DIR>(mv, t, s0, a0, a1, hist, alist, na, n_ins, n_arr);
-    sorted_place<U, T, UNIFORM, WRITE_DIR>(mv, t, s0, c0, c1, hist, alist, na, n_ins, n_arr);
-    if (na) {  // wave-uniform
-        sorted_route_list<UNIFORM, WRITE_DIR>(mv, t, alist, na);
-        n_arr += __lane_id() == 0 ? na : 0u;
+    uint32_t n_ins = 0;
+    if (tid == 0) {
+        sh_base[1] = 0u;  // the segment's arrivals so far
     }
+    __syncthreads();
+    const Window win{a0, a1, c0, c1};
+    uint16_t* map = reinterpret_cast<uint16_t*>(hist + S);
+    // window indices must fit the map's 16 bits; a window past that moves
+    // every agent straight from the rank pass
+    const uint32_t cap = win.size() <= 65536u ? 2u * S : 0u;
+    sorted_rank<U, T, UNIFORM, WRITE_DIR>(mv, t, s0, a0, a1, 0u, hist, map, base, cap, &sh_base[1], base, n_ins);
+    sorted_rank<U, T, UNIFORM, WRITE_DIR>(mv, t, s0, c0, c1, a1 - a0, hist, map, base, cap, &sh_base[1], base,
+                                          n_ins);
+    __syncthreads();
+    sorted_store<U, T, UNIFORM, WRITE_DIR>(mv, t, win, map, base, min(total, cap), &sh_base[1], base, n_ins);
+    __syncthreads();
     // block totals of insertions and arrivals, one slot update each
     const uint32_t si = block_sum<T>(n_ins, lds);
-    __syncthreads();
-    const uint32_t sa = block_sum<T>(n_arr, lds);
     if (tid == 0) {
+        const uint32_t sa = sh_base[1];
         slot_add(mv.part_stats, g, kWordInserts, si);
         slot_add(mv.part_stats, g, kWordArrivals, sa);
+        mv.seg_base[g] = base;
+        mv.seg_arr[g] = sa;
     }
     *base_out = base;
     *total_out = total;
 }
 
-// Sorted move of one table (grid: n_seg workgroups; dynamic LDS: the
-// segment's counters followed by the per-wave arrival lists).
+// Sorted move of one table (grid: n_seg workgroups; dynamic LDS:
+// sorted_lds_bytes(seg_log)).
 template <uint32_t U, uint32_t T, bool UNIFORM, bool WRITE_DIR>
 static __global__ __launch_bounds__(T) void k_move_sorted(SortedMove mv, BinDev b) {
     extern __shared__ uint32_t hist[];
     __shared__ uint32_t lds[T / 64];
-    __shared__ uint32_t sh[3];
+    __shared__ uint32_t sh[4];
     take_segment(b.ticket, b.n_seg, sh);
     const uint32_t seg = sh[0];
-    uint2* alist = reinterpret_cast<uint2*>(hist + (1u << b.seg_log));
     uint32_t base, total;
-    sorted_segment<U, T, UNIFORM, WRITE_DIR>(mv, table_view(b), seg << b.seg_log, hist, alist, lds, &sh[2],
-                                             b.new_start, b.status, seg, 0u, 0u, sh[1], &base, &total);
+    sorted_segment<U, T, UNIFORM, WRITE_DIR>(mv, table_view(b), seg << b.seg_log, hist, lds, &sh[2], b.new_start,
+                                             b.status, seg, 0u, 0u, sh[1], &base, &total);
     if (seg == b.n_seg - 1 && threadIdx.x == 0) {
         b.new_start[b.m] = base + total;
         if (base + total != b.n) {
             atomicOr(mv.faults, kFaultBinCount);
         }
     }
+}
+
+// Route choice of the arrivals listed by the sorted move: one workgroup per
+// segment (grid n_seg), its list at arr_slot[seg_base[g] ...].  An arrival
+// stands on its old target (dst.pos, stored by the move); the route choice
+// (random_move.comp:778-828) updates the caller-order road and PRNG state
+// and gives the new target, the bucket of the next move and (last tick of a
+// batch) the heading.
+template <bool UNIFORM, bool WRITE_DIR>
+__device__ __forceinline__ void route_segment_list(const SortedMove& mv, const TableView& t, uint32_t g) {
+    const SortedBufs& dst = mv.dst;
+    const uint32_t base = mv.seg_base[g], cnt = mv.seg_arr[g];
+    for (uint32_t j = threadIdx.x; j < cnt; j += blockDim.x) {
+        const uint32_t q = mv.arr_slot[base + j];
+        const float2 at = dst.pos[q];
+        const uint32_t hh = dst.perm[q];
+        const float s = UNIFORM ? mv.spd_const : dst.spd[q];
+        float2 nt = at;
+        uint32_t r = mv.road[hh];
+        uint4 st = mv.rng[hh];
+        choose_route(nt, r, st, mv.roads, mv.conns);
+        mv.road[hh] = r;
+        mv.rng[hh] = st;
+        dst.tgt[q] = nt;
+        dst.nxt[q] = bucket_of_pos(next_position(at, nt, s), t);
+        if (WRITE_DIR) dst.dir[q] = heading(nt, at, s);
+    }
+}
+
+template <bool UNIFORM, bool WRITE_DIR>
+static __global__ __launch_bounds__(kBlock) void k_route_sorted(SortedMove mv, BinDev b) {
+    route_segment_list<UNIFORM, WRITE_DIR>(mv, table_view(b), blockIdx.x);
 }
 
 // nxt for every slot of one table: the bucket of the agent's next position.

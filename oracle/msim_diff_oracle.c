@@ -15,6 +15,7 @@
  * decisions; orc_diff_adjoint64 is the reverse mode of that replay.
  */
 #include <math.h>
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -131,6 +132,184 @@ static int neighbour_lists(uint32_t n, const void* pts, int f64, double r, doubl
     return 0;
 }
 
+/* ---- threads ------------------------------------------------------------
+ * The binary64 replay and adjoint split their per-agent loops over
+ * g_threads POSIX threads (orc_diff_set_threads).  Every agent's value is
+ * computed by the same operations in the same order whatever the split; the
+ * gradient sums are kept per thread and added in thread order at the end,
+ * which only reassociates binary64 sums. */
+static int g_threads = 1;
+
+void orc_diff_set_threads(int n) { g_threads = n < 1 ? 1 : (n > 256 ? 256 : n); }
+
+typedef void (*range_fn)(void* ctx, uint32_t lo, uint32_t hi, int worker);
+
+typedef struct {
+    range_fn f;
+    void* ctx;
+    uint32_t lo, hi;
+    int worker;
+} par_job;
+
+static void* par_run(void* arg) {
+    par_job* j = (par_job*)arg;
+    j->f(j->ctx, j->lo, j->hi, j->worker);
+    return NULL;
+}
+
+/* f(ctx, lo, hi, worker) over [0, n) in g_threads contiguous slices. */
+static void par_for(uint32_t n, range_fn f, void* ctx) {
+    int t = g_threads;
+    if (t <= 1 || n < 2048) {
+        f(ctx, 0, n, 0);
+        return;
+    }
+    pthread_t th[256];
+    par_job jobs[256];
+    for (int k = 0; k < t; k++) {
+        jobs[k] = (par_job){f, ctx, (uint32_t)(((uint64_t)n * k) / t), (uint32_t)(((uint64_t)n * (k + 1)) / t), k};
+        pthread_create(&th[k], NULL, par_run, &jobs[k]);
+    }
+    for (int k = 0; k < t; k++) pthread_join(th[k], NULL);
+}
+
+/* ---- neighbour lists -------------------------------------------------- */
+
+/* The points binned into a toroidal table of 2^lw x 2^lh buckets (at least
+ * 4 x 4, about one bucket per point): bucket of cell (cx, cy) =
+ * (cy mod 2^lh) * 2^lw + (cx mod 2^lw); the 3 x 3 cells around any cell fall
+ * in 9 distinct buckets, so every neighbour is found once (the exact
+ * predicate removes the other cells folded onto the same buckets).  A
+ * counting sort: ids ascending within a bucket. */
+typedef struct {
+    uint32_t n;
+    const void* pts;
+    int f64;
+    double r, cell;
+    uint32_t lw, lh;
+    uint32_t* start;  /* [m + 1] */
+    uint32_t* ids;    /* [n] */
+    uint32_t* off;    /* pass 1: counts at off[i + 1]; pass 2: the CSR offsets */
+    uint32_t* idx;    /* pass 2 */
+    int fill;
+} list_ctx;
+
+static void point_cell(const list_ctx* c, uint32_t i, int64_t* cx, int64_t* cy) {
+    const float* pf = (const float*)c->pts;
+    const double* pd = (const double*)c->pts;
+    double x = c->f64 ? pd[2 * i] : (double)pf[2 * i];
+    double y = c->f64 ? pd[2 * i + 1] : (double)pf[2 * i + 1];
+    *cx = (int64_t)floor(x / c->cell);
+    *cy = (int64_t)floor(y / c->cell);
+}
+
+static uint32_t bucket_at(const list_ctx* c, int64_t cx, int64_t cy) {
+    return ((uint32_t)((uint64_t)cy & ((1u << c->lh) - 1u)) << c->lw) | (uint32_t)((uint64_t)cx & ((1u << c->lw) - 1u));
+}
+
+/* The j != i within range of point i, over the 3x3 cells, in ascending j
+ * (into out when given); returns their number. */
+static uint32_t neighbours_of(const list_ctx* c, uint32_t i, uint32_t* out) {
+    pred_fn pred = c->f64 ? pred_f64 : pred_f32;
+    int64_t cx, cy;
+    point_cell(c, i, &cx, &cy);
+    uint32_t m = 0;
+    for (int64_t dy = -1; dy <= 1; dy++) {
+        for (int64_t dx = -1; dx <= 1; dx++) {
+            uint32_t b = bucket_at(c, cx + dx, cy + dy);
+            for (uint32_t k = c->start[b]; k < c->start[b + 1]; k++) {
+                uint32_t j = c->ids[k];
+                if (j != i && pred(c->pts, i, j, c->r)) {
+                    if (out) out[m] = j;
+                    m++;
+                }
+            }
+        }
+    }
+    if (out) { /* ascending j */
+        for (uint32_t a = 1; a < m; a++) {
+            uint32_t v = out[a], q = a;
+            while (q > 0 && out[q - 1] > v) {
+                out[q] = out[q - 1];
+                q--;
+            }
+            out[q] = v;
+        }
+    }
+    return m;
+}
+
+static void list_range(void* c_, uint32_t lo, uint32_t hi, int w) {
+    (void)w;
+    list_ctx* c = (list_ctx*)c_;
+    for (uint32_t i = lo; i < hi; i++) {
+        if (c->fill) {
+            (void)neighbours_of(c, i, c->idx + c->off[i]);
+        } else {
+            c->off[i + 1] = neighbours_of(c, i, NULL);
+        }
+    }
+}
+
+/* Neighbour lists of n points (cell >= R): for each i, the j != i with
+ * pred(i, j), in ascending j.  Returns CSR (off[n+1], idx) via malloc.
+ * (Counted, then filled, each pass split over the threads.) */
+static int neighbour_lists_mt(uint32_t n, const void* pts, int f64, double r, double cell, uint32_t** off_out,
+                              uint32_t** idx_out) {
+    uint32_t lg = 4;
+    while (lg < 26 && (1u << lg) < n) lg++;
+    list_ctx c = {n, pts, f64, r, cell, lg / 2, lg - lg / 2, NULL, NULL, NULL, NULL, 0};
+    const uint32_t m = 1u << lg;
+    c.start = (uint32_t*)calloc((size_t)m + 1, sizeof(uint32_t));
+    c.ids = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
+    uint32_t* bk = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
+    uint32_t* off = (uint32_t*)calloc((size_t)n + 1, sizeof(uint32_t));
+    if (!c.start || !c.ids || !bk || !off) {
+        free(c.start);
+        free(c.ids);
+        free(bk);
+        free(off);
+        return -1;
+    }
+    for (uint32_t i = 0; i < n; i++) {
+        int64_t cx, cy;
+        point_cell(&c, i, &cx, &cy);
+        bk[i] = bucket_at(&c, cx, cy);
+        c.start[bk[i] + 1]++;
+    }
+    for (uint32_t q = 0; q < m; q++) c.start[q + 1] += c.start[q];
+    uint32_t* cur = (uint32_t*)malloc(sizeof(uint32_t) * m);
+    if (!cur) {
+        free(c.start);
+        free(c.ids);
+        free(bk);
+        free(off);
+        return -1;
+    }
+    memcpy(cur, c.start, sizeof(uint32_t) * m);
+    for (uint32_t i = 0; i < n; i++) c.ids[cur[bk[i]]++] = i;  /* ascending ids within each bucket */
+    free(cur);
+    free(bk);
+    c.off = off;
+    par_for(n, list_range, &c);
+    for (uint32_t i = 0; i < n; i++) off[i + 1] += off[i];
+    uint32_t* idx = (uint32_t*)malloc(sizeof(uint32_t) * (off[n] ? off[n] : 1));
+    if (!idx) {
+        free(c.start);
+        free(c.ids);
+        free(off);
+        return -1;
+    }
+    c.idx = idx;
+    c.fill = 1;
+    par_for(n, list_range, &c);
+    free(c.start);
+    free(c.ids);
+    *off_out = off;
+    *idx_out = idx;
+    return 0;
+}
+
 int orc_diff_record(uint32_t n, uint32_t n_scen, const uint32_t* agent_off, float* pos, float* tgt,
                     uint32_t* road, uint32_t* rng, const float* speed, const orc_road* roads, uint32_t n_roads,
                     const uint32_t* conns, uint32_t n_conns, const float* theta, float alpha, float radius,
@@ -239,6 +418,90 @@ static int carry_geom(double px, double py, double Tx, double Ty, double T2x, do
 /* carry != 0: an arrival carries its leftover time onto the next road,
  * p' = T + (1 - L/s) s2 u2 (s2: the next road's speed), instead of stopping
  * at T; final_tgt/final_road give the decisions after the last tick. */
+/* Per-agent step of the binary64 replay, tick c->t: p -> pt. */
+typedef struct {
+    uint32_t n, n_ticks, t;
+    const float* speed;
+    const double* theta;
+    const float *sched_tgt;
+    const uint32_t* sched_road;
+    const uint8_t* sched_flag;
+    const float* final_tgt;
+    const uint32_t* final_road;
+    int carry;
+    double *p, *pt;
+    uint32_t pad;
+    double alpha, radius;
+} replay_ctx;
+
+static void replay_move_range(void* c_, uint32_t lo, uint32_t hi, int w) {
+    (void)w;
+    const replay_ctx* c = (const replay_ctx*)c_;
+    const uint32_t n = c->n, t = c->t;
+    const double* theta = c->theta;
+    double* p = c->p;
+    double* pt = c->pt;
+    for (uint32_t i = lo; i < hi; i++) {
+        uint8_t f = c->sched_flag[(size_t)t * n + i];
+        double Tx = c->sched_tgt[((size_t)t * n + i) * 2], Ty = c->sched_tgt[((size_t)t * n + i) * 2 + 1];
+        if (f & 2) {
+            pt[2 * i] = p[2 * i];
+            pt[2 * i + 1] = p[2 * i + 1];
+        } else if (f & 1) {
+            pt[2 * i] = Tx;
+            pt[2 * i + 1] = Ty;
+            double T2x, T2y, L, ux, uy, u2x, u2y;
+            uint32_t r2;
+            if (c->carry) {
+                next_decision(n, c->n_ticks, t, i, c->sched_tgt, c->sched_road, c->final_tgt, c->final_road, &T2x,
+                              &T2y, &r2);
+                if (carry_geom(p[2 * i], p[2 * i + 1], Tx, Ty, T2x, T2y, &L, &ux, &uy, &u2x, &u2y)) {
+                    double v = (double)(c->speed ? c->speed[i] : ORC_SPEED);
+                    double sp = v * theta[c->sched_road[(size_t)t * n + i]], s2 = v * theta[r2];
+                    pt[2 * i] = Tx + (1.0 - L / sp) * s2 * u2x;
+                    pt[2 * i + 1] = Ty + (1.0 - L / sp) * s2 * u2y;
+                }
+            }
+        } else {
+            double s = (double)(c->speed ? c->speed[i] : ORC_SPEED) * theta[c->sched_road[(size_t)t * n + i]];
+            double dx = Tx - p[2 * i], dy = Ty - p[2 * i + 1];
+            double len = sqrt(dx * dx + dy * dy);
+            pt[2 * i] = p[2 * i] + dx / len * s;
+            pt[2 * i + 1] = p[2 * i + 1] + dy / len * s;
+        }
+    }
+}
+
+/* Avoidance push of one scenario's agents (offset o): p = pt + alpha A. */
+typedef struct {
+    const replay_ctx* c;
+    uint32_t o;
+    const uint32_t *off, *idx;
+} avoid_ctx;
+
+static void replay_avoid_range(void* a_, uint32_t lo, uint32_t hi, int w) {
+    (void)w;
+    const avoid_ctx* a = (const avoid_ctx*)a_;
+    const double* pt = a->c->pt;
+    double* p = a->c->p;
+    const double radius = a->c->radius, alpha = a->c->alpha;
+    for (uint32_t i = lo; i < hi; i++) {
+        const double* me = &pt[2 * (size_t)(a->o + i)];
+        double ax = 0.0, ay = 0.0;
+        for (uint32_t k = a->off[i]; k < a->off[i + 1]; k++) {
+            const double* q = &pt[2 * (size_t)(a->o + a->idx[k])];
+            double x = me[0] - q[0], y = me[1] - q[1];
+            double h = 1.0 - (x * x + y * y) / (radius * radius);
+            ax += h * h * h * x;
+            ay += h * h * h * y;
+        }
+        p[2 * (size_t)(a->o + i)] = me[0] + alpha * ax;
+        p[2 * (size_t)(a->o + i) + 1] = me[1] + alpha * ay;
+    }
+}
+
+/* The binary64 replay under recorded decisions; traj (nullable) receives
+ * every tick's pre-move and post-move (pre-push) positions. */
 static int replay64(uint32_t n, uint32_t n_scen, const uint32_t* agent_off, const float* pos0, const float* speed,
                     const double* theta, double alpha, double radius, uint32_t n_ticks, const float* sched_tgt,
                     const uint32_t* sched_road, const uint8_t* sched_flag, const float* final_tgt,
@@ -252,60 +515,25 @@ static int replay64(uint32_t n, uint32_t n_scen, const uint32_t* agent_off, cons
         return -1;
     }
     for (uint32_t i = 0; i < 2 * n; i++) p[i] = (double)pos0[i];
+    replay_ctx c = {n, n_ticks, 0, speed, theta, sched_tgt, sched_road, sched_flag, final_tgt, final_road, carry,
+                    p, pt, 0, alpha, radius};
     for (uint32_t t = 0; t < n_ticks; t++) {
         if (traj) memcpy(traj + (size_t)t * 4 * n, p, sizeof(double) * 2 * n);
-        for (uint32_t i = 0; i < n; i++) {
-            uint8_t f = sched_flag[(size_t)t * n + i];
-            double Tx = sched_tgt[((size_t)t * n + i) * 2], Ty = sched_tgt[((size_t)t * n + i) * 2 + 1];
-            if (f & 2) {
-                pt[2 * i] = p[2 * i];
-                pt[2 * i + 1] = p[2 * i + 1];
-            } else if (f & 1) {
-                pt[2 * i] = Tx;
-                pt[2 * i + 1] = Ty;
-                double T2x, T2y, L, ux, uy, u2x, u2y;
-                uint32_t r2;
-                if (carry) {
-                    next_decision(n, n_ticks, t, i, sched_tgt, sched_road, final_tgt, final_road, &T2x, &T2y, &r2);
-                    if (carry_geom(p[2 * i], p[2 * i + 1], Tx, Ty, T2x, T2y, &L, &ux, &uy, &u2x, &u2y)) {
-                        double v = (double)(speed ? speed[i] : ORC_SPEED);
-                        double sp = v * theta[sched_road[(size_t)t * n + i]], s2 = v * theta[r2];
-                        pt[2 * i] = Tx + (1.0 - L / sp) * s2 * u2x;
-                        pt[2 * i + 1] = Ty + (1.0 - L / sp) * s2 * u2y;
-                    }
-                }
-            } else {
-                double s = (double)(speed ? speed[i] : ORC_SPEED) * theta[sched_road[(size_t)t * n + i]];
-                double dx = Tx - p[2 * i], dy = Ty - p[2 * i + 1];
-                double len = sqrt(dx * dx + dy * dy);
-                pt[2 * i] = p[2 * i] + dx / len * s;
-                pt[2 * i + 1] = p[2 * i + 1] + dy / len * s;
-            }
-        }
+        c.t = t;
+        par_for(n, replay_move_range, &c);
         if (traj) memcpy(traj + (size_t)t * 4 * n + 2 * (size_t)n, pt, sizeof(double) * 2 * n);
         memcpy(p, pt, sizeof(double) * 2 * n);
         if (alpha != 0.0) {
             for (uint32_t sc = 0; sc < n_scen; sc++) {
                 uint32_t o = agent_off[sc], m = agent_off[sc + 1] - o;
                 uint32_t *off = NULL, *idx = NULL;
-                if (neighbour_lists(m, pt + 2 * (size_t)o, 1, radius, cell, &off, &idx) != 0) {
+                if (neighbour_lists_mt(m, pt + 2 * (size_t)o, 1, radius, cell, &off, &idx) != 0) {
                     free(p);
                     free(pt);
                     return -1;
                 }
-                for (uint32_t i = 0; i < m; i++) {
-                    const double* me = &pt[2 * (size_t)(o + i)];
-                    double ax = 0.0, ay = 0.0;
-                    for (uint32_t k = off[i]; k < off[i + 1]; k++) {
-                        const double* q = &pt[2 * (size_t)(o + idx[k])];
-                        double x = me[0] - q[0], y = me[1] - q[1];
-                        double h = 1.0 - (x * x + y * y) / (radius * radius);
-                        ax += h * h * h * x;
-                        ay += h * h * h * y;
-                    }
-                    p[2 * (size_t)(o + i)] = me[0] + alpha * ax;
-                    p[2 * (size_t)(o + i) + 1] = me[1] + alpha * ay;
-                }
+                avoid_ctx a = {&c, o, off, idx};
+                par_for(m, replay_avoid_range, &a);
                 free(off);
                 free(idx);
             }
@@ -341,6 +569,108 @@ int orc_diff_adjoint64(uint32_t n, uint32_t n_scen, const uint32_t* agent_off, c
                                  sched_road, sched_flag, NULL, NULL, 0, 0, grad_pos, grad_theta, grad_alpha);
 }
 
+/* Reverse sweep of one tick: the avoidance part (per scenario) and the move
+ * part, each split over the threads; per-worker sums ga[w], gt + w * n_roads. */
+typedef struct {
+    uint32_t n, n_ticks, t, o, n_roads;
+    const double *pre, *pt, *lam;
+    double* lt;
+    double* lam_out;
+    const uint32_t *off, *idx;
+    double alpha, inv_r2;
+    double* ga;  /* [threads] */
+    double* gt;  /* [threads][n_roads] */
+    const float* speed;
+    const double* theta;
+    const float* sched_tgt;
+    const uint32_t* sched_road;
+    const uint8_t* sched_flag;
+    const float* final_tgt;
+    const uint32_t* final_road;
+    int adj_carry;
+} adj_ctx;
+
+static void adj_avoid_range(void* c_, uint32_t lo, uint32_t hi, int w) {
+    adj_ctx* c = (adj_ctx*)c_;
+    const double *pt = c->pt, *lam = c->lam;
+    double ga = 0.0;
+    for (uint32_t i = lo; i < hi; i++) {
+        size_t gi = c->o + i;
+        double gx = 0.0, gy = 0.0, ax = 0.0, ay = 0.0;
+        for (uint32_t k = c->off[i]; k < c->off[i + 1]; k++) {
+            size_t gj = c->o + c->idx[k];
+            double x = pt[2 * gi] - pt[2 * gj], y = pt[2 * gi + 1] - pt[2 * gj + 1];
+            double h = 1.0 - (x * x + y * y) * c->inv_r2;
+            double wv = h * h * h, wp = -3.0 * h * h * c->inv_r2;
+            double dlx = lam[2 * gi] - lam[2 * gj], dly = lam[2 * gi + 1] - lam[2 * gj + 1];
+            double xd = 2.0 * wp * (x * dlx + y * dly);
+            gx += wv * dlx + xd * x;
+            gy += wv * dly + xd * y;
+            ax += wv * x;
+            ay += wv * y;
+        }
+        c->lt[2 * gi] = lam[2 * gi] + c->alpha * gx;
+        c->lt[2 * gi + 1] = lam[2 * gi + 1] + c->alpha * gy;
+        ga += lam[2 * gi] * ax + lam[2 * gi + 1] * ay;
+    }
+    c->ga[w] += ga;
+}
+
+static void adj_move_range(void* c_, uint32_t lo, uint32_t hi, int w) {
+    adj_ctx* c = (adj_ctx*)c_;
+    const uint32_t n = c->n, t = c->t;
+    const double *pre = c->pre, *lt = c->lt, *theta = c->theta;
+    double* lam = c->lam_out;
+    double* gt = c->gt + (size_t)w * c->n_roads;
+    for (uint32_t i = lo; i < hi; i++) {
+        uint8_t f = c->sched_flag[(size_t)t * n + i];
+        if (f & 1) {
+            /* frozen: the snap is constant; carry: the derivatives of
+             * p' = T + (1 - L/s) s2 u2 (L = |T - p|, s, s2 the two roads' speeds) */
+            double T2x, T2y, L, ux, uy, u2x, u2y;
+            uint32_t r2;
+            double lx = lt[2 * i], ly = lt[2 * i + 1];
+            lam[2 * i] = 0.0;
+            lam[2 * i + 1] = 0.0;
+            if (c->adj_carry) {
+                next_decision(n, c->n_ticks, t, i, c->sched_tgt, c->sched_road, c->final_tgt, c->final_road, &T2x,
+                              &T2y, &r2);
+                double Tx = c->sched_tgt[((size_t)t * n + i) * 2], Ty = c->sched_tgt[((size_t)t * n + i) * 2 + 1];
+                if (carry_geom(pre[2 * i], pre[2 * i + 1], Tx, Ty, T2x, T2y, &L, &ux, &uy, &u2x, &u2y)) {
+                    uint32_t r = c->sched_road[(size_t)t * n + i];
+                    double v = (double)(c->speed ? c->speed[i] : ORC_SPEED);
+                    double sp = v * theta[r], s2 = v * theta[r2];
+                    if (sp > 0.0) {  /* a zero speed carries nothing: the frozen snap */
+                        double g2 = u2x * lx + u2y * ly;
+                        lam[2 * i] = ux * (s2 / sp) * g2;
+                        lam[2 * i + 1] = uy * (s2 / sp) * g2;
+                        gt[r] += v * (L * s2 / (sp * sp)) * g2;
+                        gt[r2] += v * (1.0 - L / sp) * g2;
+                    }
+                }
+            }
+            continue;
+        }
+        if (f & 2) {
+            lam[2 * i] = lt[2 * i];
+            lam[2 * i + 1] = lt[2 * i + 1];
+            continue;
+        }
+        uint32_t r = c->sched_road[(size_t)t * n + i];
+        double v = (double)(c->speed ? c->speed[i] : ORC_SPEED);
+        double s = v * theta[r];
+        double dx = (double)c->sched_tgt[((size_t)t * n + i) * 2] - pre[2 * i];
+        double dy = (double)c->sched_tgt[((size_t)t * n + i) * 2 + 1] - pre[2 * i + 1];
+        double len = sqrt(dx * dx + dy * dy);
+        double ux = dx / len, uy = dy / len;
+        double ul = ux * lt[2 * i] + uy * lt[2 * i + 1];
+        double fct = s / len;
+        lam[2 * i] = lt[2 * i] - fct * (lt[2 * i] - ux * ul);
+        lam[2 * i + 1] = lt[2 * i + 1] - fct * (lt[2 * i + 1] - uy * ul);
+        gt[r] += v * ul;
+    }
+}
+
 int orc_diff_adjoint64_ex(uint32_t n, uint32_t n_scen, const uint32_t* agent_off, const float* pos0,
                           const float* speed, const double* theta, uint32_t n_roads, double alpha, double radius,
                           uint32_t n_ticks, const float* sched_tgt, const uint32_t* sched_road,
@@ -348,96 +678,46 @@ int orc_diff_adjoint64_ex(uint32_t n, uint32_t n_scen, const uint32_t* agent_off
                           int traj_carry, int adj_carry, const double* grad_pos, double* grad_theta,
                           double* grad_alpha) {
     const double cell = (double)orc_cell_size((float)radius);
-    const double inv_r2 = 1.0 / (radius * radius);
+    const int nw = g_threads;
     double* traj = (double*)malloc(sizeof(double) * 4 * (size_t)n * (n_ticks ? n_ticks : 1));
     double* fin = (double*)malloc(sizeof(double) * 2 * (n ? n : 1));
     double* lam = (double*)malloc(sizeof(double) * 2 * (n ? n : 1));
     double* lt = (double*)malloc(sizeof(double) * 2 * (n ? n : 1));
+    double* gt = (double*)calloc((size_t)nw * (n_roads ? n_roads : 1), sizeof(double));
+    double* ga = (double*)calloc((size_t)nw, sizeof(double));
     int rc = -1;
-    if (!traj || !fin || !lam || !lt) goto done;
+    if (!traj || !fin || !lam || !lt || !gt || !ga) goto done;
     if (replay64(n, n_scen, agent_off, pos0, speed, theta, alpha, radius, n_ticks, sched_tgt, sched_road,
                  sched_flag, final_tgt, final_road, traj_carry, fin, traj) != 0) {
         goto done;
     }
-    memset(grad_theta, 0, sizeof(double) * n_roads);
-    *grad_alpha = 0.0;
     memcpy(lam, grad_pos, sizeof(double) * 2 * n);
     for (uint32_t t = n_ticks; t-- > 0;) {
         const double* pre = traj + (size_t)t * 4 * n;
         const double* pt = pre + 2 * (size_t)n;
+        adj_ctx c = {n, n_ticks, t, 0, n_roads, pre, pt, lam, lt, lam, NULL, NULL, alpha, 1.0 / (radius * radius),
+                     ga, gt, speed, theta, sched_tgt, sched_road, sched_flag, final_tgt, final_road, adj_carry};
         /* avoidance: lt = lam + alpha sum J (lam_i - lam_j); dL/dalpha += lam . A */
         memcpy(lt, lam, sizeof(double) * 2 * n);
         for (uint32_t sc = 0; sc < n_scen; sc++) {
             uint32_t o = agent_off[sc], m = agent_off[sc + 1] - o;
             uint32_t *off = NULL, *idx = NULL;
-            if (neighbour_lists(m, pt + 2 * (size_t)o, 1, radius, cell, &off, &idx) != 0) goto done;
-            for (uint32_t i = 0; i < m; i++) {
-                size_t gi = o + i;
-                double gx = 0.0, gy = 0.0, ax = 0.0, ay = 0.0;
-                for (uint32_t k = off[i]; k < off[i + 1]; k++) {
-                    size_t gj = o + idx[k];
-                    double x = pt[2 * gi] - pt[2 * gj], y = pt[2 * gi + 1] - pt[2 * gj + 1];
-                    double h = 1.0 - (x * x + y * y) * inv_r2;
-                    double w = h * h * h, wp = -3.0 * h * h * inv_r2;
-                    double dlx = lam[2 * gi] - lam[2 * gj], dly = lam[2 * gi + 1] - lam[2 * gj + 1];
-                    double xd = 2.0 * wp * (x * dlx + y * dly);
-                    gx += w * dlx + xd * x;
-                    gy += w * dly + xd * y;
-                    ax += w * x;
-                    ay += w * y;
-                }
-                lt[2 * gi] = lam[2 * gi] + alpha * gx;
-                lt[2 * gi + 1] = lam[2 * gi + 1] + alpha * gy;
-                *grad_alpha += lam[2 * gi] * ax + lam[2 * gi + 1] * ay;
-            }
+            if (neighbour_lists_mt(m, pt + 2 * (size_t)o, 1, radius, cell, &off, &idx) != 0) goto done;
+            c.o = o;
+            c.off = off;
+            c.idx = idx;
+            par_for(m, adj_avoid_range, &c);
             free(off);
             free(idx);
         }
-        /* move */
-        for (uint32_t i = 0; i < n; i++) {
-            uint8_t f = sched_flag[(size_t)t * n + i];
-            if (f & 1) {
-                /* frozen: the snap is constant; carry: the derivatives of
-                 * p' = T + (1 - L/s) s2 u2 (L = |T - p|, s, s2 the two roads' speeds) */
-                double T2x, T2y, L, ux, uy, u2x, u2y;
-                uint32_t r2;
-                double lx = lt[2 * i], ly = lt[2 * i + 1];
-                lam[2 * i] = 0.0;
-                lam[2 * i + 1] = 0.0;
-                if (adj_carry) {
-                    next_decision(n, n_ticks, t, i, sched_tgt, sched_road, final_tgt, final_road, &T2x, &T2y, &r2);
-                    double Tx = sched_tgt[((size_t)t * n + i) * 2], Ty = sched_tgt[((size_t)t * n + i) * 2 + 1];
-                    if (carry_geom(pre[2 * i], pre[2 * i + 1], Tx, Ty, T2x, T2y, &L, &ux, &uy, &u2x, &u2y)) {
-                        uint32_t r = sched_road[(size_t)t * n + i];
-                        double v = (double)(speed ? speed[i] : ORC_SPEED);
-                        double sp = v * theta[r], s2 = v * theta[r2];
-                        double g2 = u2x * lx + u2y * ly;
-                        lam[2 * i] = ux * (s2 / sp) * g2;
-                        lam[2 * i + 1] = uy * (s2 / sp) * g2;
-                        grad_theta[r] += v * (L * s2 / (sp * sp)) * g2;
-                        grad_theta[r2] += v * (1.0 - L / sp) * g2;
-                    }
-                }
-                continue;
-            }
-            if (f & 2) {
-                lam[2 * i] = lt[2 * i];
-                lam[2 * i + 1] = lt[2 * i + 1];
-                continue;
-            }
-            uint32_t r = sched_road[(size_t)t * n + i];
-            double v = (double)(speed ? speed[i] : ORC_SPEED);
-            double s = v * theta[r];
-            double dx = (double)sched_tgt[((size_t)t * n + i) * 2] - pre[2 * i];
-            double dy = (double)sched_tgt[((size_t)t * n + i) * 2 + 1] - pre[2 * i + 1];
-            double len = sqrt(dx * dx + dy * dy);
-            double ux = dx / len, uy = dy / len;
-            double ul = ux * lt[2 * i] + uy * lt[2 * i + 1];
-            double fct = s / len;
-            lam[2 * i] = lt[2 * i] - fct * (lt[2 * i] - ux * ul);
-            lam[2 * i + 1] = lt[2 * i + 1] - fct * (lt[2 * i + 1] - uy * ul);
-            grad_theta[r] += v * ul;
-        }
+        /* move (writes lam from lt) */
+        par_for(n, adj_move_range, &c);
+    }
+    memset(grad_theta, 0, sizeof(double) * n_roads);
+    *grad_alpha = 0.0;
+    for (int w = 0; w < nw; w++) {
+        for (uint32_t r = 0; r < n_roads; r++) grad_theta[r] += gt[(size_t)w * n_roads + r];
+        *grad_alpha += ga[w];
     }
     rc = 0;
 done:
@@ -445,5 +725,7 @@ done:
     free(fin);
     free(lam);
     free(lt);
+    free(gt);
+    free(ga);
     return rc;
 }

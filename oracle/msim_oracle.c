@@ -344,6 +344,96 @@ int64_t orc_collide_brute(uint32_t n, const float* pos, float radius,
 
 /* ---- threaded variants (CPU baseline) ---------------------------------- */
 
+/* orc_bin on n_threads threads: each thread keys and sorts one slice, then
+ * sorted runs are merged pairwise, the merges of a round in parallel.  The
+ * order (cy, cx, id) is total, so the result equals orc_bin's. */
+typedef struct {
+    orc_binned* b;
+    uint32_t lo, hi;
+    const float* pos;
+    float cell;
+} orc_sort_job;
+
+static void* orc_sort_worker(void* arg) {
+    orc_sort_job* j = (orc_sort_job*)arg;
+    for (uint32_t i = j->lo; i < j->hi; i++) {
+        j->b[i].key = orc_key(orc_cell_coord(j->pos[2 * i], j->cell), orc_cell_coord(j->pos[2 * i + 1], j->cell));
+        j->b[i].id = i;
+    }
+    qsort(j->b + j->lo, j->hi - j->lo, sizeof(orc_binned), orc_cmp_binned);
+    return NULL;
+}
+
+typedef struct {
+    const orc_binned* src;
+    orc_binned* dst;
+    uint32_t lo, mid, hi;
+} orc_merge_job;
+
+static void* orc_merge_worker(void* arg) {
+    orc_merge_job* j = (orc_merge_job*)arg;
+    uint32_t a = j->lo, b = j->mid, o = j->lo;
+    while (a < j->mid && b < j->hi) {
+        j->dst[o++] = (orc_cmp_binned(&j->src[b], &j->src[a]) < 0) ? j->src[b++] : j->src[a++];
+    }
+    while (a < j->mid) j->dst[o++] = j->src[a++];
+    while (b < j->hi) j->dst[o++] = j->src[b++];
+    return NULL;
+}
+
+static orc_binned* orc_bin_mt(uint32_t n, const float* pos, float cell, int n_threads) {
+    if (n_threads <= 1 || n < 4096) {
+        return orc_bin(n, pos, cell);
+    }
+    orc_binned* b = (orc_binned*)malloc(sizeof(orc_binned) * n);
+    orc_binned* t = (orc_binned*)malloc(sizeof(orc_binned) * n);
+    uint32_t* cut = (uint32_t*)malloc(sizeof(uint32_t) * ((size_t)n_threads + 1));
+    pthread_t* th = (pthread_t*)calloc((size_t)n_threads, sizeof(pthread_t));
+    orc_sort_job* sj = (orc_sort_job*)calloc((size_t)n_threads, sizeof(orc_sort_job));
+    orc_merge_job* mj = (orc_merge_job*)calloc((size_t)n_threads, sizeof(orc_merge_job));
+    if (!b || !t || !cut || !th || !sj || !mj) {
+        free(b);
+        free(t);
+        free(cut);
+        free(th);
+        free(sj);
+        free(mj);
+        return NULL;
+    }
+    for (int k = 0; k <= n_threads; k++) cut[k] = (uint32_t)(((uint64_t)n * k) / n_threads);
+    for (int k = 0; k < n_threads; k++) {
+        sj[k] = (orc_sort_job){b, cut[k], cut[k + 1], pos, cell};
+        pthread_create(&th[k], NULL, orc_sort_worker, &sj[k]);
+    }
+    for (int k = 0; k < n_threads; k++) pthread_join(th[k], NULL);
+    int runs = n_threads;
+    while (runs > 1) {  /* merge runs 2i and 2i+1 (cut[] holds the run bounds) */
+        int m = 0;
+        for (int k = 0; k + 1 < runs; k += 2) {
+            mj[m] = (orc_merge_job){b, t, cut[k], cut[k + 1], cut[k + 2]};
+            pthread_create(&th[m], NULL, orc_merge_worker, &mj[m]);
+            m++;
+        }
+        if (runs & 1) { /* the odd run out is copied through */
+            memcpy(t + cut[runs - 1], b + cut[runs - 1], sizeof(orc_binned) * (cut[runs] - cut[runs - 1]));
+        }
+        for (int k = 0; k < m; k++) pthread_join(th[k], NULL);
+        int nr = 0;
+        for (int k = 0; k < runs; k += 2) cut[nr++] = cut[k];
+        cut[nr] = n;
+        runs = nr;
+        orc_binned* x = b;
+        b = t;
+        t = x;
+    }
+    free(t);
+    free(cut);
+    free(th);
+    free(sj);
+    free(mj);
+    return b;
+}
+
 typedef struct {
     uint32_t lo, hi, n;
     const float* pos;
@@ -371,7 +461,7 @@ int64_t orc_collide_grid_mt(uint32_t n, const float* pos, float radius,
         n_threads = 1;
     }
     float cell = orc_cell_size(radius);
-    orc_binned* b = orc_bin(n, pos, cell);
+    orc_binned* b = orc_bin_mt(n, pos, cell, n_threads);
     if (!b) {
         return -1;
     }

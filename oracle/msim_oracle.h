@@ -101,6 +101,9 @@ void orc_advance_f64(uint32_t n, const float* pos, const float* tgt,
  * are [n_ticks][n] (targets: 2 floats per agent): the target and road each
  * move used and a flag (bit 0 arrived, bit 1 no move; the first tick does not
  * move when no_move_first). */
+/* Threads for orc_diff_replay64* and orc_diff_adjoint64* (default 1). */
+void orc_diff_set_threads(int n);
+
 int orc_diff_record(uint32_t n, uint32_t n_scen, const uint32_t* agent_off, float* pos, float* tgt,
                     uint32_t* road, uint32_t* rng, const float* speed, const orc_road* roads, uint32_t n_roads,
                     const uint32_t* conns, uint32_t n_conns, const float* theta, float alpha, float radius,

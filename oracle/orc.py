@@ -45,6 +45,7 @@ def _load():
         "orc_advance_f64": (None, [_u32, _vp, _vp, _vp, _vp, _vp]),
         "orc_diff_record": (ctypes.c_int, [_u32, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _vp, _u32, _vp,
                                            ctypes.c_float, ctypes.c_float, _u32, ctypes.c_int, _vp, _vp, _vp]),
+        "orc_diff_set_threads": (None, [ctypes.c_int]),
         "orc_diff_replay64": (ctypes.c_int, [_u32, _u32, _vp, _vp, _vp, _vp, ctypes.c_double, ctypes.c_double,
                                              _u32, _vp, _vp, _vp, _vp]),
         "orc_diff_adjoint64": (ctypes.c_int, [_u32, _u32, _vp, _vp, _vp, _vp, _u32, ctypes.c_double,
@@ -317,3 +318,8 @@ def diff_replay64_carry(pos0, speed, agent_off, theta, alpha, radius, sched):
     if rc != 0:
         raise RuntimeError("orc_diff_replay64_carry failed")
     return out
+
+
+def diff_set_threads(n):
+    """Threads for the binary64 replay and adjoint (diff_replay64*, diff_adjoint64)."""
+    L.orc_diff_set_threads(int(n))

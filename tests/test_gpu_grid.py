@@ -19,7 +19,8 @@ def histogram(pos):
 
 
 def sorted_entries(cells, counts):
-    order = np.lexsort((cells[:, 0], cells[:, 1]))
+    """Rows in np.unique's order: by cx, then cy."""
+    order = np.lexsort((cells[:, 1], cells[:, 0]))
     return cells[order], counts[order]
 
 
@@ -50,6 +51,13 @@ def test_occupancy_folded_table_stacks_and_capacity():
     ents["pos"][:2000] = ents["pos"][7]
     sc2 = S.Scenario("stack", sc.roads, sc.conns, ents, sc.speed, sc.pc, 0, sc.world)
     g = msim.Simulation(sc2.roads, sc2.conns, sc2.entities, sc2.pc, speed=sc2.speed, table_log2=12, binning=3)
+    # before the first tick the 2,000 stacked agents share one cell
+    cells, counts, total = g.grid_occupancy()
+    assert counts.max() >= 2000
+    got_c, got_n = sorted_entries(cells, counts)
+    want_c, want_n = histogram(ents["pos"])
+    np.testing.assert_array_equal(got_c, want_c)
+    np.testing.assert_array_equal(got_n, want_n)
     g.run_ticks(3)
     pos = g.state()["pos"]
     cells, counts, total = g.grid_occupancy()
@@ -57,7 +65,6 @@ def test_occupancy_folded_table_stacks_and_capacity():
     got_c, got_n = sorted_entries(cells, counts)
     np.testing.assert_array_equal(got_c, want_c)
     np.testing.assert_array_equal(got_n, want_n)
-    assert counts.max() >= 1990
     part, pcnt, total2 = g.grid_occupancy(capacity=100)
     assert total2 == total and part.shape == (100, 2)
     # every returned entry is one of the occupied cells with its count
