@@ -172,6 +172,16 @@ int msim_read_cells(msim_sim* sim, int32_t* cell_xy);
 int msim_read_grid_occupancy(msim_sim* sim, int32_t* cell_xy, uint32_t* counts, uint32_t capacity,
                              uint32_t* n_cells);
 
+/* msim_read_cells without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  The cells are staged
+ * in a static device array of 2^21 agents (16 MiB, held by each GPU the
+ * library runs on until the process ends); handles with more agents are
+ * rejected with MSIM_ERR_INVALID.  Every call on one GPU shares that array:
+ * calls that overlap on one GPU (from different threads) race.
+ * Synchronous. */
+int msim_read_cells_noalloc(msim_sim* sim, int32_t* cell_xy);
+
 int msim_get_info(msim_sim* sim, msim_info* out);
 
 /* Time n_ticks fused ticks on the handle's stream with HIP events.

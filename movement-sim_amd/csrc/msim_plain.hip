@@ -12,6 +12,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -1302,6 +1303,66 @@ extern "C" int msim_read_cells(msim_sim* s, int32_t* cell_xy) {
         if (e != hipSuccess) rc = fail(s, MSIM_ERR_HIP, "cell readback: %s", hipGetErrorString(e));
     }
     (void)hipFreeAsync(d, s->stream);
+    if (rc == MSIM_OK) rc = wait_stream(s);
+    return rc;
+}
+
+namespace {
+
+// Staging of msim_read_cells_noalloc: one array per GPU, in the code object,
+// so that the readback needs no device allocation.  Host code never passes
+// the symbol itself to a kernel or a copy: it stands for a host-side shadow,
+// and the device address comes from hipGetSymbolAddress, per device.
+constexpr uint32_t kCellsScratchAgents = 1u << 21;  // 16 MiB of int2
+__device__ __align__(256) int2 cells_scratch[kCellsScratchAgents];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+int2* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of cells_scratch on the handle's device (looked up once per
+// device, with that device current).
+int cells_scratch_ptr(msim_sim* s, int2** out) {
+    if (s->device < 0 || s->device >= kMaxDevices) {
+        return fail(s, MSIM_ERR_INVALID, "device %d out of range for the allocation-free readback", s->device);
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mutex);
+    if (!g_scratch_ptr[s->device]) {
+        int prev = 0;
+        MSIM_TRY_HIP(s, hipGetDevice(&prev));
+        MSIM_TRY_HIP(s, hipSetDevice(s->device));
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(cells_scratch));
+        (void)hipSetDevice(prev);
+        if (e != hipSuccess || p == nullptr) {
+            return fail(s, MSIM_ERR_HIP, "hipGetSymbolAddress(cells_scratch) failed: %s", hipGetErrorString(e));
+        }
+        g_scratch_ptr[s->device] = static_cast<int2*>(p);
+    }
+    *out = g_scratch_ptr[s->device];
+    return MSIM_OK;
+}
+
+}  // namespace
+
+extern "C" int msim_read_cells_noalloc(msim_sim* s, int32_t* cell_xy) {
+    if (!s || (s->n && !cell_xy)) return fail(s, MSIM_ERR_INVALID, "NULL argument");
+    if (s->n > kCellsScratchAgents) {
+        return fail(s, MSIM_ERR_INVALID,
+                    "msim_read_cells_noalloc handles at most %u agents (this handle has %u); use msim_read_cells",
+                    kCellsScratchAgents, s->n);
+    }
+    if (s->n == 0) return MSIM_OK;
+    int2* d = nullptr;
+    int rc = cells_scratch_ptr(s, &d);
+    if (rc != MSIM_OK) return rc;
+    hipLaunchKernelGGL(k_export_cells, dim3(agent_blocks(s->n)), dim3(kBlock), 0, s->stream, s->agents(s->cur),
+                       s->grid.inv_cell, d);
+    rc = check_launch(s, "k_export_cells");
+    if (rc == MSIM_OK) {
+        const hipError_t e = hipMemcpyAsync(cell_xy, d, sizeof(int2) * s->n, hipMemcpyDeviceToHost, s->stream);
+        if (e != hipSuccess) rc = fail(s, MSIM_ERR_HIP, "cell readback: %s", hipGetErrorString(e));
+    }
     if (rc == MSIM_OK) rc = wait_stream(s);
     return rc;
 }

@@ -82,6 +82,7 @@ SIGNATURES = {
     "msim_read_state": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP, _VP, _VP]),
     "msim_read_pairs": (ctypes.c_int, [_VP, _VP, _VP, _U32, _VP]),
     "msim_read_cells": (ctypes.c_int, [_VP, _VP]),
+    "msim_read_cells_noalloc": (ctypes.c_int, [_VP, _VP]),
     "msim_read_grid_occupancy": (ctypes.c_int, [_VP, _VP, _VP, _U32, _VP]),
     "msim_get_info": (ctypes.c_int, [_VP, _VP]),
     "msim_time_ticks": (ctypes.c_int, [_VP, _U32, _VP, _VP, _U32]),
@@ -290,6 +291,12 @@ class Simulation:
         self._check(self.lib.msim_read_grid_occupancy(self.h, _ptr(cells), _ptr(counts), cap, ctypes.byref(total)))
         k = min(total.value, cap)
         return cells[:k].copy(), counts[:k].copy(), total.value
+
+    def cells_noalloc(self):
+        """cells() through the allocation-free entry point."""
+        out = np.zeros((self.n, 2), np.int32)
+        self._check(self.lib.msim_read_cells_noalloc(self.h, _ptr(out)))
+        return out
 
     def info(self):
         i = Info()
