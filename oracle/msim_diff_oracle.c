@@ -10,8 +10,8 @@
  *   p' = p~ + alpha * sum_{j in scenario, j != i, in range} w(|x|^2) x,
  *   x = p~_i - p~_j, w(q) = (1 - q / R^2)^3 (C^2 across the range boundary).
  * orc_diff_record runs it in binary32 (the move exactly as orc_move_agent;
- * the push summed over neighbours in ascending index order) and records the
- * decisions.  orc_diff_replay64 re-runs it in binary64 under recorded
+ * the push summed in fixed point, independent of the neighbours' order) and
+ * records the decisions.  orc_diff_replay64 re-runs it in binary64 under recorded
  * decisions; orc_diff_adjoint64 is the reverse mode of that replay.
  */
 #include <math.h>
@@ -23,22 +23,7 @@
 
 #define ORC_SPEED 1.4f
 
-typedef struct {
-    int64_t key;
-    uint32_t id;
-} dbin;
-
-static int dbin_cmp(const void* a, const void* b) {
-    const dbin* x = (const dbin*)a;
-    const dbin* y = (const dbin*)b;
-    if (x->key != y->key) return x->key < y->key ? -1 : 1;
-    return x->id < y->id ? -1 : (x->id > y->id ? 1 : 0);
-}
-
-static int64_t dkey(int64_t cx, int64_t cy) { return (cy << 32) + cx; }
-
-/* Neighbour lists of n points (cell >= R): for each i, the j != i with
- * pred(i, j), in ascending j.  Returns CSR (off[n+1], idx) via malloc. */
+/* The neighbour predicate: binary32 (orc_in_range) or binary64. */
 typedef int (*pred_fn)(const void* pts, uint32_t i, uint32_t j, double r);
 
 static int pred_f32(const void* pts, uint32_t i, uint32_t j, double r) {
@@ -50,86 +35,6 @@ static int pred_f64(const void* pts, uint32_t i, uint32_t j, double r) {
     const double* p = (const double*)pts;
     double dx = p[2 * i] - p[2 * j], dy = p[2 * i + 1] - p[2 * j + 1];
     return dx * dx + dy * dy < r * r;
-}
-
-static int neighbour_lists(uint32_t n, const void* pts, int f64, double r, double cell, uint32_t** off_out,
-                           uint32_t** idx_out) {
-    dbin* b = (dbin*)malloc(sizeof(dbin) * (n ? n : 1));
-    uint32_t* off = (uint32_t*)calloc((size_t)n + 1, sizeof(uint32_t));
-    if (!b || !off) {
-        free(b);
-        free(off);
-        return -1;
-    }
-    const float* pf = (const float*)pts;
-    const double* pd = (const double*)pts;
-    for (uint32_t i = 0; i < n; i++) {
-        double x = f64 ? pd[2 * i] : (double)pf[2 * i];
-        double y = f64 ? pd[2 * i + 1] : (double)pf[2 * i + 1];
-        b[i].key = dkey((int64_t)floor(x / cell), (int64_t)floor(y / cell));
-        b[i].id = i;
-    }
-    qsort(b, n, sizeof(dbin), dbin_cmp);
-    pred_fn pred = f64 ? pred_f64 : pred_f32;
-    size_t cap = 16, cnt = 0;
-    uint32_t* idx = (uint32_t*)malloc(sizeof(uint32_t) * cap);
-    uint32_t* tmp = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
-    if (!idx || !tmp) {
-        free(b);
-        free(off);
-        free(idx);
-        free(tmp);
-        return -1;
-    }
-    for (uint32_t i = 0; i < n; i++) {
-        double x = f64 ? pd[2 * i] : (double)pf[2 * i];
-        double y = f64 ? pd[2 * i + 1] : (double)pf[2 * i + 1];
-        int64_t cx = (int64_t)floor(x / cell), cy = (int64_t)floor(y / cell);
-        uint32_t m = 0;
-        for (int64_t dy = -1; dy <= 1; dy++) {
-            for (int64_t dx = -1; dx <= 1; dx++) {
-                int64_t key = dkey(cx + dx, cy + dy);
-                uint32_t lo = 0, hi = n;
-                while (lo < hi) {
-                    uint32_t mid = lo + (hi - lo) / 2;
-                    if (b[mid].key < key) lo = mid + 1; else hi = mid;
-                }
-                for (uint32_t k = lo; k < n && b[k].key == key; k++) {
-                    uint32_t j = b[k].id;
-                    if (j != i && pred(pts, i, j, r)) tmp[m++] = j;
-                }
-            }
-        }
-        /* ascending j */
-        for (uint32_t a = 1; a < m; a++) {
-            uint32_t v = tmp[a], c = a;
-            while (c > 0 && tmp[c - 1] > v) {
-                tmp[c] = tmp[c - 1];
-                c--;
-            }
-            tmp[c] = v;
-        }
-        if (cnt + m > cap) {
-            while (cnt + m > cap) cap *= 2;
-            uint32_t* grown = (uint32_t*)realloc(idx, sizeof(uint32_t) * cap);
-            if (!grown) {
-                free(b);
-                free(off);
-                free(idx);
-                free(tmp);
-                return -1;
-            }
-            idx = grown;
-        }
-        memcpy(idx + cnt, tmp, sizeof(uint32_t) * m);
-        cnt += m;
-        off[i + 1] = (uint32_t)cnt;
-    }
-    free(b);
-    free(tmp);
-    *off_out = off;
-    *idx_out = idx;
-    return 0;
 }
 
 /* ---- threads ------------------------------------------------------------
@@ -310,66 +215,144 @@ static int neighbour_lists_mt(uint32_t n, const void* pts, int f64, double r, do
     return 0;
 }
 
+/* The binary32 record, split over the threads per agent (the move, and the
+ * push of each scenario); every agent's result is independent of the split. */
+typedef struct {
+    uint32_t n, t;
+    int no_move_first;
+    float *pos, *tgt, *ptil;
+    uint32_t *road, *rng;
+    const float* speed;
+    const orc_road* roads;
+    uint32_t n_roads;
+    const uint32_t* conns;
+    uint32_t n_conns;
+    const float* theta;
+    float* sched_tgt;
+    uint32_t* sched_road;
+    uint8_t* sched_flag;
+    int err[256];
+} rec_ctx;
+
+static void rec_move_range(void* c_, uint32_t lo, uint32_t hi, int w) {
+    rec_ctx* c = (rec_ctx*)c_;
+    const uint32_t n = c->n, t = c->t;
+    float dir[2];
+    for (uint32_t i = lo; i < hi; i++) {
+        uint32_t r = c->road[i];
+        if (r >= c->n_roads) {
+            c->err[w] = ORC_ERR_ROAD;
+            return;
+        }
+        uint8_t flag = 0;
+        c->sched_tgt[((size_t)t * n + i) * 2] = c->tgt[2 * i];
+        c->sched_tgt[((size_t)t * n + i) * 2 + 1] = c->tgt[2 * i + 1];
+        c->sched_road[(size_t)t * n + i] = r;
+        if (t == 0 && c->no_move_first) {
+            flag = 2;
+        } else {
+            float s = (c->speed ? c->speed[i] : ORC_SPEED) * c->theta[r];
+            int arrived = 0;
+            int rc = orc_move_agent(&c->pos[2 * i], &c->tgt[2 * i], dir, &c->road[i], &c->rng[4 * i], s, c->roads,
+                                    c->n_roads, c->conns, c->n_conns, &arrived);
+            if (rc != ORC_OK) {
+                c->err[w] = rc;
+                return;
+            }
+            flag = arrived ? 1 : 0;
+        }
+        c->sched_flag[(size_t)t * n + i] = flag;
+        c->ptil[2 * i] = c->pos[2 * i];
+        c->ptil[2 * i + 1] = c->pos[2 * i + 1];
+    }
+}
+
+typedef struct {
+    const float* ptil;
+    float* pos;
+    uint32_t o;
+    const uint32_t *off, *idx;
+    float alpha, inv_r2;
+    double fix_scale;
+} rec_push_ctx;
+
+/* p' = p~ + alpha A, A = sum_j w x_ij: each binary32 term w x rounded to a
+ * multiple of 1/fix_scale (ties to even) and the terms summed as 64-bit
+ * integers, so that A does not depend on the order of the neighbours. */
+static void rec_push_range(void* c_, uint32_t lo, uint32_t hi, int w) {
+    (void)w;
+    const rec_push_ctx* c = (const rec_push_ctx*)c_;
+    for (uint32_t i = lo; i < hi; i++) {
+        const float* me = &c->ptil[2 * (size_t)(c->o + i)];
+        long long ix = 0, iy = 0;
+        for (uint32_t k = c->off[i]; k < c->off[i + 1]; k++) {
+            const float* q = &c->ptil[2 * (size_t)(c->o + c->idx[k])];
+            float x = me[0] - q[0], y = me[1] - q[1];
+            float h = 1.0f - (x * x + y * y) * c->inv_r2;
+            float wv = h * h * h;
+            ix += llrint((double)(wv * x) * c->fix_scale);
+            iy += llrint((double)(wv * y) * c->fix_scale);
+        }
+        float ax = (float)((double)ix / c->fix_scale), ay = (float)((double)iy / c->fix_scale);
+        float px = c->alpha * ax, py = c->alpha * ay;
+        c->pos[2 * (size_t)(c->o + i)] = me[0] + px;
+        c->pos[2 * (size_t)(c->o + i) + 1] = me[1] + py;
+    }
+}
+
+double orc_diff_fix_scale(float radius) {
+    int e = (int)ceilf(log2f(radius));
+    return ldexp(1.0, 40 - (e > 0 ? e : 0));
+}
+
 int orc_diff_record(uint32_t n, uint32_t n_scen, const uint32_t* agent_off, float* pos, float* tgt,
                     uint32_t* road, uint32_t* rng, const float* speed, const orc_road* roads, uint32_t n_roads,
                     const uint32_t* conns, uint32_t n_conns, const float* theta, float alpha, float radius,
                     uint32_t n_ticks, int no_move_first, float* sched_tgt, uint32_t* sched_road,
-                    uint8_t* sched_flag) {
+                    uint8_t* sched_flag, float* traj) {
     const double cell = (double)orc_cell_size(radius);
-    const float inv_r2 = 1.0f / (radius * radius);
     float* ptil = (float*)malloc(sizeof(float) * 2 * (n ? n : 1));
-    float dir[2];
     if (!ptil) return -1;
+    rec_ctx c;
+    memset(&c, 0, sizeof(c));
+    c.n = n;
+    c.no_move_first = no_move_first;
+    c.pos = pos;
+    c.tgt = tgt;
+    c.ptil = ptil;
+    c.road = road;
+    c.rng = rng;
+    c.speed = speed;
+    c.roads = roads;
+    c.n_roads = n_roads;
+    c.conns = conns;
+    c.n_conns = n_conns;
+    c.theta = theta;
+    c.sched_tgt = sched_tgt;
+    c.sched_road = sched_road;
+    c.sched_flag = sched_flag;
     for (uint32_t t = 0; t < n_ticks; t++) {
-        for (uint32_t i = 0; i < n; i++) {
-            uint32_t r = road[i];
-            if (r >= n_roads) {
+        if (traj) memcpy(traj + (size_t)t * 4 * n, pos, sizeof(float) * 2 * n);
+        c.t = t;
+        par_for(n, rec_move_range, &c);
+        for (int w = 0; w < 256; w++) {
+            if (c.err[w] != 0) {
                 free(ptil);
-                return ORC_ERR_ROAD;
+                return c.err[w];
             }
-            uint8_t flag = 0;
-            sched_tgt[((size_t)t * n + i) * 2] = tgt[2 * i];
-            sched_tgt[((size_t)t * n + i) * 2 + 1] = tgt[2 * i + 1];
-            sched_road[(size_t)t * n + i] = r;
-            if (t == 0 && no_move_first) {
-                flag = 2;
-            } else {
-                float s = (speed ? speed[i] : ORC_SPEED) * theta[r];
-                int arrived = 0;
-                int rc = orc_move_agent(&pos[2 * i], &tgt[2 * i], dir, &road[i], &rng[4 * i], s, roads, n_roads,
-                                        conns, n_conns, &arrived);
-                if (rc != ORC_OK) {
-                    free(ptil);
-                    return rc;
-                }
-                flag = arrived ? 1 : 0;
-            }
-            sched_flag[(size_t)t * n + i] = flag;
-            ptil[2 * i] = pos[2 * i];
-            ptil[2 * i + 1] = pos[2 * i + 1];
         }
+        if (traj) memcpy(traj + (size_t)t * 4 * n + 2 * (size_t)n, ptil, sizeof(float) * 2 * n);
         if (alpha != 0.0f) {
             for (uint32_t sc = 0; sc < n_scen; sc++) {
                 uint32_t o = agent_off[sc], m = agent_off[sc + 1] - o;
                 uint32_t *off = NULL, *idx = NULL;
-                if (neighbour_lists(m, ptil + 2 * (size_t)o, 0, radius, cell, &off, &idx) != 0) {
+                if (neighbour_lists_mt(m, ptil + 2 * (size_t)o, 0, radius, cell, &off, &idx) != 0) {
                     free(ptil);
                     return -1;
                 }
-                for (uint32_t i = 0; i < m; i++) {
-                    const float* me = &ptil[2 * (size_t)(o + i)];
-                    float ax = 0.0f, ay = 0.0f;
-                    for (uint32_t k = off[i]; k < off[i + 1]; k++) {
-                        const float* q = &ptil[2 * (size_t)(o + idx[k])];
-                        float x = me[0] - q[0], y = me[1] - q[1];
-                        float h = 1.0f - (x * x + y * y) * inv_r2;
-                        float w = h * h * h;
-                        ax += w * x;
-                        ay += w * y;
-                    }
-                    pos[2 * (size_t)(o + i)] = me[0] + alpha * ax;
-                    pos[2 * (size_t)(o + i) + 1] = me[1] + alpha * ay;
-                }
+                rec_push_ctx pc = {ptil, pos, o, off, idx, alpha, 1.0f / (radius * radius),
+                                   orc_diff_fix_scale(radius)};
+                par_for(m, rec_push_range, &pc);
                 free(off);
                 free(idx);
             }
@@ -566,7 +549,7 @@ int orc_diff_adjoint64(uint32_t n, uint32_t n_scen, const uint32_t* agent_off, c
                        uint32_t n_ticks, const float* sched_tgt, const uint32_t* sched_road,
                        const uint8_t* sched_flag, const double* grad_pos, double* grad_theta, double* grad_alpha) {
     return orc_diff_adjoint64_ex(n, n_scen, agent_off, pos0, speed, theta, n_roads, alpha, radius, n_ticks, sched_tgt,
-                                 sched_road, sched_flag, NULL, NULL, 0, 0, grad_pos, grad_theta, grad_alpha);
+                                 sched_road, sched_flag, NULL, NULL, 0, 0, NULL, grad_pos, grad_theta, grad_alpha);
 }
 
 /* Reverse sweep of one tick: the avoidance part (per scenario) and the move
@@ -675,26 +658,33 @@ int orc_diff_adjoint64_ex(uint32_t n, uint32_t n_scen, const uint32_t* agent_off
                           const float* speed, const double* theta, uint32_t n_roads, double alpha, double radius,
                           uint32_t n_ticks, const float* sched_tgt, const uint32_t* sched_road,
                           const uint8_t* sched_flag, const float* final_tgt, const uint32_t* final_road,
-                          int traj_carry, int adj_carry, const double* grad_pos, double* grad_theta,
-                          double* grad_alpha) {
+                          int traj_carry, int adj_carry, const float* traj32, const double* grad_pos,
+                          double* grad_theta, double* grad_alpha) {
     const double cell = (double)orc_cell_size((float)radius);
     const int nw = g_threads;
-    double* traj = (double*)malloc(sizeof(double) * 4 * (size_t)n * (n_ticks ? n_ticks : 1));
-    double* fin = (double*)malloc(sizeof(double) * 2 * (n ? n : 1));
-    double* lam = (double*)malloc(sizeof(double) * 2 * (n ? n : 1));
-    double* lt = (double*)malloc(sizeof(double) * 2 * (n ? n : 1));
+    const size_t n2 = 2 * (size_t)(n ? n : 1);
+    double* traj = (double*)malloc(sizeof(double) * (traj32 ? 2 * n2 : 2 * n2 * (n_ticks ? n_ticks : 1)));
+    double* fin = (double*)malloc(sizeof(double) * n2);
+    double* lam = (double*)malloc(sizeof(double) * n2);
+    double* lt = (double*)malloc(sizeof(double) * n2);
     double* gt = (double*)calloc((size_t)nw * (n_roads ? n_roads : 1), sizeof(double));
     double* ga = (double*)calloc((size_t)nw, sizeof(double));
     int rc = -1;
     if (!traj || !fin || !lam || !lt || !gt || !ga) goto done;
-    if (replay64(n, n_scen, agent_off, pos0, speed, theta, alpha, radius, n_ticks, sched_tgt, sched_road,
-                 sched_flag, final_tgt, final_road, traj_carry, fin, traj) != 0) {
+    if (!traj32 && replay64(n, n_scen, agent_off, pos0, speed, theta, alpha, radius, n_ticks, sched_tgt, sched_road,
+                            sched_flag, final_tgt, final_road, traj_carry, fin, traj) != 0) {
         goto done;
     }
     memcpy(lam, grad_pos, sizeof(double) * 2 * n);
     for (uint32_t t = n_ticks; t-- > 0;) {
-        const double* pre = traj + (size_t)t * 4 * n;
+        const double* pre = traj32 ? traj : traj + (size_t)t * 4 * n;
         const double* pt = pre + 2 * (size_t)n;
+        const float* pt32 = NULL;
+        if (traj32) {  /* the given binary32 trajectory, widened */
+            const float* src = traj32 + (size_t)t * 4 * n;
+            for (size_t k = 0; k < 4 * (size_t)n; k++) traj[k] = (double)src[k];
+            pt32 = src + 2 * (size_t)n;
+        }
         adj_ctx c = {n, n_ticks, t, 0, n_roads, pre, pt, lam, lt, lam, NULL, NULL, alpha, 1.0 / (radius * radius),
                      ga, gt, speed, theta, sched_tgt, sched_road, sched_flag, final_tgt, final_road, adj_carry};
         /* avoidance: lt = lam + alpha sum J (lam_i - lam_j); dL/dalpha += lam . A */
@@ -702,7 +692,10 @@ int orc_diff_adjoint64_ex(uint32_t n, uint32_t n_scen, const uint32_t* agent_off
         for (uint32_t sc = 0; sc < n_scen; sc++) {
             uint32_t o = agent_off[sc], m = agent_off[sc + 1] - o;
             uint32_t *off = NULL, *idx = NULL;
-            if (neighbour_lists_mt(m, pt + 2 * (size_t)o, 1, radius, cell, &off, &idx) != 0) goto done;
+            /* neighbours by the binary32 predicate on a binary32 trajectory */
+            int lists = pt32 ? neighbour_lists_mt(m, pt32 + 2 * (size_t)o, 0, radius, cell, &off, &idx)
+                             : neighbour_lists_mt(m, pt + 2 * (size_t)o, 1, radius, cell, &off, &idx);
+            if (lists != 0) goto done;
             c.o = o;
             c.off = off;
             c.idx = idx;

@@ -101,14 +101,21 @@ void orc_advance_f64(uint32_t n, const float* pos, const float* tgt,
  * are [n_ticks][n] (targets: 2 floats per agent): the target and road each
  * move used and a flag (bit 0 arrived, bit 1 no move; the first tick does not
  * move when no_move_first). */
-/* Threads for orc_diff_replay64* and orc_diff_adjoint64* (default 1). */
+/* Threads for orc_diff_record, orc_diff_replay64* and orc_diff_adjoint64*
+ * (default 1). */
 void orc_diff_set_threads(int n);
 
+/* binary32 run recording its decisions; the push terms are summed in fixed
+ * point (orc_diff_fix_scale), the same order-independent sum as the device's.
+ * traj (nullable) receives each tick's pre-move and post-move (pre-push)
+ * positions, [n_ticks][2][n][2]. */
 int orc_diff_record(uint32_t n, uint32_t n_scen, const uint32_t* agent_off, float* pos, float* tgt,
                     uint32_t* road, uint32_t* rng, const float* speed, const orc_road* roads, uint32_t n_roads,
                     const uint32_t* conns, uint32_t n_conns, const float* theta, float alpha, float radius,
                     uint32_t n_ticks, int no_move_first, float* sched_tgt, uint32_t* sched_road,
-                    uint8_t* sched_flag);
+                    uint8_t* sched_flag, float* traj);
+/* The record's fixed-point unit for the push terms: 2^(40 - max(0, ceil(log2 R))). */
+double orc_diff_fix_scale(float radius);
 /* binary64 replay under a schedule: final positions (2 doubles per agent). */
 int orc_diff_replay64(uint32_t n, uint32_t n_scen, const uint32_t* agent_off, const float* pos0,
                       const float* speed, const double* theta, double alpha, double radius, uint32_t n_ticks,
@@ -126,7 +133,10 @@ int orc_diff_adjoint64(uint32_t n, uint32_t n_scen, const uint32_t* agent_off, c
  * the decisions after the last tick.  orc_diff_adjoint64_ex runs the reverse
  * pass on the snapped (traj_carry = 0) or carried (1) trajectory, with
  * arrivals cutting the gradient (adj_carry = 0) or differentiated as the
- * carry-over (1). */
+ * carry-over (1).  traj32 (nullable): the binary32 trajectory to linearise
+ * around instead of the binary64 replay, as orc_diff_record's traj gives it
+ * ([n_ticks][2][n][2]: each tick's pre-move and post-move positions), with
+ * the binary32 neighbour predicate. */
 int orc_diff_replay64_carry(uint32_t n, uint32_t n_scen, const uint32_t* agent_off, const float* pos0,
                             const float* speed, const double* theta, double alpha, double radius, uint32_t n_ticks,
                             const float* sched_tgt, const uint32_t* sched_road, const uint8_t* sched_flag,
@@ -135,8 +145,8 @@ int orc_diff_adjoint64_ex(uint32_t n, uint32_t n_scen, const uint32_t* agent_off
                           const float* speed, const double* theta, uint32_t n_roads, double alpha, double radius,
                           uint32_t n_ticks, const float* sched_tgt, const uint32_t* sched_road,
                           const uint8_t* sched_flag, const float* final_tgt, const uint32_t* final_road,
-                          int traj_carry, int adj_carry, const double* grad_pos, double* grad_theta,
-                          double* grad_alpha);
+                          int traj_carry, int adj_carry, const float* traj32, const double* grad_pos,
+                          double* grad_theta, double* grad_alpha);
 
 #ifdef __cplusplus
 }

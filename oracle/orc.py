@@ -44,7 +44,9 @@ def _load():
         "orc_collide_grid_mt": (ctypes.c_int64, [_u32, _vp, ctypes.c_float, _vp, ctypes.c_int]),
         "orc_advance_f64": (None, [_u32, _vp, _vp, _vp, _vp, _vp]),
         "orc_diff_record": (ctypes.c_int, [_u32, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _vp, _u32, _vp,
-                                           ctypes.c_float, ctypes.c_float, _u32, ctypes.c_int, _vp, _vp, _vp]),
+                                           ctypes.c_float, ctypes.c_float, _u32, ctypes.c_int, _vp, _vp, _vp,
+                                           _vp]),
+        "orc_diff_fix_scale": (ctypes.c_double, [ctypes.c_float]),
         "orc_diff_set_threads": (None, [ctypes.c_int]),
         "orc_diff_replay64": (ctypes.c_int, [_u32, _u32, _vp, _vp, _vp, _vp, ctypes.c_double, ctypes.c_double,
                                              _u32, _vp, _vp, _vp, _vp]),
@@ -54,7 +56,7 @@ def _load():
                                                    _u32, _vp, _vp, _vp, _vp, _vp, _vp]),
         "orc_diff_adjoint64_ex": (ctypes.c_int, [_u32, _u32, _vp, _vp, _vp, _vp, _u32, ctypes.c_double,
                                                  ctypes.c_double, _u32, _vp, _vp, _vp, _vp, _vp, ctypes.c_int,
-                                                 ctypes.c_int, _vp, _vp, _vp]),
+                                                 ctypes.c_int, _vp, _vp, _vp, _vp]),
     }
     for k, (r, a) in sig.items():
         f = getattr(lib, k)
@@ -244,9 +246,11 @@ class DiffSchedule:
         self.final_road = None if final_road is None else np.ascontiguousarray(final_road, np.uint32)  # [n]
 
 
-def diff_record(roads, conns, entities, speed, agent_off, theta, alpha, radius, ticks):
+def diff_record(roads, conns, entities, speed, agent_off, theta, alpha, radius, ticks, trajectory=False):
     """binary32 differentiable run from the entities' state (the first tick
-    does not move when the agents are uninitialised): (final pos, schedule)."""
+    does not move when the agents are uninitialised): (final pos, schedule),
+    and with trajectory=True also every tick's pre-move and post-move
+    positions, float32 [ticks, 2, n, 2]."""
     st = State(entities, speed)
     n = st.n
     off = np.ascontiguousarray(agent_off, np.uint32)
@@ -257,12 +261,14 @@ def diff_record(roads, conns, entities, speed, agent_off, theta, alpha, radius, 
     rd = np.zeros((ticks, n), np.uint32)
     fl = np.zeros((ticks, n), np.uint8)
     no_move = int((st.init == 0).all()) if n else 0
+    traj = np.zeros((ticks, 2, n, 2), np.float32) if trajectory else None
     rc = L.orc_diff_record(n, off.shape[0] - 1, _p(off), _p(st.pos), _p(st.tgt), _p(st.road), _p(st.rng),
                            _p(st.speed), _p(rb), roads.shape[0], _p(cn), cn.shape[0], _p(th), alpha, radius, ticks,
-                           no_move, _p(tg), _p(rd), _p(fl))
+                           no_move, _p(tg), _p(rd), _p(fl), _p(traj))
     if rc != 0:
         raise RuntimeError(f"orc_diff_record failed: {rc}")
-    return st.pos, DiffSchedule(tg, rd, fl, st.tgt.copy(), st.road.copy())
+    sched = DiffSchedule(tg, rd, fl, st.tgt.copy(), st.road.copy())
+    return (st.pos, sched, traj) if trajectory else (st.pos, sched)
 
 
 def diff_replay64(pos0, speed, agent_off, theta, alpha, radius, sched):
@@ -279,11 +285,14 @@ def diff_replay64(pos0, speed, agent_off, theta, alpha, radius, sched):
     return out
 
 
-def diff_adjoint64(pos0, speed, agent_off, theta, alpha, radius, sched, grad_pos, carry=False, carry_traj=False):
+def diff_adjoint64(pos0, speed, agent_off, theta, alpha, radius, sched, grad_pos, carry=False, carry_traj=False,
+                   traj32=None):
     """(dL/dtheta, dL/dalpha) in binary64 for dL/d(final pos) = grad_pos.
     carry: arrivals differentiated as carrying their leftover time onto the
     next road (the surrogate); carry_traj: on the carried trajectory
-    (diff_replay64_carry's) instead of the snapped one."""
+    (diff_replay64_carry's) instead of the snapped one; traj32: linearised
+    around this binary32 trajectory (diff_record's) instead of the binary64
+    replay."""
     p0 = np.ascontiguousarray(pos0, np.float32)
     n = p0.shape[0]
     off = np.ascontiguousarray(agent_off, np.uint32)
@@ -294,9 +303,13 @@ def diff_adjoint64(pos0, speed, agent_off, theta, alpha, radius, sched, grad_pos
     ga = np.zeros(1, np.float64)
     if (carry or carry_traj) and sched.final_tgt is None:
         raise ValueError("the carry-over needs the schedule's final decisions")
+    tj = None if traj32 is None else np.ascontiguousarray(traj32, np.float32)
+    if tj is not None and tj.shape != (sched.ticks, 2, n, 2):
+        raise ValueError("traj32 must be [ticks, 2, n, 2]")
     rc = L.orc_diff_adjoint64_ex(n, off.shape[0] - 1, _p(off), _p(p0), _p(sp), _p(th), th.shape[0], float(alpha),
                                  float(radius), sched.ticks, _p(sched.tgt), _p(sched.road), _p(sched.flag),
-                                 _p(sched.final_tgt), _p(sched.final_road), int(carry_traj), int(carry), _p(g),
+                                 _p(sched.final_tgt), _p(sched.final_road), int(carry_traj), int(carry),
+                                 _p(tj), _p(g),
                                  _p(gt), _p(ga))
     if rc != 0:
         raise RuntimeError("orc_diff_adjoint64 failed")
@@ -320,6 +333,12 @@ def diff_replay64_carry(pos0, speed, agent_off, theta, alpha, radius, sched):
     return out
 
 
+def diff_fix_scale(radius):
+    """The binary32 record's fixed-point unit for push terms (1 / its value)."""
+    return float(L.orc_diff_fix_scale(radius))
+
+
 def diff_set_threads(n):
-    """Threads for the binary64 replay and adjoint (diff_replay64*, diff_adjoint64)."""
+    """Threads for the binary32 record, the binary64 replay and the adjoint
+    (diff_record, diff_replay64*, diff_adjoint64)."""
     L.orc_diff_set_threads(int(n))
