@@ -119,11 +119,13 @@ def plain_line(args, sc):
     for k in ("binning", "collide_mode", "bin_unroll", "reorder_interval", "segment_log2"):
         if getattr(args, k) is not None:
             opts[k] = getattr(args, k)
+    opts["graph_ticks"] = graph_ticks_for(args.steps, opts.get("graph_ticks"))
     sim = msim.Simulation(sc.roads, sc.conns, sc.entities, sc.pc, speed=sc.speed, **opts)
     info = sim.info()
 
     sim.run_ticks(max(1, args.warmup))  # includes the insert-only first dispatch
     sim.sync()
+    prime_graphs(sim, args.steps, opts)
     t0 = time.perf_counter()
     sim.run_ticks(args.steps)
     sim.sync()
@@ -147,6 +149,28 @@ def plain_line(args, sc):
     }
     extra = {"fused_tick": fused(el / args.steps, n, per_launch_ms)}
     return n * args.steps / el, 1e3 * el / args.steps, config, roof, extra
+
+
+def dense_record(args):
+    """The dense stress workload (16M agents on one city map) through the
+    plain handle with its tuned options: ms per tick, the fused tick's
+    fraction of HBM peak and the dominant kernel against its roofline."""
+    from msim import scenarios as S
+
+    t = time.perf_counter()
+    sc = S.config(3)
+    gen_s = time.perf_counter() - t
+    a = argparse.Namespace(**vars(args))
+    a.config, a.steps, a.warmup = 3, args.dense_steps, 20
+    for k in ("binning", "collide_mode", "bin_unroll", "reorder_interval", "segment_log2"):
+        setattr(a, k, None)
+    print(f"# dense stress workload: {sc.entities.shape[0]} agents (generated in {gen_s:.1f} s)",
+          file=sys.stderr, flush=True)
+    value, ms, config, roof, extra = plain_line(a, sc)
+    kus = roof.pop("kernels_us_per_tick")
+    return {"workload": WORKLOADS[3], "agents": config["agents"], "steps": a.steps, "ms_per_tick": ms,
+            "agent_ticks_per_s": value, "fused_tick": extra["fused_tick"], "roofline": roof,
+            "kernels_us_per_tick": kus, "options": config["options"]}
 
 
 def roofline(per_launch_ms, n, uniform, cfg):
@@ -341,6 +365,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--untuned", action="store_true", help="library defaults instead of msim/tuned.json")
+    ap.add_argument("--dense-steps", type=int, default=500, help="batched workload: timed ticks of the dense "
+                    "stress sub-record (0 leaves it out)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only; multi-GPU runs are not part of it "
@@ -362,6 +388,8 @@ def main():
               f"{time.perf_counter() - t:.1f} s)", file=sys.stderr, flush=True)
         value, ms_per_step, config, roof, extra = batched_line(args, scs)
         sample_sc = scs
+        if args.dense_steps > 0:
+            extra["dense16M"] = dense_record(args)
     else:
         sc = S.config(args.config)
         print(f"# workload config {args.config}: {sc.entities.shape[0]} agents, {sc.roads.shape[0]} roads "

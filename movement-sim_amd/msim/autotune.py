@@ -12,7 +12,7 @@ a JSON file keyed by the workload's shape.
 
 The batched handle's knobs: rebinning segment size (for every scenario),
 reorder interval, ticks per captured graph segment, the neighbour walk and
-the rebinning mode (two passes or one).
+the rebinning mode (two passes, one pass, or the bucket-sorted state).
 """
 import argparse
 import json
@@ -27,26 +27,28 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CACHE = os.path.join(HERE, "tuned.json")
 
 # Knob -> candidate values (0 = the library's automatic choice where it has one).
+# The descent runs once per layout (binning), so that the segment size and
+# load batching are tuned for each layout, and keeps the best.
 SPACE = {
-    "table_width_log2": [0, 5, 6, 7, 8, 9, 10, 11, 12],
+    "binning": [0, 1, 2, 3],
     "segment_log2": [0, 9, 10, 11, 12, 13, 14],
-    "binning": [0, 1, 2],
+    "bin_unroll": [0, 2, 4, 8],
+    "table_width_log2": [0, 5, 6, 7, 8, 9, 10, 11, 12],
     "route_queue": [0, 1],
     "reorder_interval": [8, 16, 32, 64],
     "collide_mode": [0, 1, 2],
-    "bin_unroll": [0, 4, 8],
 }
 DEFAULT = {"table_width_log2": 0, "segment_log2": 0, "binning": 0, "route_queue": 0,
            "reorder_interval": 16, "collide_mode": 0, "bin_unroll": 0}
 
 
 BATCH_SPACE = {
+    "binning": [0, 2, 3],
     "segment_log2": [0, 11, 12, 13],
+    "bin_unroll": [0, 2, 4, 8],
     "reorder_interval": [10, 20, 25, 50],
     "graph_ticks": [50, 100, 200],
     "collide_mode": [0, 1, 2],
-    "binning": [0, 2],
-    "bin_unroll": [0, 4, 8],
 }
 BATCH_DEFAULT = {"segment_log2": 0, "reorder_interval": 20, "graph_ticks": 100, "collide_mode": 0, "binning": 0,
                  "bin_unroll": 0}
@@ -77,7 +79,7 @@ def time_batch_options(scs, opts, warmup, ticks):
 
 
 def tune_batch(scs, warmup=101, ticks=200, log=print):
-    return _descend(lambda o: time_batch_options(scs, o, warmup, ticks), BATCH_DEFAULT, BATCH_SPACE, log)
+    return _descend_per_layout(lambda o: time_batch_options(scs, o, warmup, ticks), BATCH_DEFAULT, BATCH_SPACE, log)
 
 
 def tuned_batch_options(scs, path=CACHE):
@@ -112,7 +114,7 @@ def time_options(sc, opts, warmup, ticks):
 
 def tune(sc, warmup=20, ticks=100, space=None, log=print):
     space = SPACE if space is None else space
-    return _descend(lambda o: time_options(sc, o, warmup, ticks), DEFAULT, space, log)
+    return _descend_per_layout(lambda o: time_options(sc, o, warmup, ticks), DEFAULT, space, log)
 
 
 def _descend(timer, default, space, log):
@@ -132,6 +134,22 @@ def _descend(timer, default, space, log):
             log(f"  {knob}={v}: {us:.1f} us/tick" + (" (new best)" if us < best_us else ""))
             if us < best_us:
                 best, best_us = cand, us
+    return best, best_us, trials
+
+
+def _descend_per_layout(timer, default, space, log):
+    """_descend over the other knobs from each binning value; the best of
+    those descents, with every trial."""
+    best, best_us, trials = None, float("inf"), []
+    rest = {k: v for k, v in space.items() if k != "binning"}
+    for b in space.get("binning", [default.get("binning", 0)]):
+        start = dict(default)
+        start["binning"] = b
+        log(f" binning {b}")
+        cand, us, tr = _descend(timer, start, rest, log)
+        trials += tr
+        if us < best_us:
+            best, best_us = cand, us
     return best, best_us, trials
 
 
