@@ -70,14 +70,14 @@ int benqueue_move_sorted(msim_batch* b, bool write_dir) {
     SortedMove mv{};
     mv.src = bsorted_bufs(b, b->cur);
     mv.dst = bsorted_bufs(b, nb);
-    mv.road = b->cold_road;
-    mv.rng = b->cold_rng;
+    mv.cold = b->cold;
     mv.roads = b->roads;
     mv.conns = b->conns;
     mv.spd_const = kShaderSpeed;
     mv.part_stats = b->part_stats;
     mv.faults = b->faults;
-    mv.arr_slot = b->arr_slot;
+    mv.arr_rec = b->arr_rec;
+    mv.arr_spd = b->arr_spd;
     mv.seg_base = b->seg_base;
     mv.seg_arr = b->seg_arr;
     BatchBin bb{};
@@ -155,10 +155,13 @@ int btick(msim_batch* b, bool write_dir) {
 }
 
 // Enters the sorted state right after the creation's renumbering (buf[cur]
-// in the order of binning bcur; road and PRNG state already uploaded in
-// caller order): the next bucket of every slot.
+// in the order of binning bcur): road and PRNG state into caller order, the
+// next bucket of every slot.
 int benter_sorted(msim_batch* b) {
     const dim3 g(bblocks(b->n));
+    const BBuffers& x = b->buf[b->cur];
+    hipLaunchKernelGGL(k_cold_enter, g, dim3(kBlock), 0, b->stream, x.perm, x.road, x.rng, x.tgt, b->roads, b->cold,
+                       b->n);
     if (b->uniform) {
         hipLaunchKernelGGL(k_bsorted_next<true>, g, dim3(kBlock), 0, b->stream, bsorted_bufs(b, b->cur), b->blk_scen,
                            b->scen, b->agent_off, b->n, kShaderSpeed, b->inv_cell);
@@ -286,8 +289,7 @@ int bleave_sorted(msim_batch* b) {
     if (!b->sorted) return MSIM_OK;
     const BBuffers& x = b->buf[b->cur];
     const dim3 g(bblocks(b->n));
-    hipLaunchKernelGGL(k_gather_perm<uint32_t>, g, dim3(kBlock), 0, b->stream, x.perm, b->cold_road, x.road, b->n);
-    hipLaunchKernelGGL(k_gather_perm<uint4>, g, dim3(kBlock), 0, b->stream, x.perm, b->cold_rng, x.rng, b->n);
+    hipLaunchKernelGGL(k_cold_publish, g, dim3(kBlock), 0, b->stream, x.perm, b->cold, x.road, x.rng, b->n);
     hipLaunchKernelGGL(k_bid_identity, g, dim3(kBlock), 0, b->stream, b->bbid[b->bcur], b->n);
     int rc = bcheck(b, "sorted state hand-over");
     if (rc != MSIM_OK) return rc;
@@ -491,9 +493,9 @@ extern "C" int msim_batch_create(uint32_t n_scen, const uint32_t* road_off, cons
     if (b->binning == 3) {
         BALLOC(b->nxt[0], n);
         BALLOC(b->nxt[1], n);
-        BALLOC(b->cold_road, n);
-        BALLOC(b->cold_rng, n);
-        BALLOC(b->arr_slot, n);
+        BALLOC(b->cold, 2 * static_cast<size_t>(n));
+        BALLOC(b->arr_rec, n);
+        if (!b->uniform) BALLOC(b->arr_spd, n);
     }
     BALLOC(b->scen, n_scen);
     BALLOC(b->agent_off, n_scen + 1);
@@ -565,10 +567,6 @@ extern "C" int msim_batch_create(uint32_t n_scen, const uint32_t* road_off, cons
     up(b0.init, init.data(), sizeof(uint32_t) * n);
     up(b0.ncount, zeros.data(), sizeof(uint32_t) * n);
     up(b0.perm, perm.data(), sizeof(uint32_t) * n);
-    if (b->binning == 3) {  // caller order: the sorted state's road and PRNG state
-        up(b->cold_road, road.data(), sizeof(uint32_t) * n);
-        up(b->cold_rng, rng.data(), sizeof(uint4) * n);
-    }
     up(b->scen, sc.data(), sizeof(ScenDev) * n_scen);
     up(b->agent_off, agent_off, sizeof(uint32_t) * (n_scen + 1));
     std::vector<BlockScen> blk(bblocks(n));
@@ -690,8 +688,17 @@ extern "C" int msim_batch_read_state(msim_batch* b, uint32_t scen, float* pos, f
     dn(t.data(), bb.tgt + off, sizeof(float2) * na);
     dn(d.data(), bb.dir + off, sizeof(float2) * na);
     // sorted state: road and PRNG state are kept in caller order
-    dn(r.data(), (b->sorted ? b->cold_road : bb.road) + off, sizeof(uint32_t) * na);
-    dn(g.data(), (b->sorted ? b->cold_rng : bb.rng) + off, sizeof(uint4) * na);
+    if (b->sorted) {
+        std::vector<uint4> cw(2 * static_cast<size_t>(na));
+        dn(cw.data(), b->cold + 2 * static_cast<size_t>(off), sizeof(uint4) * 2 * na);
+        for (uint32_t k = 0; k < na; k++) {
+            r[k] = cw[2 * k].x;
+            g[k] = cw[2 * k + 1];
+        }
+    } else {
+        dn(r.data(), bb.road + off, sizeof(uint32_t) * na);
+        dn(g.data(), bb.rng + off, sizeof(uint4) * na);
+    }
     dn(c.data(), bb.ncount + off, sizeof(uint32_t) * na);
     dn(pm.data(), bb.perm + off, sizeof(uint32_t) * na);
     if (rc != MSIM_OK) return rc;

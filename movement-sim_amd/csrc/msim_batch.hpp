@@ -387,9 +387,9 @@ struct msim_batch {
     // with cur on every move; road and PRNG state stay in packed caller order.
     bool sorted = false;
     uint32_t* nxt[2] = {nullptr, nullptr};  // per buffer: bucket of each slot's next position
-    uint32_t* cold_road = nullptr;          // [n] packed caller order
-    uint4* cold_rng = nullptr;              // [n] packed caller order
-    uint32_t* arr_slot = nullptr;           // [n] arrivals' new slots, listed per segment
+    uint4* cold = nullptr;                  // [2n] packed caller order: cold_word, PRNG state
+    uint4* arr_rec = nullptr;               // [n] arrival records, listed per segment
+    float* arr_spd = nullptr;               // [n] their speeds (per-agent speeds only)
     uint32_t* seg_base = nullptr;           // [n_seg] first slot of each segment
     uint32_t* seg_arr = nullptr;            // [n_seg] arrivals of each segment
     uint32_t bin_unroll = 4;  // window slots per rebinning thread and load round

@@ -117,6 +117,41 @@ __device__ __forceinline__ void choose_route(float2& tgt, uint32_t& road, uint4&
     road = nr;
 }
 
+// The endpoint choose_route calls c for an agent on road r heading to tgt:
+// the start when tgt equals it, else the end.  Returned as (connection index,
+// connection count).  Target and road change only in the route choice, so
+// this value, taken when they are set, is the one the next route choice sees.
+__device__ __forceinline__ uint2 route_endpoint(const Endpoint& s, const Endpoint& e, float2 tgt) {
+    return (tgt.x == s.x && tgt.y == s.y) ? make_uint2(s.idx, s.cnt) : make_uint2(e.idx, e.cnt);
+}
+
+// choose_route for an agent standing on its target, with that endpoint's
+// connection index and count (c, from route_endpoint) known: the same
+// decisions and PRNG draws; the current road's record is read only at a dead
+// end.  c receives the endpoint of the new road and target.
+__device__ __forceinline__ void choose_route_at(float2& tgt, uint32_t& road, uint2& c, uint4& rng,
+                                                const msim_road* __restrict__ roads,
+                                                const uint32_t* __restrict__ conns) {
+    if (c.y <= 1u) {  // dead end: back along the same road
+        Endpoint s, e;
+        load_road(roads, road, s, e);
+        tgt = (tgt.x == s.x && tgt.y == s.y) ? make_float2(e.x, e.y) : make_float2(s.x, s.y);
+        c = route_endpoint(s, e, tgt);
+        return;
+    }
+    const uint32_t off = (c.y == 2u) ? 1u : xs128_range(rng, 1u, c.y);
+    const uint32_t nr = conns[c.x + off];
+    Endpoint ns, ne;
+    load_road(roads, nr, ns, ne);
+    if (ns.x == tgt.x && ns.y == tgt.y) {
+        tgt = make_float2(ne.x, ne.y);
+    } else {
+        tgt = make_float2(ns.x, ns.y);
+    }
+    road = nr;
+    c = route_endpoint(ns, ne, tgt);
+}
+
 // Uniform-grid cell coordinate: floor(v / cell) with cell a power of two, so
 // v * inv_cell is exact; clamped to +-2^30 (NaN maps to the low clamp).
 __device__ __forceinline__ int32_t cell_coord(float v, float inv_cell) {

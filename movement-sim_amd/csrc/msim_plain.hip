@@ -69,13 +69,13 @@ struct msim_sim {
     // Sorted state (binning 3), entered at the first move dispatch of
     // initialised agents: pos, tgt, spd, perm (and dir, ncount) of buf[cur]
     // follow the slot order of binning bcur, which flips with cur on every
-    // move; road and PRNG state stay in caller order in cold_road / cold_rng.
+    // move; road and PRNG state stay in caller order in cold (cold_word, rng).
     bool sorted = false;
     uint32_t* nxt[2] = {nullptr, nullptr};  // per buffer: bucket of each slot's next position
-    uint32_t* cold_road = nullptr;          // [n] caller order
-    uint4* cold_rng = nullptr;              // [n] caller order
+    uint4* cold = nullptr;                  // [2n] caller order
     uint32_t* ncount_caller = nullptr;      // [n] neighbour counts kept in caller order
-    uint32_t* arr_slot = nullptr;           // [n] arrivals' new slots, listed per segment
+    uint4* arr_rec = nullptr;               // [n] arrival records, listed per segment
+    float* arr_spd = nullptr;               // [n] their speeds (per-agent speeds only)
     uint32_t* seg_base = nullptr;           // [n_seg] first slot of each segment
     uint32_t* seg_arr = nullptr;            // [n_seg] arrivals of each segment
     bool mark_move = false;  // capture: record ev_move right after the move kernel
@@ -559,8 +559,8 @@ int enter_sorted(msim_sim* s) {
     if (rc != MSIM_OK) return rc;
     const Buffers& b = s->buf[s->cur];
     const dim3 g(agent_blocks(s->n));
-    hipLaunchKernelGGL(k_scatter_perm<uint32_t>, g, dim3(kBlock), 0, s->stream, b.perm, b.road, s->cold_road, s->n);
-    hipLaunchKernelGGL(k_scatter_perm<uint4>, g, dim3(kBlock), 0, s->stream, b.perm, b.rng, s->cold_rng, s->n);
+    hipLaunchKernelGGL(k_cold_enter, g, dim3(kBlock), 0, s->stream, b.perm, b.road, b.rng, b.tgt, s->roads, s->cold,
+                       s->n);
     const GridDev& gd = s->grid;
     if (s->uniform) {
         hipLaunchKernelGGL(k_sorted_next<true>, g, dim3(kBlock), 0, s->stream, sorted_bufs(s, s->cur), s->n,
@@ -617,15 +617,15 @@ int enqueue_move_sorted(msim_sim* s, bool write_dir) {
     SortedMove mv{};
     mv.src = sorted_bufs(s, s->cur);
     mv.dst = sorted_bufs(s, nb);
-    mv.road = s->cold_road;
-    mv.rng = s->cold_rng;
+    mv.cold = s->cold;
     mv.roads = s->roads;
     mv.conns = s->conns;
     mv.spd_const = s->spd_const;
     s->stats_par ^= 1;  // the next tick's collision dispatch folds this half
     mv.part_stats = stats_buf(s);
     mv.faults = &s->ctr->faults;
-    mv.arr_slot = s->arr_slot;
+    mv.arr_rec = s->arr_rec;
+    mv.arr_spd = s->arr_spd;
     mv.seg_base = s->seg_base;
     mv.seg_arr = s->seg_arr;
     BinDev b{};
@@ -670,8 +670,7 @@ int publish_sorted(msim_sim* s) {
                            b.ncount, s->n);
         s->ncount_at = s->cur;
     }
-    hipLaunchKernelGGL(k_gather_perm<uint32_t>, g, dim3(kBlock), 0, s->stream, b.perm, s->cold_road, b.road, s->n);
-    hipLaunchKernelGGL(k_gather_perm<uint4>, g, dim3(kBlock), 0, s->stream, b.perm, s->cold_rng, b.rng, s->n);
+    hipLaunchKernelGGL(k_cold_publish, g, dim3(kBlock), 0, s->stream, b.perm, s->cold, b.road, b.rng, s->n);
     hipLaunchKernelGGL(k_set_init, g, dim3(kBlock), 0, s->stream, b.init, s->n);
     return check_launch(s, "sorted state readback");
 }
@@ -1026,10 +1025,10 @@ extern "C" int msim_create(const msim_road* roads, uint32_t n_roads, const uint3
     if (s->binning == 3) {
         ALLOC(s->nxt[0], n);
         ALLOC(s->nxt[1], n);
-        ALLOC(s->cold_road, n);
-        ALLOC(s->cold_rng, n);
+        ALLOC(s->cold, 2 * static_cast<size_t>(n));
         ALLOC(s->ncount_caller, n);
-        ALLOC(s->arr_slot, n);
+        ALLOC(s->arr_rec, n);
+        if (!s->uniform) ALLOC(s->arr_spd, n);
         ALLOC(s->seg_base, s->n_seg);
         ALLOC(s->seg_arr, s->n_seg);
     }

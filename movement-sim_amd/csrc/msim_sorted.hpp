@@ -22,11 +22,12 @@
 //     and are stored at their new slots with the bucket of their following
 //     move as the new `nxt`;
 //   * arrivals are stored at once but for what the route choice decides
-//     (target, next bucket, heading): their new slots are listed in the
-//     segment's own slice [base, base + total) of the arrival list, and
-//     k_route_sorted, launched next, routes every segment's list
-//     (random_move.comp:778-828; road and PRNG state live in caller order and
-//     are touched only there).
+//     (target, next bucket, heading): each is listed (new slot, caller index,
+//     position, speed) in the segment's own slice [base, base + total) of
+//     the arrival list, and k_route_sorted, launched next, routes every
+//     segment's list (random_move.comp:778-828).  Road and PRNG state live in
+//     caller order, 32 bytes per agent with the connection index and count
+//     of the endpoint the agent heads to, and are touched only there.
 //
 // Each agent's arithmetic is the plain tick's exactly; only where the state
 // is stored changes.  The agents' order inside a bucket follows the place
@@ -76,13 +77,19 @@ struct SortedBufs {
     float2* dir;     // [n + pad] heading, written on a batch's last tick only
 };
 
+// Caller-order route state of one agent (cold[2 h], cold[2 h + 1]): road,
+// connection index and count of the endpoint it heads to (route_endpoint),
+// and the PRNG state.
+__device__ __forceinline__ uint4 cold_word(uint32_t road, uint2 c) { return make_uint4(road, c.x, c.y, 0u); }
+
 struct SortedMove {
     SortedBufs src, dst;
-    uint32_t* arr_slot;  // [n] arrivals' new slots, segment g's at [seg_base[g], + seg_arr[g])
+    uint4* arr_rec;      // [n] arrivals {new slot, caller index, x bits, y bits}, segment g's at
+                         // [seg_base[g], + seg_arr[g])
+    float* arr_spd;      // [n] their speeds; nullptr with spd_const
     uint32_t* seg_base;  // [n_seg] first slot of each segment
     uint32_t* seg_arr;   // [n_seg] arrivals of each segment
-    uint32_t* road;  // [n] caller order
-    uint4* rng;      // [n] caller order
+    uint4* cold;         // [2 n] caller order: cold_word, PRNG state
     const msim_road* roads;
     const uint32_t* conns;
     float spd_const;
@@ -114,11 +121,12 @@ __device__ __forceinline__ void sorted_count(const uint32_t* __restrict__ nxt, u
 
 // Moves the agent of old slot k to new slot q: loads its hot state, steps
 // it (random_move.comp:841-852) and stores it; for an arrival, what the route
-// choice decides (target, next bucket, heading) is left to k_route_sorted.
-// Returns whether the agent arrived; *ins counts a cell change.
+// choice decides (target, next bucket, heading) is left to k_route_sorted,
+// and rec / rec_spd receive its arrival record.  Returns whether the agent
+// arrived; n_ins counts a cell change.
 template <bool UNIFORM, bool WRITE_DIR>
 __device__ __forceinline__ bool move_agent(const SortedMove& mv, const TableView& t, uint32_t k, uint32_t q,
-                                           uint32_t& n_ins) {
+                                           uint32_t& n_ins, uint4& rec, float& rec_spd) {
     const SortedBufs& src = mv.src;
     const SortedBufs& dst = mv.dst;
     const uint32_t b = src.nxt[k];
@@ -149,7 +157,16 @@ __device__ __forceinline__ bool move_agent(const SortedMove& mv, const TableView
         dst.nxt[q] = bucket_of_pos(next_position(np, tg, sp), t);
         if (WRITE_DIR) dst.dir[q] = d;
     }
+    rec = make_uint4(q, h, __float_as_uint(np.x), __float_as_uint(np.y));
+    rec_spd = sp;
     return arr;
+}
+
+// Appends one arrival record at list index i.
+template <bool UNIFORM>
+__device__ __forceinline__ void put_arrival(const SortedMove& mv, uint32_t i, const uint4& rec, float spd) {
+    mv.arr_rec[i] = rec;
+    if (!UNIFORM) mv.arr_spd[i] = spd;
 }
 
 // The segment's window: old slots [a0, a1) then [c0, c1), numbered from 0.
@@ -199,10 +216,12 @@ __device__ __forceinline__ void sorted_rank(const SortedMove& mv, const TableVie
             const bool mine = k < hi && b[u] - s0 < S;
             const uint32_t j = wave_rank_add(cursor, b[u] - s0, mine) - base;
             if (mine) {
+                uint4 rec;
+                float rs;
                 if (j < cap) {
                     map[j] = static_cast<uint16_t>(rel0 + (k - lo));
-                } else if (move_agent<UNIFORM, WRITE_DIR>(mv, t, k, base + j, n_ins)) {
-                    mv.arr_slot[arr_base + atomicAdd(arr_count, 1u)] = base + j;
+                } else if (move_agent<UNIFORM, WRITE_DIR>(mv, t, k, base + j, n_ins, rec, rs)) {
+                    put_arrival<UNIFORM>(mv, arr_base + atomicAdd(arr_count, 1u), rec, rs);
                 }
             }
         }
@@ -221,12 +240,14 @@ __device__ __forceinline__ void sorted_store(const SortedMove& mv, const TableVi
     const unsigned long long lt = (1ull << __lane_id()) - 1ull;
     for (uint32_t j0 = 0; j0 < count; j0 += U * T) {
         bool arr[U];
+        uint4 rec[U];
+        float rs[U];
 #pragma unroll
         for (uint32_t u = 0; u < U; u++) {
             const uint32_t j = j0 + u * T + threadIdx.x;
             arr[u] = false;
             if (j < count) {
-                arr[u] = move_agent<UNIFORM, WRITE_DIR>(mv, t, win.slot(map[j]), base + j, n_ins);
+                arr[u] = move_agent<UNIFORM, WRITE_DIR>(mv, t, win.slot(map[j]), base + j, n_ins, rec[u], rs[u]);
             }
         }
         unsigned long long am[U];
@@ -243,7 +264,7 @@ __device__ __forceinline__ void sorted_store(const SortedMove& mv, const TableVi
 #pragma unroll
             for (uint32_t u = 0; u < U; u++) {
                 if (arr[u]) {
-                    mv.arr_slot[at + static_cast<uint32_t>(__popcll(am[u] & lt))] = base + j0 + u * T + threadIdx.x;
+                    put_arrival<UNIFORM>(mv, at + static_cast<uint32_t>(__popcll(am[u] & lt)), rec[u], rs[u]);
                 }
                 at += static_cast<uint32_t>(__popcll(am[u]));
             }
@@ -349,26 +370,28 @@ static __global__ __launch_bounds__(T) void k_move_sorted(SortedMove mv, BinDev 
 }
 
 // Route choice of the arrivals listed by the sorted move: one workgroup per
-// segment (grid n_seg), its list at arr_slot[seg_base[g] ...].  An arrival
-// stands on its old target (dst.pos, stored by the move); the route choice
-// (random_move.comp:778-828) updates the caller-order road and PRNG state
-// and gives the new target, the bucket of the next move and (last tick of a
-// batch) the heading.
+// segment (grid n_seg), its list at arr_rec[seg_base[g] ...].  An arrival
+// stands on its old target (the position in its record); the route choice
+// (random_move.comp:778-828) updates the caller-order route state and gives
+// the new target, the bucket of the next move and (last tick of a batch) the
+// heading.
 template <bool UNIFORM, bool WRITE_DIR>
 __device__ __forceinline__ void route_segment_list(const SortedMove& mv, const TableView& t, uint32_t g) {
     const SortedBufs& dst = mv.dst;
     const uint32_t base = mv.seg_base[g], cnt = mv.seg_arr[g];
     for (uint32_t j = threadIdx.x; j < cnt; j += blockDim.x) {
-        const uint32_t q = mv.arr_slot[base + j];
-        const float2 at = dst.pos[q];
-        const uint32_t hh = dst.perm[q];
-        const float s = UNIFORM ? mv.spd_const : dst.spd[q];
+        const uint4 a = mv.arr_rec[base + j];
+        const uint32_t q = a.x, hh = a.y;
+        const float2 at = make_float2(__uint_as_float(a.z), __uint_as_float(a.w));
+        const float s = UNIFORM ? mv.spd_const : mv.arr_spd[base + j];
+        const uint4 w = mv.cold[2u * hh];
+        uint4 st = mv.cold[2u * hh + 1u];
         float2 nt = at;
-        uint32_t r = mv.road[hh];
-        uint4 st = mv.rng[hh];
-        choose_route(nt, r, st, mv.roads, mv.conns);
-        mv.road[hh] = r;
-        mv.rng[hh] = st;
+        uint32_t r = w.x;
+        uint2 c = make_uint2(w.y, w.z);
+        choose_route_at(nt, r, c, st, mv.roads, mv.conns);
+        mv.cold[2u * hh] = cold_word(r, c);
+        mv.cold[2u * hh + 1u] = st;
         dst.tgt[q] = nt;
         dst.nxt[q] = bucket_of_pos(next_position(at, nt, s), t);
         if (WRITE_DIR) dst.dir[q] = heading(nt, at, s);
@@ -391,6 +414,39 @@ static __global__ __launch_bounds__(kBlock) void k_sorted_next(SortedBufs st, ui
     }
     const float2 np = next_position(st.pos[i], st.tgt[i], UNIFORM ? spd_const : st.spd[i]);
     st.nxt[i] = bucket_of(cell_coord(np.x, inv_cell), cell_coord(np.y, inv_cell), lw, wmask, hmask);
+}
+
+// Route state into caller order: cold[perm[i]] from slot i's road, PRNG
+// state and target.
+static __global__ __launch_bounds__(kBlock) void k_cold_enter(const uint32_t* __restrict__ perm,
+                                                        const uint32_t* __restrict__ road,
+                                                        const uint4* __restrict__ rng,
+                                                        const float2* __restrict__ tgt,
+                                                        const msim_road* __restrict__ roads, uint4* __restrict__ cold,
+                                                        uint32_t n) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    const uint32_t h = perm[i], r = road[i];
+    Endpoint s, e;
+    load_road(roads, r, s, e);
+    cold[2u * h] = cold_word(r, route_endpoint(s, e, tgt[i]));
+    cold[2u * h + 1u] = rng[i];
+}
+
+// Route state back into slot order: road[i], rng[i] of caller perm[i].
+static __global__ __launch_bounds__(kBlock) void k_cold_publish(const uint32_t* __restrict__ perm,
+                                                          const uint4* __restrict__ cold,
+                                                          uint32_t* __restrict__ road, uint4* __restrict__ rng,
+                                                          uint32_t n) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    const uint32_t h = perm[i];
+    road[i] = cold[2u * h].x;
+    rng[i] = cold[2u * h + 1u];
 }
 
 // dst[perm[i]] = src[i]
