@@ -39,8 +39,14 @@ SortedBufs bsorted_bufs(const msim_batch* b, int k) {
     return SortedBufs{x.pos, x.tgt, b->uniform ? nullptr : x.spd, x.perm, b->nxt[k], x.dir};
 }
 
+constexpr uint32_t kBScanThreads = 1024;
+
 template <uint32_t U, bool UN, bool W>
 void blaunch_sorted(msim_batch* b, const SortedMove& mv, const BatchBin& bb) {
+    bprobe_pre(b);
+    hipLaunchKernelGGL(k_seg_scan<kBScanThreads>, dim3(1), dim3(kBScanThreads), 0, b->stream, b->seg_cnt[b->cur],
+                       b->seg_start, b->n_seg, b->n, b->faults);
+    bprobe_post(b);
     bprobe_pre(b);
     hipLaunchKernelGGL((k_bmove_sorted<U, kBSortedThreads, UN, W>), dim3(b->n_seg), dim3(kBSortedThreads),
                        bsorted_lds_bytes(b->max_seg_log), b->stream, mv, bb);
@@ -78,8 +84,10 @@ int benqueue_move_sorted(msim_batch* b, bool write_dir) {
     mv.faults = b->faults;
     mv.arr_rec = b->arr_rec;
     mv.arr_spd = b->arr_spd;
-    mv.seg_base = b->seg_base;
+    mv.seg_start = b->seg_start;
+    mv.seg_next = b->seg_cnt[nb];
     mv.seg_arr = b->seg_arr;
+    mv.n = b->n;
     BatchBin bb{};
     bb.scen = b->scen;
     bb.seg_scen = b->seg_scen;
@@ -156,18 +164,22 @@ int btick(msim_batch* b, bool write_dir) {
 
 // Enters the sorted state right after the creation's renumbering (buf[cur]
 // in the order of binning bcur): road and PRNG state into caller order, the
-// next bucket of every slot.
+// next bucket of every slot and the agent count of every segment.
 int benter_sorted(msim_batch* b) {
     const dim3 g(bblocks(b->n));
     const BBuffers& x = b->buf[b->cur];
     hipLaunchKernelGGL(k_cold_enter, g, dim3(kBlock), 0, b->stream, x.perm, x.road, x.rng, x.tgt, b->roads, b->cold,
                        b->n);
+    const dim3 gs((b->n_seg + kBlock - 1) / kBlock);
+    for (int k = 0; k < 2; k++) {
+        hipLaunchKernelGGL(k_fill_u32, gs, dim3(kBlock), 0, b->stream, b->seg_cnt[k], 0u, b->n_seg);
+    }
     if (b->uniform) {
         hipLaunchKernelGGL(k_bsorted_next<true>, g, dim3(kBlock), 0, b->stream, bsorted_bufs(b, b->cur), b->blk_scen,
-                           b->scen, b->agent_off, b->n, kShaderSpeed, b->inv_cell);
+                           b->scen, b->agent_off, b->n, kShaderSpeed, b->inv_cell, b->seg_cnt[b->cur]);
     } else {
         hipLaunchKernelGGL(k_bsorted_next<false>, g, dim3(kBlock), 0, b->stream, bsorted_bufs(b, b->cur), b->blk_scen,
-                           b->scen, b->agent_off, b->n, kShaderSpeed, b->inv_cell);
+                           b->scen, b->agent_off, b->n, kShaderSpeed, b->inv_cell, b->seg_cnt[b->cur]);
     }
     b->sorted = true;
     return bcheck(b, "sorted state set-up");
@@ -505,7 +517,9 @@ extern "C" int msim_batch_create(uint32_t n_scen, const uint32_t* road_off, cons
     BALLOC(b->bin_ticket, 1);
     BALLOC(b->seg_status, b->n_seg);
     if (b->binning == 3) {
-        BALLOC(b->seg_base, b->n_seg);
+        BALLOC(b->seg_start, b->n_seg + 1);
+        BALLOC(b->seg_cnt[0], b->n_seg);
+        BALLOC(b->seg_cnt[1], b->n_seg);
         BALLOC(b->seg_arr, b->n_seg);
     }
     BALLOC(b->part, static_cast<size_t>(n_scen) * kScenSlots * kScenSlotStride);

@@ -120,15 +120,16 @@ template <uint32_t U, uint32_t T, bool UNIFORM, bool WRITE_DIR>
 static __global__ __launch_bounds__(T) void k_bmove_sorted(SortedMove mv, BatchBin b) {
     extern __shared__ uint32_t hist[];
     __shared__ uint32_t lds[T / 64];
-    __shared__ uint32_t sh[4];
-    take_segment(b.ticket, b.n_seg, sh);
-    const uint32_t g = sh[0];
+    __shared__ uint32_t nh[2 * kNextReach + 1];
+    __shared__ uint32_t sh[1];
+    const uint32_t g = xcd_block(blockIdx.x, gridDim.x);
     const ScenDev sc = b.scen[b.seg_scen[g]];
+    const uint32_t gl = g - sc.seg_first;
+    const NextCount nc{nh, mv.seg_next, gl, sc.n_seg, sc.seg_first, sc.seg_log};
     uint32_t base, total;
-    sorted_segment<U, T, UNIFORM, WRITE_DIR>(mv, scen_view(sc, b), (g - sc.seg_first) << sc.seg_log, hist, lds,
-                                             &sh[2], b.new_start + sc.start_base, b.status, g, sc.seg_first,
-                                             sc.agent_off, sh[1], &base, &total);
-    if (g == sc.seg_first + sc.n_seg - 1 && threadIdx.x == 0) {
+    sorted_segment<U, T, UNIFORM, WRITE_DIR>(mv, scen_view(sc, b), nc, gl << sc.seg_log, g, hist, lds, sh,
+                                             b.new_start + sc.start_base, &base, &total);
+    if (gl == sc.n_seg - 1 && threadIdx.x == 0) {
         const uint32_t end = base + total;
         b.new_start[sc.start_base + sc.m] = end;
         if (end != sc.agent_off + sc.n) {
@@ -161,31 +162,39 @@ struct BatchCollide {
 // workgroup per segment, on the segment's own scenario table.
 template <bool UNIFORM, bool WRITE_DIR>
 static __global__ __launch_bounds__(kBlock) void k_broute_sorted(SortedMove mv, BatchBin b) {
-    const uint32_t g = blockIdx.x;
+    __shared__ uint32_t nh[2 * kNextReach + 1];
+    const uint32_t g = xcd_block(blockIdx.x, gridDim.x);
     const ScenDev sc = b.scen[b.seg_scen[g]];
-    route_segment_list<UNIFORM, WRITE_DIR>(mv, scen_view(sc, b), g);
+    const NextCount nc{nh, mv.seg_next, g - sc.seg_first, sc.n_seg, sc.seg_first, sc.seg_log};
+    route_segment_list<UNIFORM, WRITE_DIR>(mv, scen_view(sc, b), nc, g);
 }
 
 // nxt for every slot of every scenario: the bucket of the agent's next
-// position in its own scenario's table.
+// position in its own scenario's table; seg_next (cleared before) counts them
+// per (global) segment.
 template <bool UNIFORM>
 static __global__ __launch_bounds__(kBlock) void k_bsorted_next(SortedBufs st, const BlockScen* __restrict__ blk,
                                                                 const ScenDev* __restrict__ scen,
                                                                 const uint32_t* __restrict__ agent_off, uint32_t n,
-                                                                float spd_const, float inv_cell) {
+                                                                float spd_const, float inv_cell,
+                                                                uint32_t* __restrict__ seg_next) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) {
-        return;
+    uint32_t gseg = 0;
+    if (i < n) {
+        const BlockScen bs = blk[blockIdx.x];
+        uint32_t sid = bs.sid;
+        if (i >= bs.end) {
+            sid++;
+            while (i >= agent_off[sid + 1]) sid++;
+        }
+        const ScenDev sc = scen[sid];
+        const float2 np = next_position(st.pos[i], st.tgt[i], UNIFORM ? spd_const : st.spd[i]);
+        const uint32_t b = bucket_of(cell_coord(np.x, inv_cell), cell_coord(np.y, inv_cell), sc.lw, sc.wmask,
+                                     sc.hmask);
+        st.nxt[i] = b;
+        gseg = sc.seg_first + (b >> sc.seg_log);
     }
-    const BlockScen bs = blk[blockIdx.x];
-    uint32_t sid = bs.sid;
-    if (i >= bs.end) {
-        sid++;
-        while (i >= agent_off[sid + 1]) sid++;
-    }
-    const ScenDev sc = scen[sid];
-    const float2 np = next_position(st.pos[i], st.tgt[i], UNIFORM ? spd_const : st.spd[i]);
-    st.nxt[i] = bucket_of(cell_coord(np.x, inv_cell), cell_coord(np.y, inv_cell), sc.lw, sc.wmask, sc.hmask);
+    count_next_segment(seg_next, gseg, i < n);
 }
 
 // Per-scenario partial sums of a wave's counts: one atomic per wave when
@@ -390,7 +399,8 @@ struct msim_batch {
     uint4* cold = nullptr;                  // [2n] packed caller order: cold_word, PRNG state
     uint4* arr_rec = nullptr;               // [n] arrival records, listed per segment
     float* arr_spd = nullptr;               // [n] their speeds (per-agent speeds only)
-    uint32_t* seg_base = nullptr;           // [n_seg] first slot of each segment
+    uint32_t* seg_start = nullptr;          // [n_seg + 1] first slot of each segment (k_seg_scan)
+    uint32_t* seg_cnt[2] = {nullptr, nullptr};  // per buffer: agents per segment for the move leaving it
     uint32_t* seg_arr = nullptr;            // [n_seg] arrivals of each segment
     uint32_t bin_unroll = 4;  // window slots per rebinning thread and load round
     uint32_t tick = 1;
@@ -469,12 +479,13 @@ inline int bcheck(msim_batch* b, const char* what) {
 // Launches of a batched tick, in order, by kernel name (template arguments
 // dropped).
 constexpr int kBatchKernelMax = 5;  // largest batch_kernel_count
-inline int batch_kernel_count(const msim_batch* b) { return b->sorted ? 4 : (b->binning == 2 ? 4 : 5); }
+inline int batch_kernel_count(const msim_batch* b) { return b->sorted ? 5 : (b->binning == 2 ? 4 : 5); }
 inline const char* batch_kernel_name(const msim_batch* b, int k) {
     if (b->sorted) {
-        if (k == 0) return "k_bmove_sorted";
-        if (k == 1) return "k_broute_sorted";
-        return k == 2 ? (b->collide_mode == 1 ? "k_bcollide" : "k_bcollide_staged") : "k_bfinish";
+        if (k == 0) return "k_seg_scan";
+        if (k == 1) return "k_bmove_sorted";
+        if (k == 2) return "k_broute_sorted";
+        return k == 3 ? (b->collide_mode == 1 ? "k_bcollide" : "k_bcollide_staged") : "k_bfinish";
     }
     if (k == 0) return "k_advance4";
     if (b->binning == 2) {

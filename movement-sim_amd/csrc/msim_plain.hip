@@ -76,7 +76,8 @@ struct msim_sim {
     uint32_t* ncount_caller = nullptr;      // [n] neighbour counts kept in caller order
     uint4* arr_rec = nullptr;               // [n] arrival records, listed per segment
     float* arr_spd = nullptr;               // [n] their speeds (per-agent speeds only)
-    uint32_t* seg_base = nullptr;           // [n_seg] first slot of each segment
+    uint32_t* seg_start = nullptr;          // [n_seg + 1] first slot of each segment (k_seg_scan)
+    uint32_t* seg_cnt[2] = {nullptr, nullptr};  // per buffer: agents per segment for the move leaving it
     uint32_t* seg_arr = nullptr;            // [n_seg] arrivals of each segment
     bool mark_move = false;  // capture: record ev_move right after the move kernel
     int ncount_at = 0;  // buffer whose slot order ncount follows (-1: ncount_caller)
@@ -156,15 +157,16 @@ const char* const kAtomicKernels[] = {"k_scan_reduce", "k_scan_partials", "k_sca
 const char* const kFusedKernels[] = {"k_bin_fused"};
 uint32_t bin_kernels(const msim_sim* s) { return s->binning == 1 ? 4u : (s->binning == 2 ? 1u : 2u); }
 uint32_t kernels_per_tick(const msim_sim* s) {
-    return s->binning == 3 ? 4u : (s->route_inline ? 1u : 2u) + bin_kernels(s) + 2u;
+    return s->binning == 3 ? 5u : (s->route_inline ? 1u : 2u) + bin_kernels(s) + 2u;
 }
 // Name of launch k of a fused tick (the kernel's name, template arguments
 // dropped).
 const char* tick_kernel_name(const msim_sim* s, uint32_t k) {
     if (s->binning == 3) {
-        if (k == 0) return "k_move_sorted";
-        if (k == 1) return "k_route_sorted";
-        return k == 2 ? (s->collide_mode == 1 ? "k_collide" : "k_collide_staged") : "k_collide_finish";
+        if (k == 0) return "k_seg_scan";
+        if (k == 1) return "k_move_sorted";
+        if (k == 2) return "k_route_sorted";
+        return k == 3 ? (s->collide_mode == 1 ? "k_collide" : "k_collide_staged") : "k_collide_finish";
     }
     if (k == 0) return (s->route_inline && s->binning != 1) ? "k_advance4" : "k_advance";
     if (!s->route_inline) {
@@ -562,12 +564,16 @@ int enter_sorted(msim_sim* s) {
     hipLaunchKernelGGL(k_cold_enter, g, dim3(kBlock), 0, s->stream, b.perm, b.road, b.rng, b.tgt, s->roads, s->cold,
                        s->n);
     const GridDev& gd = s->grid;
+    const dim3 gs((s->n_seg + kBlock - 1) / kBlock);
+    for (int k = 0; k < 2; k++) {
+        hipLaunchKernelGGL(k_fill_u32, gs, dim3(kBlock), 0, s->stream, s->seg_cnt[k], 0u, s->n_seg);
+    }
     if (s->uniform) {
         hipLaunchKernelGGL(k_sorted_next<true>, g, dim3(kBlock), 0, s->stream, sorted_bufs(s, s->cur), s->n,
-                           s->spd_const, gd.inv_cell, gd.lw, gd.wmask, gd.hmask);
+                           s->spd_const, gd.inv_cell, gd.lw, gd.wmask, gd.hmask, s->seg_log, s->seg_cnt[s->cur]);
     } else {
         hipLaunchKernelGGL(k_sorted_next<false>, g, dim3(kBlock), 0, s->stream, sorted_bufs(s, s->cur), s->n,
-                           s->spd_const, gd.inv_cell, gd.lw, gd.wmask, gd.hmask);
+                           s->spd_const, gd.inv_cell, gd.lw, gd.wmask, gd.hmask, s->seg_log, s->seg_cnt[s->cur]);
     }
     s->ncount_at = s->cur;
     s->sorted = true;
@@ -581,8 +587,14 @@ int ensure_sorted(msim_sim* s) {
     return MSIM_OK;
 }
 
+constexpr uint32_t kScanThreads = 1024;
+
 template <uint32_t U, bool UN, bool W>
 void launch_sorted(msim_sim* s, const SortedMove& mv, const BinDev& b) {
+    probe_pre(s);
+    hipLaunchKernelGGL(k_seg_scan<kScanThreads>, dim3(1), dim3(kScanThreads), 0, s->stream, s->seg_cnt[s->cur],
+                       s->seg_start, s->n_seg, s->n, &s->ctr->faults);
+    probe_post(s);
     probe_pre(s);
     hipLaunchKernelGGL((k_move_sorted<U, kSortedThreads, UN, W>), dim3(s->n_seg), dim3(kSortedThreads),
                        sorted_lds_bytes(s->seg_log), s->stream, mv, b);
@@ -626,8 +638,10 @@ int enqueue_move_sorted(msim_sim* s, bool write_dir) {
     mv.faults = &s->ctr->faults;
     mv.arr_rec = s->arr_rec;
     mv.arr_spd = s->arr_spd;
-    mv.seg_base = s->seg_base;
+    mv.seg_start = s->seg_start;
+    mv.seg_next = s->seg_cnt[nb];
     mv.seg_arr = s->seg_arr;
+    mv.n = s->n;
     BinDev b{};
     b.old_start = s->bstart[s->bcur];
     b.new_start = s->bstart[1 - s->bcur];
@@ -1029,7 +1043,9 @@ extern "C" int msim_create(const msim_road* roads, uint32_t n_roads, const uint3
         ALLOC(s->ncount_caller, n);
         ALLOC(s->arr_rec, n);
         if (!s->uniform) ALLOC(s->arr_spd, n);
-        ALLOC(s->seg_base, s->n_seg);
+        ALLOC(s->seg_start, s->n_seg + 1);
+        ALLOC(s->seg_cnt[0], s->n_seg);
+        ALLOC(s->seg_cnt[1], s->n_seg);
         ALLOC(s->seg_arr, s->n_seg);
     }
     ALLOC(s->seg_total, s->n_seg);

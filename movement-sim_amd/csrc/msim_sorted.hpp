@@ -6,17 +6,20 @@
 // every agent (random_move.comp:830-852) and writes it, together with the
 // rest of its hot state, straight into the next binning's slot order:
 //
-//   * the toroidal bucket table is cut into segments; workgroup `seg` (taken
-//     by ticket, as in the one-pass rebinning) owns the slots of the agents
-//     whose new bucket lies in its segment;
+//   * the toroidal bucket table is cut into segments; workgroup `seg` owns
+//     the slots of the agents whose new bucket lies in its segment (segments
+//     dealt to the XCDs in contiguous runs, so that neighbouring windows
+//     share an L2);
 //   * an agent moves at most D cells per axis, so it lands within `margin`
 //     buckets of its previous bucket: the segment's agents all sit in a
 //     window of the previous slot order;
 //   * count pass: every window slot carries `nxt`, the bucket of the agent's
 //     next position, computed when the agent was last placed; counting reads
 //     these 4 bytes per slot and no state;
-//   * look-back over the earlier segments' totals gives the segment's first
-//     slot;
+//   * the segment's first slot is known before the launch: the previous
+//     move and route choice counted, per segment, the agents whose next
+//     bucket lies in it, and k_seg_scan turned those counts into first slots,
+//     so no segment waits for another;
 //   * place pass: the window slots whose `nxt` lies in the segment load
 //     their state (coalesced: the window is a contiguous slot range), move,
 //     and are stored at their new slots with the bucket of their following
@@ -85,11 +88,13 @@ __device__ __forceinline__ uint4 cold_word(uint32_t road, uint2 c) { return make
 struct SortedMove {
     SortedBufs src, dst;
     uint4* arr_rec;      // [n] arrivals {new slot, caller index, x bits, y bits}, segment g's at
-                         // [seg_base[g], + seg_arr[g])
+                         // [seg_start[g], + seg_arr[g])
     float* arr_spd;      // [n] their speeds; nullptr with spd_const
-    uint32_t* seg_base;  // [n_seg] first slot of each segment
-    uint32_t* seg_arr;   // [n_seg] arrivals of each segment
-    uint4* cold;         // [2 n] caller order: cold_word, PRNG state
+    const uint32_t* seg_start;  // [n_seg + 1] first slot of each segment (k_seg_scan)
+    uint32_t* seg_next;         // [n_seg] agents whose next bucket lies in each segment, for the next move
+    uint32_t* seg_arr;          // [n_seg] arrivals of each segment
+    uint4* cold;                // [2 n] caller order: cold_word, PRNG state
+    uint32_t n;
     const msim_road* roads;
     const uint32_t* conns;
     float spd_const;
@@ -98,6 +103,50 @@ struct SortedMove {
 };
 
 enum : unsigned int { kFaultNext = 4u };  // an agent's position left the bucket its count used
+
+// Per-workgroup counts of next segments.  A workgroup works on segment
+// g_local of a table of n_local segments (global index seg_first + g_local);
+// its agents' next buckets lie within the window margin of that segment, so
+// nh[d + kNextReach] counts those d segments away (cyclically) and farther
+// ones go straight to seg_next.  next_flush adds nh to seg_next.
+constexpr int kNextReach = 2;
+struct NextCount {
+    uint32_t* nh;  // LDS [2 kNextReach + 1]
+    uint32_t* seg_next;
+    uint32_t g_local, n_local, seg_first, seg_log;
+};
+
+// Wave-level (every lane of the wave calls it; inactive ones with
+// active = false): counts the next buckets of the active lanes.
+__device__ __forceinline__ void next_add(const NextCount& nc, uint32_t bucket, bool active) {
+    const uint32_t ls = bucket >> nc.seg_log;
+    const bool own = active && ls == nc.g_local;
+    const unsigned long long om = __ballot(own);
+    if (own && static_cast<int>(__lane_id()) == __ffsll(static_cast<long long>(om)) - 1) {
+        atomicAdd(&nc.nh[kNextReach], static_cast<uint32_t>(__popcll(om)));
+    }
+    if (active && !own) {
+        int d = static_cast<int>(ls) - static_cast<int>(nc.g_local);
+        const int nl = static_cast<int>(nc.n_local);
+        if (2 * d > nl) d -= nl;
+        if (2 * d < -nl) d += nl;
+        if (d >= -kNextReach && d <= kNextReach) {
+            atomicAdd(&nc.nh[d + kNextReach], 1u);
+        } else {
+            atomicAdd(&nc.seg_next[nc.seg_first + ls], 1u);
+        }
+    }
+}
+
+// Workgroup-level, after a barrier: nh into seg_next.
+__device__ __forceinline__ void next_flush(const NextCount& nc) {
+    const int k = static_cast<int>(threadIdx.x);
+    if (k < 2 * kNextReach + 1 && nc.nh[k]) {
+        const int nl = static_cast<int>(nc.n_local);
+        const int l = ((static_cast<int>(nc.g_local) + k - kNextReach) % nl + nl) % nl;
+        atomicAdd(&nc.seg_next[nc.seg_first + static_cast<uint32_t>(l)], nc.nh[k]);
+    }
+}
 
 // Count pass over window slots [lo, hi): agents whose next bucket lies in
 // [s0, s0 + S) bump their LDS counter.
@@ -126,7 +175,7 @@ __device__ __forceinline__ void sorted_count(const uint32_t* __restrict__ nxt, u
 // arrived; n_ins counts a cell change.
 template <bool UNIFORM, bool WRITE_DIR>
 __device__ __forceinline__ bool move_agent(const SortedMove& mv, const TableView& t, uint32_t k, uint32_t q,
-                                           uint32_t& n_ins, uint4& rec, float& rec_spd) {
+                                           uint32_t& n_ins, uint4& rec, float& rec_spd, uint32_t& next_bucket) {
     const SortedBufs& src = mv.src;
     const SortedBufs& dst = mv.dst;
     const uint32_t b = src.nxt[k];
@@ -153,8 +202,9 @@ __device__ __forceinline__ bool move_agent(const SortedMove& mv, const TableView
     if (!UNIFORM) dst.spd[q] = sp;
     dst.perm[q] = h;
     if (!arr) {
+        next_bucket = bucket_of_pos(next_position(np, tg, sp), t);
         dst.tgt[q] = tg;
-        dst.nxt[q] = bucket_of_pos(next_position(np, tg, sp), t);
+        dst.nxt[q] = next_bucket;
         if (WRITE_DIR) dst.dir[q] = d;
     }
     rec = make_uint4(q, h, __float_as_uint(np.x), __float_as_uint(np.y));
@@ -180,15 +230,17 @@ struct Window {
 // each agent whose next bucket lies in the segment takes cursor[bucket]++,
 // its new slot.  The segment's output position j = slot - base records the
 // agent's window index in map[j] when j < cap; past cap (a segment more
-// crowded than the map holds) the agent is moved and stored at once.  Reads
+// crowded than the map holds) the agent is moved and stored at once.  A
+// position at or past limit (the slots the segment was given) is a count
+// mismatch: the agent is not stored and kFaultBinCount is raised.  Reads
 // only the next buckets; the next round's are loaded before this round is
 // ranked.  Waves run the rounds independently (the rank helper needs whole
 // waves, no workgroup barrier).
 template <uint32_t U, uint32_t T, bool UNIFORM, bool WRITE_DIR>
-__device__ __forceinline__ void sorted_rank(const SortedMove& mv, const TableView& t, uint32_t s0, uint32_t lo,
-                                            uint32_t hi, uint32_t rel0, uint32_t* cursor, uint16_t* map,
-                                            uint32_t base, uint32_t cap, uint32_t* arr_count, uint32_t arr_base,
-                                            uint32_t& n_ins) {
+__device__ __forceinline__ void sorted_rank(const SortedMove& mv, const TableView& t, const NextCount& nc,
+                                            uint32_t s0, uint32_t lo, uint32_t hi, uint32_t rel0, uint32_t* cursor,
+                                            uint16_t* map, uint32_t base, uint32_t cap, uint32_t limit,
+                                            uint32_t* arr_count, uint32_t arr_base, uint32_t& n_ins) {
     if (lo >= hi) {
         return;
     }
@@ -215,15 +267,22 @@ __device__ __forceinline__ void sorted_rank(const SortedMove& mv, const TableVie
             const uint32_t k = r0 + u * T + threadIdx.x;
             const bool mine = k < hi && b[u] - s0 < S;
             const uint32_t j = wave_rank_add(cursor, b[u] - s0, mine) - base;
+            bool moved = false, arr = false;
+            uint32_t nb = 0;
             if (mine) {
                 uint4 rec;
                 float rs;
-                if (j < cap) {
+                if (j >= limit) {
+                    atomicOr(mv.faults, kFaultBinCount);
+                } else if (j < cap) {
                     map[j] = static_cast<uint16_t>(rel0 + (k - lo));
-                } else if (move_agent<UNIFORM, WRITE_DIR>(mv, t, k, base + j, n_ins, rec, rs)) {
-                    put_arrival<UNIFORM>(mv, arr_base + atomicAdd(arr_count, 1u), rec, rs);
+                } else {
+                    moved = true;
+                    arr = move_agent<UNIFORM, WRITE_DIR>(mv, t, k, base + j, n_ins, rec, rs, nb);
+                    if (arr) put_arrival<UNIFORM>(mv, arr_base + atomicAdd(arr_count, 1u), rec, rs);
                 }
             }
+            next_add(nc, nb, moved && !arr);
         }
     }
 }
@@ -234,8 +293,8 @@ __device__ __forceinline__ void sorted_rank(const SortedMove& mv, const TableVie
 // output line is written whole.  Arrivals are listed for k_route_sorted (one
 // LDS atomic per wave and round).
 template <uint32_t U, uint32_t T, bool UNIFORM, bool WRITE_DIR>
-__device__ __forceinline__ void sorted_store(const SortedMove& mv, const TableView& t, const Window& win,
-                                             const uint16_t* map, uint32_t base, uint32_t count,
+__device__ __forceinline__ void sorted_store(const SortedMove& mv, const TableView& t, const NextCount& nc,
+                                             const Window& win, const uint16_t* map, uint32_t base, uint32_t count,
                                              uint32_t* arr_count, uint32_t arr_base, uint32_t& n_ins) {
     const unsigned long long lt = (1ull << __lane_id()) - 1ull;
     for (uint32_t j0 = 0; j0 < count; j0 += U * T) {
@@ -246,9 +305,11 @@ __device__ __forceinline__ void sorted_store(const SortedMove& mv, const TableVi
         for (uint32_t u = 0; u < U; u++) {
             const uint32_t j = j0 + u * T + threadIdx.x;
             arr[u] = false;
+            uint32_t nb = 0;
             if (j < count) {
-                arr[u] = move_agent<UNIFORM, WRITE_DIR>(mv, t, win.slot(map[j]), base + j, n_ins, rec[u], rs[u]);
+                arr[u] = move_agent<UNIFORM, WRITE_DIR>(mv, t, win.slot(map[j]), base + j, n_ins, rec[u], rs[u], nb);
             }
+            next_add(nc, nb, j < count && !arr[u]);
         }
         unsigned long long am[U];
         uint32_t round_arr = 0;
@@ -276,20 +337,26 @@ __device__ __forceinline__ void sorted_store(const SortedMove& mv, const TableVi
 // new-slot -> window-index map (2 bytes per slot, 2 slots per bucket).
 __host__ __device__ constexpr size_t sorted_lds_bytes(uint32_t seg_log) { return size_t{8} << seg_log; }
 
-// Count, look back, place.  Writes the segment's absolute bucket offsets to
-// start_out[s0 + j]; *base_out / *total_out receive the segment's first slot
-// and agent count (all threads).  The segment's insert and arrival counts go
-// to the statistics slots.
+// Count and place segment g (global index; s0 its first bucket in table t).
+// Writes the segment's absolute bucket offsets to start_out[s0 + j];
+// *base_out / *total_out receive the segment's first slot and agent count
+// (all threads).  The segment's insert and arrival counts go to the
+// statistics slots; its agents' next segments are counted for the next move.
 template <uint32_t U, uint32_t T, bool UNIFORM, bool WRITE_DIR>
-__device__ __forceinline__ void sorted_segment(const SortedMove& mv, const TableView& t, uint32_t s0, uint32_t* hist,
-                                               uint32_t* lds, uint32_t* sh_base,
-                                               uint32_t* __restrict__ start_out, unsigned long long* status,
-                                               uint32_t g, uint32_t g_first, uint32_t first, uint32_t epoch,
-                                               uint32_t* base_out, uint32_t* total_out) {
+__device__ __forceinline__ void sorted_segment(const SortedMove& mv, const TableView& t, const NextCount& nc,
+                                               uint32_t s0, uint32_t g, uint32_t* hist, uint32_t* lds,
+                                               uint32_t* sh, uint32_t* __restrict__ start_out, uint32_t* base_out,
+                                               uint32_t* total_out) {
     const uint32_t tid = threadIdx.x;
     const uint32_t S = 1u << t.seg_log;
     for (uint32_t j = tid; j < S; j += T) {
         hist[j] = 0u;
+    }
+    if (tid < 2 * kNextReach + 1) {
+        nc.nh[tid] = 0u;
+    }
+    if (tid == 0) {
+        sh[0] = 0u;  // the segment's arrivals so far
     }
     __syncthreads();
     uint32_t a0, a1, c0, c1;
@@ -304,49 +371,46 @@ __device__ __forceinline__ void sorted_segment(const SortedMove& mv, const Table
     }
     uint32_t total = 0;
     uint32_t excl = block_excl_scan<T>(run, lds, total);
-    for (uint32_t q = 0; q < per; q++) {  // counts -> segment-local exclusive offsets
+    // the slots this segment was given, clamped to [0, n): counts that went
+    // wrong raise a fault but never place an agent outside the state
+    const uint32_t base = min(mv.seg_start[g], mv.n);
+    const uint32_t end = max(base, min(mv.seg_start[g + 1], mv.n));
+    const uint32_t limit = end - base;
+    if (tid == 0 && total != limit) {
+        atomicOr(mv.faults, kFaultBinCount);
+    }
+    for (uint32_t q = 0; q < per; q++) {  // counts -> absolute bucket offsets
         const uint32_t v = hist[tid * per + q];
-        hist[tid * per + q] = excl;
+        hist[tid * per + q] = base + excl;
+        start_out[s0 + tid * per + q] = base + min(excl, limit);
         excl += v;
-    }
-    if (tid == 0) {
-        *sh_base = seg_lookback(status, g, g_first, first, total, epoch, mv.faults);
-    }
-    __syncthreads();
-    const uint32_t base = *sh_base;
-    for (uint32_t j = tid; j < S; j += T) {
-        const uint32_t v = base + hist[j];
-        hist[j] = v;
-        start_out[s0 + j] = v;
     }
     __syncthreads();
     uint32_t n_ins = 0;
-    if (tid == 0) {
-        sh_base[1] = 0u;  // the segment's arrivals so far
-    }
-    __syncthreads();
     const Window win{a0, a1, c0, c1};
     uint16_t* map = reinterpret_cast<uint16_t*>(hist + S);
     // window indices must fit the map's 16 bits; a window past that moves
     // every agent straight from the rank pass
     const uint32_t cap = win.size() <= 65536u ? 2u * S : 0u;
-    sorted_rank<U, T, UNIFORM, WRITE_DIR>(mv, t, s0, a0, a1, 0u, hist, map, base, cap, &sh_base[1], base, n_ins);
-    sorted_rank<U, T, UNIFORM, WRITE_DIR>(mv, t, s0, c0, c1, a1 - a0, hist, map, base, cap, &sh_base[1], base,
+    sorted_rank<U, T, UNIFORM, WRITE_DIR>(mv, t, nc, s0, a0, a1, 0u, hist, map, base, cap, limit, &sh[0], base,
                                           n_ins);
+    sorted_rank<U, T, UNIFORM, WRITE_DIR>(mv, t, nc, s0, c0, c1, a1 - a0, hist, map, base, cap, limit, &sh[0],
+                                          base, n_ins);
     __syncthreads();
-    sorted_store<U, T, UNIFORM, WRITE_DIR>(mv, t, win, map, base, min(total, cap), &sh_base[1], base, n_ins);
+    sorted_store<U, T, UNIFORM, WRITE_DIR>(mv, t, nc, win, map, base, min(min(total, cap), limit), &sh[0], base,
+                                           n_ins);
     __syncthreads();
+    next_flush(nc);
     // block totals of insertions and arrivals, one slot update each
     const uint32_t si = block_sum<T>(n_ins, lds);
     if (tid == 0) {
-        const uint32_t sa = sh_base[1];
+        const uint32_t sa = sh[0];
         slot_add(mv.part_stats, g, kWordInserts, si);
         slot_add(mv.part_stats, g, kWordArrivals, sa);
-        mv.seg_base[g] = base;
         mv.seg_arr[g] = sa;
     }
     *base_out = base;
-    *total_out = total;
+    *total_out = limit;
 }
 
 // Sorted move of one table (grid: n_seg workgroups; dynamic LDS:
@@ -355,65 +419,151 @@ template <uint32_t U, uint32_t T, bool UNIFORM, bool WRITE_DIR>
 static __global__ __launch_bounds__(T) void k_move_sorted(SortedMove mv, BinDev b) {
     extern __shared__ uint32_t hist[];
     __shared__ uint32_t lds[T / 64];
-    __shared__ uint32_t sh[4];
-    take_segment(b.ticket, b.n_seg, sh);
-    const uint32_t seg = sh[0];
+    __shared__ uint32_t nh[2 * kNextReach + 1];
+    __shared__ uint32_t sh[1];
+    const uint32_t seg = xcd_block(blockIdx.x, gridDim.x);
+    const NextCount nc{nh, mv.seg_next, seg, b.n_seg, 0u, b.seg_log};
     uint32_t base, total;
-    sorted_segment<U, T, UNIFORM, WRITE_DIR>(mv, table_view(b), seg << b.seg_log, hist, lds, &sh[2], b.new_start,
-                                             b.status, seg, 0u, 0u, sh[1], &base, &total);
+    sorted_segment<U, T, UNIFORM, WRITE_DIR>(mv, table_view(b), nc, seg << b.seg_log, seg, hist, lds, sh,
+                                             b.new_start, &base, &total);
     if (seg == b.n_seg - 1 && threadIdx.x == 0) {
         b.new_start[b.m] = base + total;
-        if (base + total != b.n) {
-            atomicOr(mv.faults, kFaultBinCount);
+    }
+}
+
+// First slot of every segment from the per-segment agent counts (the
+// segments of all tables in order): seg_start[g] = cnt[0] + ... + cnt[g - 1],
+// seg_start[n_seg] = the total, which must be n.  cnt is cleared for its next
+// use.  One workgroup: wave w takes the contiguous chunk [w C, (w + 1) C) and
+// walks it 64 entries at a time (coalesced loads and stores), once to sum it
+// and once, after a scan of the chunk sums, to write the offsets.
+template <uint32_t T>
+static __global__ __launch_bounds__(T) void k_seg_scan(uint32_t* __restrict__ cnt, uint32_t* __restrict__ seg_start,
+                                                       uint32_t n_seg, uint32_t n, unsigned int* faults) {
+    constexpr uint32_t W = T / 64;
+    __shared__ uint32_t wsum[W];
+    const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
+    const uint32_t C = (n_seg + W - 1) / W;
+    const uint32_t c0 = min(w * C, n_seg), c1 = min(c0 + C, n_seg);
+    uint32_t part = 0;
+    for (uint32_t g = c0 + lane; g < c1; g += 64) {
+        part += cnt[g];
+    }
+    part = wave_sum(part);
+    if (lane == 0) wsum[w] = part;
+    __syncthreads();
+    uint32_t run = 0, total = 0;
+    for (uint32_t k = 0; k < W; k++) {
+        run += (k < w) ? wsum[k] : 0u;
+        total += wsum[k];
+    }
+    for (uint32_t g0 = c0; g0 < c1; g0 += 64) {
+        const uint32_t g = g0 + lane;
+        const uint32_t v = g < c1 ? cnt[g] : 0u;
+        const uint32_t inc = wave_incl_scan(v);
+        if (g < c1) {
+            seg_start[g] = run + inc - v;
+            cnt[g] = 0u;
+        }
+        run += static_cast<uint32_t>(__shfl(static_cast<int>(inc), 63, 64));
+    }
+    if (threadIdx.x == 0) {
+        seg_start[n_seg] = total;
+        if (total != n) {
+            atomicOr(faults, kFaultBinCount);
         }
     }
 }
 
 // Route choice of the arrivals listed by the sorted move: one workgroup per
-// segment (grid n_seg), its list at arr_rec[seg_base[g] ...].  An arrival
+// segment (grid n_seg), its list at arr_rec[seg_start[g] ...].  An arrival
 // stands on its old target (the position in its record); the route choice
 // (random_move.comp:778-828) updates the caller-order route state and gives
 // the new target, the bucket of the next move and (last tick of a batch) the
 // heading.
 template <bool UNIFORM, bool WRITE_DIR>
-__device__ __forceinline__ void route_segment_list(const SortedMove& mv, const TableView& t, uint32_t g) {
+__device__ __forceinline__ void route_segment_list(const SortedMove& mv, const TableView& t, const NextCount& nc,
+                                                   uint32_t g) {
     const SortedBufs& dst = mv.dst;
-    const uint32_t base = mv.seg_base[g], cnt = mv.seg_arr[g];
-    for (uint32_t j = threadIdx.x; j < cnt; j += blockDim.x) {
-        const uint4 a = mv.arr_rec[base + j];
-        const uint32_t q = a.x, hh = a.y;
-        const float2 at = make_float2(__uint_as_float(a.z), __uint_as_float(a.w));
-        const float s = UNIFORM ? mv.spd_const : mv.arr_spd[base + j];
-        const uint4 w = mv.cold[2u * hh];
-        uint4 st = mv.cold[2u * hh + 1u];
-        float2 nt = at;
-        uint32_t r = w.x;
-        uint2 c = make_uint2(w.y, w.z);
-        choose_route_at(nt, r, c, st, mv.roads, mv.conns);
-        mv.cold[2u * hh] = cold_word(r, c);
-        mv.cold[2u * hh + 1u] = st;
-        dst.tgt[q] = nt;
-        dst.nxt[q] = bucket_of_pos(next_position(at, nt, s), t);
-        if (WRITE_DIR) dst.dir[q] = heading(nt, at, s);
+    if (threadIdx.x < 2 * kNextReach + 1) {
+        nc.nh[threadIdx.x] = 0u;
     }
+    __syncthreads();
+    const uint32_t base = min(mv.seg_start[g], mv.n), cnt = mv.seg_arr[g];
+    for (uint32_t j0 = 0; j0 < cnt; j0 += blockDim.x) {  // whole-block rounds: next_add is wave-level
+        const uint32_t j = j0 + threadIdx.x;
+        uint32_t nb = 0;
+        if (j < cnt) {
+            const uint4 a = mv.arr_rec[base + j];
+            const uint32_t q = a.x, hh = a.y;
+            const float2 at = make_float2(__uint_as_float(a.z), __uint_as_float(a.w));
+            const float s = UNIFORM ? mv.spd_const : mv.arr_spd[base + j];
+            const uint4 w = mv.cold[2u * hh];
+            uint4 st = mv.cold[2u * hh + 1u];
+            float2 nt = at;
+            uint32_t r = w.x;
+            uint2 c = make_uint2(w.y, w.z);
+            choose_route_at(nt, r, c, st, mv.roads, mv.conns);
+            mv.cold[2u * hh] = cold_word(r, c);
+            mv.cold[2u * hh + 1u] = st;
+            nb = bucket_of_pos(next_position(at, nt, s), t);
+            dst.tgt[q] = nt;
+            dst.nxt[q] = nb;
+            if (WRITE_DIR) dst.dir[q] = heading(nt, at, s);
+        }
+        next_add(nc, nb, j < cnt);
+    }
+    __syncthreads();
+    next_flush(nc);
 }
 
 template <bool UNIFORM, bool WRITE_DIR>
 static __global__ __launch_bounds__(kBlock) void k_route_sorted(SortedMove mv, BinDev b) {
-    route_segment_list<UNIFORM, WRITE_DIR>(mv, table_view(b), blockIdx.x);
+    __shared__ uint32_t nh[2 * kNextReach + 1];
+    const uint32_t g = xcd_block(blockIdx.x, gridDim.x);
+    route_segment_list<UNIFORM, WRITE_DIR>(mv, table_view(b), NextCount{nh, mv.seg_next, g, b.n_seg, 0u, b.seg_log},
+                                           g);
 }
 
-// nxt for every slot of one table: the bucket of the agent's next position.
+// Wave-level: one count per distinct segment among the active lanes' gseg
+// (the lanes of a wave hold consecutive slots in bucket order, so there are
+// few).
+__device__ __forceinline__ void count_next_segment(uint32_t* seg_next, uint32_t gseg, bool active) {
+    unsigned long long todo = __ballot(active);
+    while (todo) {
+        const int leader = __ffsll(static_cast<long long>(todo)) - 1;
+        const uint32_t k = static_cast<uint32_t>(__shfl(static_cast<int>(gseg), leader, 64));
+        const unsigned long long peers = __ballot(active && gseg == k) & todo;
+        if (static_cast<int>(__lane_id()) == leader) {
+            atomicAdd(&seg_next[k], static_cast<uint32_t>(__popcll(peers)));
+        }
+        todo &= ~peers;
+    }
+}
+
+// nxt for every slot of one table: the bucket of the agent's next position;
+// seg_next (cleared before) counts them per segment.
 template <bool UNIFORM>
 static __global__ __launch_bounds__(kBlock) void k_sorted_next(SortedBufs st, uint32_t n, float spd_const,
                                                                float inv_cell, uint32_t lw, uint32_t wmask,
-                                                               uint32_t hmask) {
+                                                               uint32_t hmask, uint32_t seg_log,
+                                                               uint32_t* __restrict__ seg_next) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) {
-        return;
+    uint32_t b = 0;
+    if (i < n) {
+        const float2 np = next_position(st.pos[i], st.tgt[i], UNIFORM ? spd_const : st.spd[i]);
+        b = bucket_of(cell_coord(np.x, inv_cell), cell_coord(np.y, inv_cell), lw, wmask, hmask);
+        st.nxt[i] = b;
     }
-    const float2 np = next_position(st.pos[i], st.tgt[i], UNIFORM ? spd_const : st.spd[i]);
-    st.nxt[i] = bucket_of(cell_coord(np.x, inv_cell), cell_coord(np.y, inv_cell), lw, wmask, hmask);
+    count_next_segment(seg_next, b >> seg_log, i < n);
+}
+
+// p[0, n) = v (in graphs too, where memset nodes are avoided).
+static __global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, uint32_t n) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) {
+        p[i] = v;
+    }
 }
 
 // Route state into caller order: cold[perm[i]] from slot i's road, PRNG

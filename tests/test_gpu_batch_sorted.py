@@ -21,7 +21,7 @@ SORTED = dict(binning=3)
 def test_batch_sorted_equals_plain_per_scenario():
     scs = S.batch_config(12, scale=0.02)
     b = check_against_plain(scs, 230, **SORTED)
-    assert b.kernel_names() == ["k_bmove_sorted", "k_broute_sorted", "k_bcollide_staged", "k_bfinish"]
+    assert b.kernel_names() == ["k_seg_scan", "k_bmove_sorted", "k_broute_sorted", "k_bcollide_staged", "k_bfinish"]
     b.sync()
 
 

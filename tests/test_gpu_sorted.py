@@ -38,7 +38,7 @@ def test_star_map_reference_protocol_every_dispatch_sorted():
     assert total == o.pairs_total
     assert hist.tolist() == o.pair_hist[-len(hist):]
     assert g.debug_data()[1] == o.pairs_total % 2**32
-    assert g.kernel_names() == ["k_move_sorted", "k_route_sorted", "k_collide_staged", "k_collide_finish"]
+    assert g.kernel_names() == ["k_seg_scan", "k_move_sorted", "k_route_sorted", "k_collide_staged", "k_collide_finish"]
 
 
 def test_config1_full_parity_1000_ticks_sorted():
