@@ -349,6 +349,7 @@ __device__ __forceinline__ void sorted_segment(const SortedMove& mv, const Table
                                                uint32_t* total_out) {
     const uint32_t tid = threadIdx.x;
     const uint32_t S = 1u << t.seg_log;
+    const uint32_t st0 = mv.seg_start[g], st1 = mv.seg_start[g + 1];  // loaded while the window is counted
     for (uint32_t j = tid; j < S; j += T) {
         hist[j] = 0u;
     }
@@ -361,8 +362,35 @@ __device__ __forceinline__ void sorted_segment(const SortedMove& mv, const Table
     __syncthreads();
     uint32_t a0, a1, c0, c1;
     bin_window(t, s0, a0, a1, c0, c1);
-    sorted_count<2 * U, T>(mv.src.nxt, s0, S, a0, a1, hist);
-    sorted_count<2 * U, T>(mv.src.nxt, s0, S, c0, c1, hist);
+    const Window win{a0, a1, c0, c1};
+    const uint32_t wsize = win.size();
+    // A window of at most R T slots is counted in one round whose LDS
+    // atomics also give each agent its rank in its bucket (kept in
+    // registers, so no rank pass is needed); larger windows are counted,
+    // then ranked, in rounds.
+    constexpr uint32_t R = 2 * U;
+    const bool one_round = wsize <= R * T;  // workgroup-uniform
+    // (bucket offset << 16) | rank in the bucket, ~0 when not the segment's
+    // agent (segments have at most 2^14 buckets, windows here R T <= 2^13 slots)
+    uint32_t pk[R];
+    if (one_round) {
+        uint32_t b[R];
+#pragma unroll
+        for (uint32_t r = 0; r < R; r++) {
+            const uint32_t idx = r * T + tid;
+            b[r] = idx < wsize ? mv.src.nxt[win.slot(idx)] : 0u;
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < R; r++) {
+            const uint32_t off = b[r] - s0;
+            const bool mine = r * T + tid < wsize && off < S;
+            const uint32_t rank = wave_rank_add(hist, off, mine);
+            pk[r] = mine ? (off << 16) | rank : ~0u;
+        }
+    } else {
+        sorted_count<2 * U, T>(mv.src.nxt, s0, S, a0, a1, hist);
+        sorted_count<2 * U, T>(mv.src.nxt, s0, S, c0, c1, hist);
+    }
     __syncthreads();
     const uint32_t per = S / T;
     uint32_t run = 0;
@@ -373,8 +401,8 @@ __device__ __forceinline__ void sorted_segment(const SortedMove& mv, const Table
     uint32_t excl = block_excl_scan<T>(run, lds, total);
     // the slots this segment was given, clamped to [0, n): counts that went
     // wrong raise a fault but never place an agent outside the state
-    const uint32_t base = min(mv.seg_start[g], mv.n);
-    const uint32_t end = max(base, min(mv.seg_start[g + 1], mv.n));
+    const uint32_t base = min(st0, mv.n);
+    const uint32_t end = max(base, min(st1, mv.n));
     const uint32_t limit = end - base;
     if (tid == 0 && total != limit) {
         atomicOr(mv.faults, kFaultBinCount);
@@ -387,15 +415,38 @@ __device__ __forceinline__ void sorted_segment(const SortedMove& mv, const Table
     }
     __syncthreads();
     uint32_t n_ins = 0;
-    const Window win{a0, a1, c0, c1};
     uint16_t* map = reinterpret_cast<uint16_t*>(hist + S);
     // window indices must fit the map's 16 bits; a window past that moves
     // every agent straight from the rank pass
-    const uint32_t cap = win.size() <= 65536u ? 2u * S : 0u;
-    sorted_rank<U, T, UNIFORM, WRITE_DIR>(mv, t, nc, s0, a0, a1, 0u, hist, map, base, cap, limit, &sh[0], base,
-                                          n_ins);
-    sorted_rank<U, T, UNIFORM, WRITE_DIR>(mv, t, nc, s0, c0, c1, a1 - a0, hist, map, base, cap, limit, &sh[0],
-                                          base, n_ins);
+    const uint32_t cap = wsize <= 65536u ? 2u * S : 0u;
+    if (one_round) {
+#pragma unroll
+        for (uint32_t r = 0; r < R; r++) {
+            const uint32_t idx = r * T + tid;
+            bool moved = false, arr = false;
+            uint32_t nb = 0;
+            if (pk[r] != ~0u) {
+                const uint32_t j = hist[pk[r] >> 16] - base + (pk[r] & 0xffffu);
+                if (j >= limit) {
+                    atomicOr(mv.faults, kFaultBinCount);
+                } else if (j < cap) {
+                    map[j] = static_cast<uint16_t>(idx);
+                } else {
+                    uint4 rec;
+                    float rs;
+                    moved = true;
+                    arr = move_agent<UNIFORM, WRITE_DIR>(mv, t, win.slot(idx), base + j, n_ins, rec, rs, nb);
+                    if (arr) put_arrival<UNIFORM>(mv, base + atomicAdd(&sh[0], 1u), rec, rs);
+                }
+            }
+            next_add(nc, nb, moved && !arr);
+        }
+    } else {
+        sorted_rank<U, T, UNIFORM, WRITE_DIR>(mv, t, nc, s0, a0, a1, 0u, hist, map, base, cap, limit, &sh[0],
+                                              base, n_ins);
+        sorted_rank<U, T, UNIFORM, WRITE_DIR>(mv, t, nc, s0, c0, c1, a1 - a0, hist, map, base, cap, limit,
+                                              &sh[0], base, n_ins);
+    }
     __syncthreads();
     sorted_store<U, T, UNIFORM, WRITE_DIR>(mv, t, nc, win, map, base, min(min(total, cap), limit), &sh[0], base,
                                            n_ins);
@@ -434,20 +485,30 @@ static __global__ __launch_bounds__(T) void k_move_sorted(SortedMove mv, BinDev 
 // First slot of every segment from the per-segment agent counts (the
 // segments of all tables in order): seg_start[g] = cnt[0] + ... + cnt[g - 1],
 // seg_start[n_seg] = the total, which must be n.  cnt is cleared for its next
-// use.  One workgroup: wave w takes the contiguous chunk [w C, (w + 1) C) and
-// walks it 64 entries at a time (coalesced loads and stores), once to sum it
-// and once, after a scan of the chunk sums, to write the offsets.
+// use.  One workgroup: wave w takes the contiguous chunk [w C, (w + 1) C) in
+// pieces of 64 K entries, each loaded at once (coalesced, K loads in flight
+// per lane) and kept in registers for both the chunk sum and the offsets.
 template <uint32_t T>
 static __global__ __launch_bounds__(T) void k_seg_scan(uint32_t* __restrict__ cnt, uint32_t* __restrict__ seg_start,
                                                        uint32_t n_seg, uint32_t n, unsigned int* faults) {
-    constexpr uint32_t W = T / 64;
+    constexpr uint32_t W = T / 64, K = 16;
     __shared__ uint32_t wsum[W];
     const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
     const uint32_t C = (n_seg + W - 1) / W;
     const uint32_t c0 = min(w * C, n_seg), c1 = min(c0 + C, n_seg);
+    const bool one_piece = C <= 64u * K;  // workgroup-uniform
+    uint32_t v[K];
     uint32_t part = 0;
-    for (uint32_t g = c0 + lane; g < c1; g += 64) {
-        part += cnt[g];
+    for (uint32_t p0 = c0; p0 < c1; p0 += 64u * K) {
+#pragma unroll
+        for (uint32_t k = 0; k < K; k++) {
+            const uint32_t g = p0 + 64u * k + lane;
+            v[k] = g < c1 ? cnt[g] : 0u;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < K; k++) {
+            part += v[k];
+        }
     }
     part = wave_sum(part);
     if (lane == 0) wsum[w] = part;
@@ -457,15 +518,24 @@ static __global__ __launch_bounds__(T) void k_seg_scan(uint32_t* __restrict__ cn
         run += (k < w) ? wsum[k] : 0u;
         total += wsum[k];
     }
-    for (uint32_t g0 = c0; g0 < c1; g0 += 64) {
-        const uint32_t g = g0 + lane;
-        const uint32_t v = g < c1 ? cnt[g] : 0u;
-        const uint32_t inc = wave_incl_scan(v);
-        if (g < c1) {
-            seg_start[g] = run + inc - v;
-            cnt[g] = 0u;
+    for (uint32_t p0 = c0; p0 < c1; p0 += 64u * K) {
+        if (!one_piece) {
+#pragma unroll
+            for (uint32_t k = 0; k < K; k++) {
+                const uint32_t g = p0 + 64u * k + lane;
+                v[k] = g < c1 ? cnt[g] : 0u;
+            }
         }
-        run += static_cast<uint32_t>(__shfl(static_cast<int>(inc), 63, 64));
+#pragma unroll
+        for (uint32_t k = 0; k < K; k++) {
+            const uint32_t g = p0 + 64u * k + lane;
+            const uint32_t inc = wave_incl_scan(v[k]);
+            if (g < c1) {
+                seg_start[g] = run + inc - v[k];
+                cnt[g] = 0u;
+            }
+            run += static_cast<uint32_t>(__shfl(static_cast<int>(inc), 63, 64));
+        }
     }
     if (threadIdx.x == 0) {
         seg_start[n_seg] = total;
