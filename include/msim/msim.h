@@ -216,8 +216,11 @@ int msim_destroy(msim_sim* sim);
  * value; pc->tick must be odd (the next dispatch is a move), and only
  * pc->collisionRadius and pc->tick are used.  speed may be NULL (1.4 m/tick for
  * all).  Options: cell_size, reorder_interval (default 20), graph_ticks
- * (default 100), segment_log2, binning (0 or 2), collide_mode, device and
- * watchdog_ms apply; the table layout is chosen per scenario.  msim_batch_last_error(NULL) reports a failed create. */
+ * (default 100), segment_log2, binning (0, 2 or 3), bin_unroll, collide_mode,
+ * device and watchdog_ms apply; each scenario's table size is chosen from
+ * its agent count, and its row width from its extent unless
+ * table_width_log2 is set (then clamped per scenario to [2, table_log2 - 2]).
+ * msim_batch_last_error(NULL) reports a failed create. */
 typedef struct msim_batch msim_batch;
 
 int msim_batch_create(uint32_t n_scen, const uint32_t* road_off, const msim_road* roads,

@@ -51,7 +51,6 @@ void blaunch_sorted(msim_batch* b, const SortedMove& mv, const BatchBin& bb) {
     hipLaunchKernelGGL((k_bmove_sorted<U, kBSortedThreads, UN, W>), dim3(b->n_seg), dim3(kBSortedThreads),
                        bsorted_lds_bytes(b->max_seg_log), b->stream, mv, bb);
     bprobe_post(b);
-    if (b->mark_move) (void)hipEventRecord(b->ev_move, b->stream);
     bprobe_pre(b);
     hipLaunchKernelGGL((k_broute_sorted<UN, W>), dim3(b->n_seg), dim3(kBlock), 0, b->stream, mv, bb);
     bprobe_post(b);
@@ -192,35 +191,6 @@ int benter_sorted(msim_batch* b) {
         if (e_ != hipSuccess) return bfail(b, MSIM_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
     } while (0)
 
-// The captured ticks of the sorted state on two streams: tick t's collision
-// dispatch (stream2) only reads the buffers tick t + 1's move (stream) reads
-// too, so they run side by side; tick t + 2's move overwrites what tick t's
-// collision dispatch read and waits for it.  Collision dispatches stay in
-// order on stream2.  Ends with stream2 joined.
-int bcapture_sorted_ticks(msim_batch* b) {
-    BTRY_HIP(hipEventRecord(b->ev_fork, b->stream));
-    BTRY_HIP(hipStreamWaitEvent(b->stream2, b->ev_fork, 0));
-    int rc = MSIM_OK;
-    for (uint32_t t = 0; t < b->seg_ticks && rc == MSIM_OK; t++) {
-        if (t >= 2) BTRY_HIP(hipStreamWaitEvent(b->stream, b->ev_coll[t & 1], 0));
-        b->mark_move = true;  // ev_move right after the move kernel: the routing runs beside the collision
-        rc = benqueue_move(b, false);
-        b->mark_move = false;
-        if (rc != MSIM_OK) break;
-        BTRY_HIP(hipStreamWaitEvent(b->stream2, b->ev_move, 0));
-        std::swap(b->stream, b->stream2);  // the collision dispatch's launches go to stream2
-        rc = benqueue_collide(b);
-        std::swap(b->stream, b->stream2);
-        if (rc != MSIM_OK) break;
-        BTRY_HIP(hipEventRecord(b->ev_coll[t & 1], b->stream2));
-        b->tick += 2;
-        b->since_reorder++;
-    }
-    BTRY_HIP(hipEventRecord(b->ev_join, b->stream2));
-    BTRY_HIP(hipStreamWaitEvent(b->stream, b->ev_join, 0));
-    return rc;
-}
-
 // Captures seg_ticks ticks (whole reorder periods, each ending with its
 // reorder) from the current buffers, right after a reorder.
 int bbuild_graph(msim_batch* b) {
@@ -231,12 +201,8 @@ int bbuild_graph(msim_batch* b) {
     if (e != hipSuccess) return bfail(b, MSIM_ERR_HIP, "hipStreamBeginCapture: %s", hipGetErrorString(e));
     int rc = MSIM_OK;
     b->capturing = true;  // run segments are always followed by the run's last tick
-    if (b->sorted) {
-        rc = bcapture_sorted_ticks(b);
-    } else {
-        for (uint32_t t = 0; t < b->seg_ticks && rc == MSIM_OK; t++) {
-            rc = btick(b, false);
-        }
+    for (uint32_t t = 0; t < b->seg_ticks && rc == MSIM_OK; t++) {
+        rc = btick(b, false);
     }
     b->capturing = false;
     e = hipStreamEndCapture(b->stream, &graph);
@@ -424,16 +390,6 @@ extern "C" int msim_batch_create(uint32_t n_scen, const uint32_t* road_off, cons
         delete b;
         return bfail(nullptr, MSIM_ERR_HIP, "hipStreamCreate failed");
     }
-    {
-        bool ok = hipStreamCreateWithFlags(&b->stream2, hipStreamNonBlocking) == hipSuccess;
-        for (hipEvent_t* e : {&b->ev_fork, &b->ev_move, &b->ev_join, &b->ev_coll[0], &b->ev_coll[1]}) {
-            ok = ok && hipEventCreateWithFlags(e, hipEventDisableTiming) == hipSuccess;
-        }
-        if (!ok) {
-            msim_batch_destroy(b);
-            return bfail(nullptr, MSIM_ERR_HIP, "stream / event creation failed");
-        }
-    }
     int dev = 0;
     (void)hipGetDevice(&dev);
     int lds_max = 65536;
@@ -456,7 +412,8 @@ extern "C" int msim_batch_create(uint32_t n_scen, const uint32_t* road_off, cons
         const uint32_t tlog = auto_table_log2(d.n);
         const uint32_t rows_log = extent_rows_log2(roads + road_off[s], road_off[s + 1] - road_off[s],
                                                    entities + agent_off[s], d.n, cell);
-        d.lw = auto_width_log2(tlog, rows_log);
+        d.lw = o.table_width_log2 ? std::max(2u, std::min(o.table_width_log2, tlog - 2u))
+                                  : auto_width_log2(tlog, rows_log);
         uint32_t seg_log = auto_segment_log2(tlog, d.lw);
         if (o.segment_log2) seg_log = std::min(std::max(o.segment_log2, 9u), tlog);
         auto lds_need = [&](uint32_t l) {
@@ -821,10 +778,6 @@ extern "C" int msim_batch_destroy(msim_batch* b) {
         for (auto& g : row)
             if (g.exec) (void)hipGraphExecDestroy(g.exec);
     for (void* p : b->allocs) (void)hipFree(p);
-    for (hipEvent_t e : {b->ev_fork, b->ev_move, b->ev_join, b->ev_coll[0], b->ev_coll[1]}) {
-        if (e) (void)hipEventDestroy(e);
-    }
-    if (b->stream2) (void)hipStreamDestroy(b->stream2);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return MSIM_OK;

@@ -411,12 +411,6 @@ struct msim_batch {
         int cur = 0, bcur = 0;
     };
     Segment gexec[2][2] = {};  // [cur][bcur] at the segment's start
-    // captured segments of the sorted state: collision dispatches on stream2
-    // beside the next tick's move (default priority, like stream)
-    hipStream_t stream2 = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_move = nullptr, ev_join = nullptr;
-    hipEvent_t ev_coll[2] = {nullptr, nullptr};
-    bool mark_move = false;  // capture: record ev_move right after the move kernel
     uint32_t watchdog_ms = 600000;
     // msim_batch_time_ticks probe: launch number probe_kernel of a tick is
     // bracketed by probe_ev[0] / probe_ev[1]; -1 = no probe

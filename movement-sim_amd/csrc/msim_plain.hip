@@ -79,7 +79,6 @@ struct msim_sim {
     uint32_t* seg_start = nullptr;          // [n_seg + 1] first slot of each segment (k_seg_scan)
     uint32_t* seg_cnt[2] = {nullptr, nullptr};  // per buffer: agents per segment for the move leaving it
     uint32_t* seg_arr = nullptr;            // [n_seg] arrivals of each segment
-    bool mark_move = false;  // capture: record ev_move right after the move kernel
     int ncount_at = 0;  // buffer whose slot order ncount follows (-1: ncount_caller)
     bool route_inline = false;  // arrivals routed inside k_advance instead of k_route
     bool capturing = false;  // inside build_graph: renumberings skip the output-only arrays
@@ -104,12 +103,6 @@ struct msim_sim {
     Counters* ctr = nullptr;
     unsigned long long* part_stats = nullptr;  // [2][kSlots * kSlotStride] advance statistics
     int stats_par = 0;  // half of part_stats the last move dispatch added to (sorted state: alternates)
-    // Captured segments of the sorted state run each tick's collision dispatch
-    // on stream2, beside the next tick's move on stream (both only read the
-    // buffers the move wrote); default priority, like the handle's stream.
-    hipStream_t stream2 = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_move = nullptr, ev_join = nullptr;
-    hipEvent_t ev_coll[2] = {nullptr, nullptr};
     unsigned long long* part_pairs = nullptr;  // [kSlots * kSlotStride] neighbour sums
     uint32_t* hist = nullptr;
     uint32_t hist_cap = 0;
@@ -599,7 +592,6 @@ void launch_sorted(msim_sim* s, const SortedMove& mv, const BinDev& b) {
     hipLaunchKernelGGL((k_move_sorted<U, kSortedThreads, UN, W>), dim3(s->n_seg), dim3(kSortedThreads),
                        sorted_lds_bytes(s->seg_log), s->stream, mv, b);
     probe_post(s);
-    if (s->mark_move) (void)hipEventRecord(s->ev_move, s->stream);
     probe_pre(s);
     hipLaunchKernelGGL((k_route_sorted<UN, W>), dim3(s->n_seg), dim3(kBlock), 0, s->stream, mv, b);
     probe_post(s);
@@ -711,38 +703,6 @@ int wait_stream(msim_sim* s) {
 
 int after_move(msim_sim* s);
 
-// The captured ticks of the sorted state, two streams: tick t's collision
-// dispatch (stream2) only reads positions of buffer q_t, which neither tick
-// t's route choice (targets, next buckets, road and PRNG state) nor tick
-// t + 1's move (reads q_t) write, so they run side by side; tick t + 2's
-// move writes
-// buffer q_t and waits for tick t's collision dispatch and finish.  The
-// finish of tick t folds the statistics half move t wrote (the halves
-// alternate), so it never races move t + 1.  Collision dispatches stay in
-// order on stream2 (pair slots, history).  Ends with stream2 joined.
-int capture_sorted_ticks(msim_sim* s) {
-    MSIM_TRY_HIP(s, hipEventRecord(s->ev_fork, s->stream));
-    MSIM_TRY_HIP(s, hipStreamWaitEvent(s->stream2, s->ev_fork, 0));
-    int rc = MSIM_OK;
-    for (uint32_t t = 0; t < s->seg_ticks && rc == MSIM_OK; t++) {
-        if (t >= 2) MSIM_TRY_HIP(s, hipStreamWaitEvent(s->stream, s->ev_coll[t & 1], 0));
-        s->mark_move = true;  // ev_move right after the move kernel: the routing runs beside the collision
-        rc = enqueue_move(s, kMoveAll, false);
-        s->mark_move = false;
-        if (rc != MSIM_OK) break;
-        MSIM_TRY_HIP(s, hipStreamWaitEvent(s->stream2, s->ev_move, 0));
-        std::swap(s->stream, s->stream2);  // the collision dispatch's launches go to stream2
-        rc = enqueue_collide(s, false);
-        std::swap(s->stream, s->stream2);
-        if (rc != MSIM_OK) break;
-        MSIM_TRY_HIP(s, hipEventRecord(s->ev_coll[t & 1], s->stream2));
-        rc = after_move(s);
-    }
-    MSIM_TRY_HIP(s, hipEventRecord(s->ev_join, s->stream2));
-    MSIM_TRY_HIP(s, hipStreamWaitEvent(s->stream, s->ev_join, 0));
-    return rc;
-}
-
 // Captures seg_ticks fused ticks starting from the current (cur, bcur, qpar)
 // buffers, right after a reorder: the segment is whole reorder periods, each
 // ending with its reorder, so every replay starts on freshly ordered agents.
@@ -754,14 +714,10 @@ int build_graph(msim_sim* s) {
     MSIM_TRY_HIP(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
     int rc = MSIM_OK;
     s->capturing = true;  // run segments are always followed by the run's last tick
-    if (s->sorted) {
-        rc = capture_sorted_ticks(s);
-    } else {
-        for (uint32_t t = 0; t < s->seg_ticks && rc == MSIM_OK; t++) {
-            rc = enqueue_move(s, kMoveAll, false);
-            if (rc == MSIM_OK) rc = enqueue_collide(s, false);
-            if (rc == MSIM_OK) rc = after_move(s);
-        }
+    for (uint32_t t = 0; t < s->seg_ticks && rc == MSIM_OK; t++) {
+        rc = enqueue_move(s, kMoveAll, false);
+        if (rc == MSIM_OK) rc = enqueue_collide(s, false);
+        if (rc == MSIM_OK) rc = after_move(s);
     }
     s->capturing = false;
     hipError_t e = hipStreamEndCapture(s->stream, &graph);
@@ -956,16 +912,6 @@ extern "C" int msim_create(const msim_road* roads, uint32_t n_roads, const uint3
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
         delete s;
         return fail(nullptr, MSIM_ERR_HIP, "hipStreamCreate failed");
-    }
-    {
-        bool ok = hipStreamCreateWithFlags(&s->stream2, hipStreamNonBlocking) == hipSuccess;
-        for (hipEvent_t* e : {&s->ev_fork, &s->ev_move, &s->ev_join, &s->ev_coll[0], &s->ev_coll[1]}) {
-            ok = ok && hipEventCreateWithFlags(e, hipEventDisableTiming) == hipSuccess;
-        }
-        if (!ok) {
-            msim_destroy(s);
-            return fail(nullptr, MSIM_ERR_HIP, "stream / event creation failed");
-        }
     }
 
     // LDS per workgroup bounds the segment size (4 bytes per bucket counter).
@@ -1509,10 +1455,6 @@ extern "C" int msim_destroy(msim_sim* s) {
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     drop_graphs(s);
     for (void* p : s->allocs) (void)hipFree(p);
-    for (hipEvent_t e : {s->ev_fork, s->ev_move, s->ev_join, s->ev_coll[0], s->ev_coll[1]}) {
-        if (e) (void)hipEventDestroy(e);
-    }
-    if (s->stream2) (void)hipStreamDestroy(s->stream2);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
     return MSIM_OK;

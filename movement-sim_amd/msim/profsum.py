@@ -2,11 +2,18 @@
 
   python3 -m msim.profsum stats DIR
       per-kernel table from DIR/*kernel_stats.csv (--kernel-trace --stats)
-  python3 -m msim.profsum traffic --fetch DIR --write DIR --copy-fetch DIR --copy-write DIR \
+  python3 -m msim.profsum traffic --rdreq DIR --write DIR --copy-rdreq DIR --copy-write DIR \
       --copy-bytes N --agents N [--uniform] --workload TEXT --json OUT
-      per-kernel HBM bytes per launch from two --pmc passes (FETCH_SIZE, WRITE_SIZE; KiB per
-      dispatch), scaled by what the same counters read for movement-sim_amd/lib/copycal's
-      copies of N bytes; writes a JSON table (read by bench.py) and prints a markdown table.
+      per-kernel bytes per launch between the L2s and the fabric from two --pmc passes:
+      TCC_EA0_RDREQ_{32B,64B,128B}_sum (read requests by size) and WRITE_SIZE (KiB), each
+      checked against what the same counters read for movement-sim_amd/lib/copycal's copies of
+      N bytes (and its random 8-byte gathers); writes a JSON table (read by bench.py) and prints
+      a markdown table.
+  python3 -m msim.profsum sq DIR [--kernels REGEX]
+      per-kernel SQ summary from one --pmc pass of SQ_INSTS_VALU, SQ_WAVES,
+      SQ_ACTIVE_INST_VALU, SQ_BUSY_CYCLES, SQ_INSTS_VMEM_RD, SQ_INSTS_LDS: instructions per
+      wave and the VALU issue fraction (4 cycles per wave64 VALU instruction on each of the
+      1024 SIMDs, over the kernel's busy cycles).
 
 (Run with movement-sim_amd/ on sys.path.)
 """
@@ -71,6 +78,19 @@ def counter_means(d, counter):
     return {k: (s / c, c) for k, (s, c) in acc.items()}
 
 
+RDREQ = (("TCC_EA0_RDREQ_32B_sum", 32), ("TCC_EA0_RDREQ_64B_sum", 64), ("TCC_EA0_RDREQ_128B_sum", 128))
+
+
+def read_bytes(d):
+    """{kernel: (mean bytes read per dispatch, dispatches)} from the read-request counters."""
+    acc = {}
+    for counter, size in RDREQ:
+        for k, (v, c) in counter_means(d, counter).items():
+            s, _ = acc.get(k, (0.0, c))
+            acc[k] = (s + v * size, c)
+    return acc
+
+
 def copy_scale(d, counter, copy_bytes):
     m = counter_means(d, counter)
     vals = [m[k][0] for k in ("copy8", "copy16") if k in m]
@@ -80,28 +100,56 @@ def copy_scale(d, counter, copy_bytes):
 
 
 def traffic(a):
-    fs = copy_scale(a.copy_fetch, "FETCH_SIZE", a.copy_bytes)
+    cr = read_bytes(a.copy_rdreq)
+    copies = [cr[k][0] for k in ("copy8", "copy16") if k in cr]
+    if not copies:
+        raise SystemExit(f"no copy8/copy16 dispatches in {a.copy_rdreq}")
+    fs = a.copy_bytes / (sum(copies) / len(copies))
     ws = copy_scale(a.copy_write, "WRITE_SIZE", a.copy_bytes)
-    f = counter_means(a.fetch, "FETCH_SIZE")
+    gather = cr.get("gather8", (0.0, 0))[0] / (a.copy_bytes / 8) if "gather8" in cr else None
+    f = read_bytes(a.rdreq)
     w = counter_means(a.write, "WRITE_SIZE")
     kernels = {}
     for k in sorted(set(f) | set(w)):
         if not k.startswith("k_"):
             continue
-        fb = f.get(k, (0.0, 0))[0] * 1024.0 * fs
+        fb = f.get(k, (0.0, 0))[0] * fs
         wb = w.get(k, (0.0, 0))[0] * 1024.0 * ws
-        kernels[k] = {"fetch_bytes": fb, "write_bytes": wb, "bytes_per_launch": fb + wb,
+        kernels[k] = {"read_bytes": fb, "write_bytes": wb, "bytes_per_launch": fb + wb,
                       "algorithmic_bytes": RL.kernel_bytes(k.split(" ")[0], a.uniform) * a.agents,
                       "dispatches": max(f.get(k, (0, 0))[1], w.get(k, (0, 0))[1])}
-    out = {"workload": a.workload, "fetch_scale": fs, "write_scale": ws, "kernels": kernels}
+    out = {"workload": a.workload, "read_scale": fs, "write_scale": ws, "gather8_bytes_per_element": gather,
+           "kernels": kernels}
     with open(a.json, "w") as fh:
         json.dump(out, fh, indent=1)
-    print(f"FETCH_SIZE x {fs:.4f}, WRITE_SIZE x {ws:.4f}\n")
-    print("| kernel | dispatches | fetch MB | write MB | traffic MB | algorithmic MB |")
+    print(f"read requests x {fs:.4f}, WRITE_SIZE x {ws:.4f}" +
+          (f"; a random 8-byte gather reads {gather:.1f} bytes" if gather else "") + "\n")
+    print("| kernel | dispatches | read MB | write MB | traffic MB | algorithmic MB |")
     print("|---|---|---|---|---|---|")
     for k, v in kernels.items():
-        print(f"| {k} | {v['dispatches']} | {v['fetch_bytes'] / 1e6:.1f} | {v['write_bytes'] / 1e6:.1f} | "
+        print(f"| {k} | {v['dispatches']} | {v['read_bytes'] / 1e6:.1f} | {v['write_bytes'] / 1e6:.1f} | "
               f"{v['bytes_per_launch'] / 1e6:.1f} | {v['algorithmic_bytes'] / 1e6:.1f} |")
+
+
+SQ = ("SQ_INSTS_VALU", "SQ_WAVES", "SQ_ACTIVE_INST_VALU", "SQ_BUSY_CYCLES", "SQ_INSTS_VMEM_RD", "SQ_INSTS_LDS")
+N_SIMD, N_SE = 1024, 32  # MI355X: 256 CUs x 4 SIMDs; 32 shader engines
+
+
+def sq_table(d, pattern):
+    m = {c: counter_means(d, c) for c in SQ}
+    rx = re.compile(pattern)
+    out = ["| kernel | dispatches | waves | VALU / wave | VMEM reads / wave | LDS / wave | busy cycles | "
+           "VALU issue fraction |", "|---|---|---|---|---|---|---|---|"]
+    for k in sorted(m["SQ_WAVES"]):
+        if not k.startswith("k_") or not rx.search(k):
+            continue
+        waves, n = m["SQ_WAVES"][k]
+        busy = m["SQ_BUSY_CYCLES"][k][0] / N_SE
+        valu = m["SQ_INSTS_VALU"][k][0]
+        frac = 4.0 * m["SQ_ACTIVE_INST_VALU"][k][0] / (N_SIMD * busy) if busy else 0.0
+        out.append(f"| {k} | {n} | {waves:.0f} | {valu / waves:.0f} | {m['SQ_INSTS_VMEM_RD'][k][0] / waves:.1f} | "
+                   f"{m['SQ_INSTS_LDS'][k][0] / waves:.1f} | {busy:.0f} | {frac:.2f} |")
+    return "\n".join(out)
 
 
 def main(argv=None):
@@ -110,14 +158,19 @@ def main(argv=None):
     s = sub.add_parser("stats")
     s.add_argument("dir")
     t = sub.add_parser("traffic")
-    for k in ("fetch", "write", "copy_fetch", "copy_write", "workload", "json"):
+    for k in ("rdreq", "write", "copy_rdreq", "copy_write", "workload", "json"):
         t.add_argument("--" + k.replace("_", "-"), required=True)
     t.add_argument("--copy-bytes", type=int, required=True)
     t.add_argument("--agents", type=int, required=True)
     t.add_argument("--uniform", action="store_true")
+    q = sub.add_parser("sq")
+    q.add_argument("dir")
+    q.add_argument("--kernels", default=".")
     a = ap.parse_args(argv)
     if a.cmd == "stats":
         print(stats_table(a.dir))
+    elif a.cmd == "sq":
+        print(sq_table(a.dir, a.kernels))
     else:
         traffic(a)
 

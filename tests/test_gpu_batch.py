@@ -110,7 +110,8 @@ def test_batch_ragged_empty_and_heavy_cells_other_modes(opts):
                                   dict(binning=2), dict(binning=2, reorder_interval=1, graph_ticks=3),
                                   dict(binning=2, segment_log2=9, graph_ticks=1),
                                   dict(collide_mode=1), dict(collide_mode=2, binning=2),
-                                  dict(bin_unroll=4, binning=2), dict(bin_unroll=8, segment_log2=9)])
+                                  dict(bin_unroll=4, binning=2), dict(bin_unroll=8, segment_log2=9),
+                                  dict(table_width_log2=7), dict(binning=2, table_width_log2=4)])
 def test_batch_options_do_not_change_results(opts):
     scs = S.batch_config(5, scale=0.01, seed=5)
     check_against_plain(scs, 64, **opts)

@@ -54,7 +54,9 @@ def test_batch_sorted_many_tiny_scenarios_share_waves(mode):
 
 @pytest.mark.parametrize("opts", [dict(graph_ticks=10), dict(graph_ticks=1), dict(graph_ticks=3, segment_log2=9),
                                   dict(bin_unroll=2), dict(bin_unroll=8), dict(segment_log2=12),
-                                  dict(collide_mode=1), dict(collide_mode=2), dict(reorder_interval=4)])
+                                  dict(collide_mode=1), dict(collide_mode=2), dict(reorder_interval=4),
+                                  dict(table_width_log2=3), dict(table_width_log2=7, segment_log2=11),
+                                  dict(table_width_log2=30)])
 def test_batch_sorted_options_do_not_change_results(opts):
     scs = S.batch_config(5, scale=0.01, seed=5)
     check_against_plain(scs, 64, **SORTED, **opts)
