@@ -451,60 +451,43 @@ __device__ __forceinline__ void seg_publish(unsigned long long* w, unsigned long
     __hip_atomic_store(w, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Decoupled look-back, run by the 64 lanes of one wave (all active):
-// publishes segment g's total and returns `first` plus the totals of
-// segments [g_first, g).  Prefix words hold absolute values (`first`
-// included); segment g_first publishes its prefix at once, so every walk ends
-// there at the latest.  Each round trip reads 64 predecessors' words (lane j
-// reads segment hi - j); the ready words from lane 0 up to the nearest prefix
-// word are summed, or, when a word is not ready yet, those before it, and
-// the walk goes on from there.
-__device__ __forceinline__ uint32_t seg_lookback_wave(unsigned long long* status, uint32_t g, uint32_t g_first,
-                                                      uint32_t first, uint32_t total, uint32_t epoch,
-                                                      unsigned int* faults) {
-    const uint32_t lane = __lane_id();
+// Thread-level: publishes segment g's total and returns `first` plus the
+// totals of segments [g_first, g).  Prefix words hold absolute values
+// (`first` included); segment g_first publishes its prefix at once, so every
+// walk ends there at the latest.
+__device__ __forceinline__ uint32_t seg_lookback(unsigned long long* status, uint32_t g, uint32_t g_first,
+                                                 uint32_t first, uint32_t total, uint32_t epoch,
+                                                 unsigned int* faults) {
     if (g == g_first) {
-        if (lane == 0) seg_publish(&status[g], seg_word(epoch, kSegPrefix, first + total));
+        seg_publish(&status[g], seg_word(epoch, kSegPrefix, first + total));
         return first;
     }
-    if (lane == 0) seg_publish(&status[g], seg_word(epoch, kSegAggregate, total));
+    seg_publish(&status[g], seg_word(epoch, kSegAggregate, total));
     uint32_t excl = 0;
-    uint32_t hi = g - 1u;  // hi >= g_first throughout
+    uint32_t k = g - 1u;
     uint32_t polls = 0;
     for (;;) {
-        const bool in = lane <= hi - g_first;  // lanes [0, min(64, hi - g_first + 1))
-        unsigned long long w = 0;
-        if (in) {
-            w = __hip_atomic_load(&status[hi - lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
+        const unsigned long long w = __hip_atomic_load(&status[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const uint32_t flag = static_cast<uint32_t>(w >> 32) & 3u;
-        const bool ok = in && static_cast<uint32_t>(w >> 34) == epoch && flag != 0u;
-        const unsigned long long inm = __ballot(in);
-        const unsigned long long wait = __ballot(in && !ok);
-        const unsigned long long pm = __ballot(ok && flag == kSegPrefix);
-        const uint32_t n_ok = wait ? static_cast<uint32_t>(__ffsll(static_cast<long long>(wait)) - 1)
-                                   : static_cast<uint32_t>(__popcll(inm));
-        const uint32_t p = pm ? static_cast<uint32_t>(__ffsll(static_cast<long long>(pm)) - 1) : 64u;
-        if (p < n_ok) {  // ready words up to a prefix: done
-            excl += wave_sum(lane <= p ? static_cast<uint32_t>(w) : 0u);
-            break;
-        }
-        if (n_ok > 0) {  // ready aggregates: take them, go on below
-            excl += wave_sum(lane < n_ok ? static_cast<uint32_t>(w) : 0u);
-            if (n_ok > hi - g_first) {  // cannot happen: g_first publishes its prefix directly
-                if (lane == 0) atomicOr(faults, kFaultLookback);
+        if (static_cast<uint32_t>(w >> 34) == epoch && flag != 0u) {
+            excl += static_cast<uint32_t>(w);  // a prefix word is absolute (it includes `first`)
+            if (flag == kSegPrefix) {
                 break;
             }
-            hi -= n_ok;
+            if (k == g_first) {  // cannot happen: g_first publishes its prefix directly
+                atomicOr(faults, kFaultLookback);
+                break;
+            }
+            k--;
             continue;
         }
         if (++polls >= kLookbackPolls) {
-            if (lane == 0) atomicOr(faults, kFaultLookback);
+            atomicOr(faults, kFaultLookback);
             break;
         }
         __builtin_amdgcn_s_sleep(1);
     }
-    if (lane == 0) seg_publish(&status[g], seg_word(epoch, kSegPrefix, excl + total));
+    seg_publish(&status[g], seg_word(epoch, kSegPrefix, excl + total));
     return excl;
 }
 
@@ -552,9 +535,8 @@ __device__ __forceinline__ void bin_fused_segment(const float2* __restrict__ pos
         hist[tid * per + q] = excl;
         excl += v;
     }
-    if (tid < 64) {
-        const uint32_t v = seg_lookback_wave(status, g, g_first, first, total, epoch, faults);
-        if (tid == 0) *sh_base = v;
+    if (tid == 0) {
+        *sh_base = seg_lookback(status, g, g_first, first, total, epoch, faults);
     }
     __syncthreads();
     const uint32_t base = *sh_base;
