@@ -125,16 +125,19 @@ __device__ __forceinline__ void next_add(const NextCount& nc, uint32_t bucket, b
     if (own && static_cast<int>(__lane_id()) == __ffsll(static_cast<long long>(om)) - 1) {
         atomicAdd(&nc.nh[kNextReach], static_cast<uint32_t>(__popcll(om)));
     }
-    if (active && !own) {
-        int d = static_cast<int>(ls) - static_cast<int>(nc.g_local);
-        const int nl = static_cast<int>(nc.n_local);
-        if (2 * d > nl) d -= nl;
-        if (2 * d < -nl) d += nl;
-        if (d >= -kNextReach && d <= kNextReach) {
-            atomicAdd(&nc.nh[d + kNextReach], 1u);
-        } else {
-            atomicAdd(&nc.seg_next[nc.seg_first + ls], 1u);
-        }
+    int d = static_cast<int>(ls) - static_cast<int>(nc.g_local);
+    const int nl = static_cast<int>(nc.n_local);
+    if (2 * d > nl) d -= nl;
+    if (2 * d < -nl) d += nl;
+    const bool near = d >= -kNextReach && d <= kNextReach;
+    // Two separate atomics of different scopes: merged into one on a selected
+    // pointer they would become a flat atomic, whose completion the
+    // compiler can only await with every outstanding load.
+    if (active && !own && near) {
+        __hip_atomic_fetch_add(&nc.nh[d + kNextReach], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    if (active && !own && !near) {
+        atomicAdd(&nc.seg_next[nc.seg_first + ls], 1u);
     }
 }
 
@@ -168,21 +171,39 @@ __device__ __forceinline__ void sorted_count(const uint32_t* __restrict__ nxt, u
     }
 }
 
-// Moves the agent of old slot k to new slot q: loads its hot state, steps
-// it (random_move.comp:841-852) and stores it; for an arrival, what the route
-// choice decides (target, next bucket, heading) is left to k_route_sorted,
-// and rec / rec_spd receive its arrival record.  Returns whether the agent
-// arrived; n_ins counts a cell change.
-template <bool UNIFORM, bool WRITE_DIR>
-__device__ __forceinline__ bool move_agent(const SortedMove& mv, const TableView& t, uint32_t k, uint32_t q,
-                                           uint32_t& n_ins, uint4& rec, float& rec_spd, uint32_t& next_bucket) {
+// Hot state of the agent in one old slot: next bucket, position, target,
+// speed and caller index.
+struct HotAgent {
+    float2 p, tg;
+    float sp;
+    uint32_t h, b;
+};
+
+template <bool UNIFORM>
+__device__ __forceinline__ HotAgent load_agent(const SortedMove& mv, uint32_t k) {
     const SortedBufs& src = mv.src;
+    HotAgent a;
+    a.b = src.nxt[k];
+    a.p = src.pos[k];
+    a.tg = src.tgt[k];
+    a.sp = UNIFORM ? mv.spd_const : src.spd[k];
+    a.h = src.perm[k];
+    return a;
+}
+
+// Steps an agent (random_move.comp:841-852) and stores it at new slot q; for
+// an arrival, what the route choice decides (target, next bucket, heading)
+// is left to k_route_sorted, and rec / rec_spd receive its arrival record.
+// Returns whether the agent arrived; n_ins counts a cell change.
+template <bool UNIFORM, bool WRITE_DIR>
+__device__ __forceinline__ bool step_agent(const SortedMove& mv, const TableView& t, const HotAgent& a, uint32_t q,
+                                           uint32_t& n_ins, uint4& rec, float& rec_spd, uint32_t& next_bucket) {
     const SortedBufs& dst = mv.dst;
-    const uint32_t b = src.nxt[k];
-    const float2 p = src.pos[k];
-    const float2 tg = src.tgt[k];
-    const float sp = UNIFORM ? mv.spd_const : src.spd[k];
-    const uint32_t h = src.perm[k];
+    const uint32_t b = a.b;
+    const float2 p = a.p;
+    const float2 tg = a.tg;
+    const float sp = a.sp;
+    const uint32_t h = a.h;
     const float dx = tg.x - p.x;
     const float dy = tg.y - p.y;
     const float len = vlen(dx, dy);
@@ -210,6 +231,13 @@ __device__ __forceinline__ bool move_agent(const SortedMove& mv, const TableView
     rec = make_uint4(q, h, __float_as_uint(np.x), __float_as_uint(np.y));
     rec_spd = sp;
     return arr;
+}
+
+// Moves the agent of old slot k to new slot q (load_agent, then step_agent).
+template <bool UNIFORM, bool WRITE_DIR>
+__device__ __forceinline__ bool move_agent(const SortedMove& mv, const TableView& t, uint32_t k, uint32_t q,
+                                           uint32_t& n_ins, uint4& rec, float& rec_spd, uint32_t& next_bucket) {
+    return step_agent<UNIFORM, WRITE_DIR>(mv, t, load_agent<UNIFORM>(mv, k), q, n_ins, rec, rec_spd, next_bucket);
 }
 
 // Appends one arrival record at list index i.
@@ -290,8 +318,10 @@ __device__ __forceinline__ void sorted_rank(const SortedMove& mv, const TableVie
 // Store pass over the segment's first min(total, cap) new slots, in slot
 // order: each agent's state is gathered from its old slot (the window index
 // in map), moved and stored, so every wave writes consecutive slots and each
-// output line is written whole.  Arrivals are listed for k_route_sorted (one
-// LDS atomic per wave and round).
+// output line is written whole.  A round's U gathers are all issued before
+// the first agent is stepped (its stores would otherwise hold back the
+// next agent's loads).  Arrivals are listed for k_route_sorted (one LDS
+// atomic per wave and round).
 template <uint32_t U, uint32_t T, bool UNIFORM, bool WRITE_DIR>
 __device__ __forceinline__ void sorted_store(const SortedMove& mv, const TableView& t, const NextCount& nc,
                                              const Window& win, const uint16_t* map, uint32_t base, uint32_t count,
@@ -301,13 +331,19 @@ __device__ __forceinline__ void sorted_store(const SortedMove& mv, const TableVi
         bool arr[U];
         uint4 rec[U];
         float rs[U];
+        HotAgent ag[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) {  // j0 < count: slot count - 1 is mapped
+            const uint32_t j = min(j0 + u * T + threadIdx.x, count - 1u);
+            ag[u] = load_agent<UNIFORM>(mv, win.slot(map[j]));
+        }
 #pragma unroll
         for (uint32_t u = 0; u < U; u++) {
             const uint32_t j = j0 + u * T + threadIdx.x;
             arr[u] = false;
             uint32_t nb = 0;
             if (j < count) {
-                arr[u] = move_agent<UNIFORM, WRITE_DIR>(mv, t, win.slot(map[j]), base + j, n_ins, rec[u], rs[u], nb);
+                arr[u] = step_agent<UNIFORM, WRITE_DIR>(mv, t, ag[u], base + j, n_ins, rec[u], rs[u], nb);
             }
             next_add(nc, nb, j < count && !arr[u]);
         }
@@ -464,10 +500,19 @@ __device__ __forceinline__ void sorted_segment(const SortedMove& mv, const Table
     *total_out = limit;
 }
 
+// Register budget of the sorted move.  A CU holds four of its 512-thread
+// workgroups (8 waves per SIMD) only with at most 64 VGPRs and 80 SGPRs per
+// wave.  The compiler's own occupancy estimate admits more SGPRs than that
+// (measured on gfx950: 512-thread workgroups of 80 SGPRs reside four to a
+// CU, of 82 three), so the SGPRs are capped explicitly, the excess kept in
+// VGPR lanes.  U = 2 fits 64 VGPRs; U >= 4 would spill to scratch there,
+// which costs more than the fourth workgroup gains, so it keeps its VGPRs.
+#define MSIM_SORTED_MOVE_BOUNDS(U, T) __launch_bounds__(T, (U) <= 2 ? 8 : 4) __attribute__((amdgpu_num_sgpr(72)))
+
 // Sorted move of one table (grid: n_seg workgroups; dynamic LDS:
 // sorted_lds_bytes(seg_log)).
 template <uint32_t U, uint32_t T, bool UNIFORM, bool WRITE_DIR>
-static __global__ __launch_bounds__(T) void k_move_sorted(SortedMove mv, BinDev b) {
+static __global__ MSIM_SORTED_MOVE_BOUNDS(U, T) void k_move_sorted(SortedMove mv, BinDev b) {
     extern __shared__ uint32_t hist[];
     __shared__ uint32_t lds[T / 64];
     __shared__ uint32_t nh[2 * kNextReach + 1];
