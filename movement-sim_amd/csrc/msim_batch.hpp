@@ -117,7 +117,7 @@ static __global__ __launch_bounds__(T, 8) void k_bbin_fused(const float2* __rest
 // segment (prefix = the scenario's first slot).  Dynamic LDS: counters for
 // the largest segment.
 template <uint32_t U, uint32_t T, bool UNIFORM, bool WRITE_DIR>
-static __global__ MSIM_SORTED_MOVE_BOUNDS(U, T) void k_bmove_sorted(SortedMove mv, BatchBin b) {
+static __global__ __launch_bounds__(T) void k_bmove_sorted(SortedMove mv, BatchBin b) {
     extern __shared__ uint32_t hist[];
     __shared__ uint32_t lds[T / 64];
     __shared__ uint32_t nh[2 * kNextReach + 1];

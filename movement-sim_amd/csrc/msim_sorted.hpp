@@ -500,19 +500,11 @@ __device__ __forceinline__ void sorted_segment(const SortedMove& mv, const Table
     *total_out = limit;
 }
 
-// Register budget of the sorted move.  A CU holds four of its 512-thread
-// workgroups (8 waves per SIMD) only with at most 64 VGPRs and 80 SGPRs per
-// wave.  The compiler's own occupancy estimate admits more SGPRs than that
-// (measured on gfx950: 512-thread workgroups of 80 SGPRs reside four to a
-// CU, of 82 three), so the SGPRs are capped explicitly, the excess kept in
-// VGPR lanes.  U = 2 fits 64 VGPRs; U >= 4 would spill to scratch there,
-// which costs more than the fourth workgroup gains, so it keeps its VGPRs.
-#define MSIM_SORTED_MOVE_BOUNDS(U, T) __launch_bounds__(T, (U) <= 2 ? 8 : 4) __attribute__((amdgpu_num_sgpr(72)))
 
 // Sorted move of one table (grid: n_seg workgroups; dynamic LDS:
 // sorted_lds_bytes(seg_log)).
 template <uint32_t U, uint32_t T, bool UNIFORM, bool WRITE_DIR>
-static __global__ MSIM_SORTED_MOVE_BOUNDS(U, T) void k_move_sorted(SortedMove mv, BinDev b) {
+static __global__ __launch_bounds__(T) void k_move_sorted(SortedMove mv, BinDev b) {
     extern __shared__ uint32_t hist[];
     __shared__ uint32_t lds[T / 64];
     __shared__ uint32_t nh[2 * kNextReach + 1];

@@ -117,6 +117,39 @@ def test_sorted_two_move_dispatches_in_a_row():
     assert_same(g.state(), o.st, "run after")
 
 
+def test_sorted_collision_dispatches_in_a_row_then_captured_runs():
+    """Collision dispatches without a move between them, moves after them, a
+    run that starts after a move dispatch (each tick a collision then a move
+    dispatch), and captured runs right after repeated collision dispatches."""
+    sc = S.config(1, scale=0.2)
+    g, o = pair(sc, graph_ticks=4, **SORTED)
+    g.run_ticks(3)
+    o.run_ticks(3)
+    t = int(g.pc["tick"])  # the last dispatch was a collision dispatch (odd tick)
+    assert t % 2 == 1
+    # collision, collision, collision, move, collision, collision, move, move,
+    # collision, move
+    for tick in (t + 2, t + 4, t + 6, t + 7, t + 8, t + 10, t + 11, t + 13, t + 14, t + 15):
+        g.dispatch(tick)
+        o.dispatch(tick)
+        assert_same(g.state(), o.st, f"tick {tick}")
+        assert g.pairs()[0] == o.pairs_total
+    g.run_ticks(13)  # after a move dispatch: collision and move dispatches
+    o.run_ticks(13)
+    assert_same(g.state(), o.st, "run after a move dispatch")
+    t = int(g.pc["tick"])
+    assert t % 2 == 0
+    g.dispatch(t + 1)  # two collision dispatches
+    o.dispatch(t + 1)
+    g.dispatch(t + 3)
+    o.dispatch(t + 3)
+    g.run_ticks(9)  # after a collision dispatch: captured segments at once
+    o.run_ticks(9)
+    assert_same(g.state(), o.st, "run after collision dispatches")
+    assert g.pairs()[0] == o.pairs_total
+    g.sync()
+
+
 def test_sorted_entities_and_cells():
     sc = S.config(1, scale=0.1)
     g = msim.Simulation(sc.roads, sc.conns, sc.entities, sc.pc, speed=sc.speed, **SORTED)
