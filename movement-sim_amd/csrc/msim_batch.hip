@@ -36,7 +36,7 @@ inline size_t bsorted_lds_bytes(uint32_t seg_log) { return sorted_lds_bytes(seg_
 
 SortedBufs bsorted_bufs(const msim_batch* b, int k) {
     const BBuffers& x = b->buf[k];
-    return SortedBufs{x.pos, x.tgt, b->uniform ? nullptr : x.spd, x.perm, b->nxt[k], x.dir};
+    return SortedBufs{x.pos, b->rec[k], b->nxt[k], x.dir};
 }
 
 constexpr uint32_t kBScanThreads = 1024;
@@ -167,6 +167,8 @@ int btick(msim_batch* b, bool write_dir) {
 int benter_sorted(msim_batch* b) {
     const dim3 g(bblocks(b->n));
     const BBuffers& x = b->buf[b->cur];
+    hipLaunchKernelGGL(k_pack_rec, g, dim3(kBlock), 0, b->stream, x.tgt, b->uniform ? nullptr : x.spd, x.perm,
+                       b->rec[b->cur], b->n);
     hipLaunchKernelGGL(k_cold_enter, g, dim3(kBlock), 0, b->stream, x.perm, x.road, x.rng, x.tgt, b->roads, b->cold,
                        b->n);
     const dim3 gs((b->n_seg + kBlock - 1) / kBlock);
@@ -267,6 +269,8 @@ int bleave_sorted(msim_batch* b) {
     if (!b->sorted) return MSIM_OK;
     const BBuffers& x = b->buf[b->cur];
     const dim3 g(bblocks(b->n));
+    hipLaunchKernelGGL(k_unpack_rec, g, dim3(kBlock), 0, b->stream, b->rec[b->cur], x.tgt,
+                       b->uniform ? nullptr : x.spd, x.perm, b->n);
     hipLaunchKernelGGL(k_cold_publish, g, dim3(kBlock), 0, b->stream, x.perm, b->cold, x.road, x.rng, b->n);
     hipLaunchKernelGGL(k_bid_identity, g, dim3(kBlock), 0, b->stream, b->bbid[b->bcur], b->n);
     int rc = bcheck(b, "sorted state hand-over");
@@ -462,6 +466,8 @@ extern "C" int msim_batch_create(uint32_t n_scen, const uint32_t* road_off, cons
     if (b->binning == 3) {
         BALLOC(b->nxt[0], n);
         BALLOC(b->nxt[1], n);
+        BALLOC(b->rec[0], n);
+        BALLOC(b->rec[1], n);
         BALLOC(b->cold, 2 * static_cast<size_t>(n));
         BALLOC(b->arr_rec, n);
         if (!b->uniform) BALLOC(b->arr_spd, n);
@@ -656,10 +662,16 @@ extern "C" int msim_batch_read_state(msim_batch* b, uint32_t scen, float* pos, f
     };
     // a scenario's agents stay inside its slot range through every reorder
     dn(p.data(), bb.pos + off, sizeof(float2) * na);
-    dn(t.data(), bb.tgt + off, sizeof(float2) * na);
     dn(d.data(), bb.dir + off, sizeof(float2) * na);
-    // sorted state: road and PRNG state are kept in caller order
+    // sorted state: target and caller index in the slots' records, road and
+    // PRNG state in caller order
     if (b->sorted) {
+        std::vector<uint4> rc4(na);
+        dn(rc4.data(), b->rec[b->cur] + off, sizeof(uint4) * na);
+        for (uint32_t k = 0; k < na; k++) {
+            std::memcpy(&t[k], &rc4[k], sizeof(float2));  // the record's first 8 bytes: the target
+            pm[k] = rc4[k].w;
+        }
         std::vector<uint4> cw(2 * static_cast<size_t>(na));
         dn(cw.data(), b->cold + 2 * static_cast<size_t>(off), sizeof(uint4) * 2 * na);
         for (uint32_t k = 0; k < na; k++) {
@@ -667,11 +679,12 @@ extern "C" int msim_batch_read_state(msim_batch* b, uint32_t scen, float* pos, f
             g[k] = cw[2 * k + 1];
         }
     } else {
+        dn(t.data(), bb.tgt + off, sizeof(float2) * na);
+        dn(pm.data(), bb.perm + off, sizeof(uint32_t) * na);
         dn(r.data(), bb.road + off, sizeof(uint32_t) * na);
         dn(g.data(), bb.rng + off, sizeof(uint4) * na);
     }
     dn(c.data(), bb.ncount + off, sizeof(uint32_t) * na);
-    dn(pm.data(), bb.perm + off, sizeof(uint32_t) * na);
     if (rc != MSIM_OK) return rc;
     const uint32_t road_base = b->road_off_h[scen];
     for (uint32_t k = 0; k < na; k++) {

@@ -188,7 +188,9 @@ static __global__ __launch_bounds__(kBlock) void k_bsorted_next(SortedBufs st, c
             while (i >= agent_off[sid + 1]) sid++;
         }
         const ScenDev sc = scen[sid];
-        const float2 np = next_position(st.pos[i], st.tgt[i], UNIFORM ? spd_const : st.spd[i]);
+        const uint4 r = st.rec[i];
+        const float2 np = next_position(st.pos[i], make_float2(__uint_as_float(r.x), __uint_as_float(r.y)),
+                                        UNIFORM ? spd_const : __uint_as_float(r.z));
         const uint32_t b = bucket_of(cell_coord(np.x, inv_cell), cell_coord(np.y, inv_cell), sc.lw, sc.wmask,
                                      sc.hmask);
         st.nxt[i] = b;
@@ -396,6 +398,7 @@ struct msim_batch {
     // with cur on every move; road and PRNG state stay in packed caller order.
     bool sorted = false;
     uint32_t* nxt[2] = {nullptr, nullptr};  // per buffer: bucket of each slot's next position
+    uint4* rec[2] = {nullptr, nullptr};     // per buffer: each slot's target, speed and caller index
     uint4* cold = nullptr;                  // [2n] packed caller order: cold_word, PRNG state
     uint4* arr_rec = nullptr;               // [n] arrival records, listed per segment
     float* arr_spd = nullptr;               // [n] their speeds (per-agent speeds only)

@@ -72,6 +72,7 @@ struct msim_sim {
     // move; road and PRNG state stay in caller order in cold (cold_word, rng).
     bool sorted = false;
     uint32_t* nxt[2] = {nullptr, nullptr};  // per buffer: bucket of each slot's next position
+    uint4* rec[2] = {nullptr, nullptr};     // per buffer: each slot's target, speed and caller index
     uint4* cold = nullptr;                  // [2n] caller order
     uint32_t* ncount_caller = nullptr;      // [n] neighbour counts kept in caller order
     uint4* arr_rec = nullptr;               // [n] arrival records, listed per segment
@@ -539,7 +540,7 @@ int reorder(msim_sim* s) {
 
 SortedBufs sorted_bufs(const msim_sim* s, int k) {
     const Buffers& b = s->buf[k];
-    return SortedBufs{b.pos, b.tgt, s->uniform ? nullptr : b.spd, b.perm, s->nxt[k], b.dir};
+    return SortedBufs{b.pos, s->rec[k], s->nxt[k], b.dir};
 }
 
 constexpr uint32_t kSortedThreads = 512;
@@ -554,6 +555,8 @@ int enter_sorted(msim_sim* s) {
     if (rc != MSIM_OK) return rc;
     const Buffers& b = s->buf[s->cur];
     const dim3 g(agent_blocks(s->n));
+    hipLaunchKernelGGL(k_pack_rec, g, dim3(kBlock), 0, s->stream, b.tgt, s->uniform ? nullptr : b.spd, b.perm,
+                       s->rec[s->cur], s->n);
     hipLaunchKernelGGL(k_cold_enter, g, dim3(kBlock), 0, s->stream, b.perm, b.road, b.rng, b.tgt, s->roads, s->cold,
                        s->n);
     const GridDev& gd = s->grid;
@@ -614,8 +617,8 @@ int enqueue_move_sorted(msim_sim* s, bool write_dir) {
     const int nb = 1 - s->cur;
     const dim3 g(agent_blocks(s->n));
     if (s->ncount_at == nb) {  // the move overwrites the slot order the last counts follow
-        hipLaunchKernelGGL(k_scatter_perm<uint32_t>, g, dim3(kBlock), 0, s->stream, s->buf[nb].perm,
-                           s->buf[nb].ncount, s->ncount_caller, s->n);
+        hipLaunchKernelGGL(k_scatter_rec<uint32_t>, g, dim3(kBlock), 0, s->stream, s->rec[nb], s->buf[nb].ncount,
+                           s->ncount_caller, s->n);
         s->ncount_at = -1;
     }
     SortedMove mv{};
@@ -659,15 +662,18 @@ int enqueue_move_sorted(msim_sim* s, bool write_dir) {
     return check_launch(s, "sorted move dispatch");
 }
 
-// Makes the classic per-slot arrays of buf[cur] (road, rng, init, ncount)
-// readable for the readbacks; the sorted state stays in force.
+// Makes the classic per-slot arrays of buf[cur] (target, speed, caller index,
+// road, rng, init, ncount) readable for the readbacks; the sorted state stays
+// in force.
 int publish_sorted(msim_sim* s) {
     if (!s->sorted) return MSIM_OK;
     const Buffers& b = s->buf[s->cur];
     const dim3 g(agent_blocks(s->n));
+    hipLaunchKernelGGL(k_unpack_rec, g, dim3(kBlock), 0, s->stream, s->rec[s->cur], b.tgt,
+                       s->uniform ? nullptr : b.spd, b.perm, s->n);
     if (s->ncount_at == 1 - s->cur) {
         const Buffers& o = s->buf[1 - s->cur];
-        hipLaunchKernelGGL(k_scatter_perm<uint32_t>, g, dim3(kBlock), 0, s->stream, o.perm, o.ncount,
+        hipLaunchKernelGGL(k_scatter_rec<uint32_t>, g, dim3(kBlock), 0, s->stream, s->rec[1 - s->cur], o.ncount,
                            s->ncount_caller, s->n);
         s->ncount_at = -1;
     }
@@ -985,6 +991,8 @@ extern "C" int msim_create(const msim_road* roads, uint32_t n_roads, const uint3
     if (s->binning == 3) {
         ALLOC(s->nxt[0], n);
         ALLOC(s->nxt[1], n);
+        ALLOC(s->rec[0], n);
+        ALLOC(s->rec[1], n);
         ALLOC(s->cold, 2 * static_cast<size_t>(n));
         ALLOC(s->ncount_caller, n);
         ALLOC(s->arr_rec, n);
@@ -1251,13 +1259,28 @@ extern "C" int msim_read_pairs(msim_sim* s, uint64_t* total, uint32_t* per_tick,
     return MSIM_OK;
 }
 
+namespace {
+
+// Every agent's cell into out[caller index]; in the sorted state the caller
+// index comes from the slots' records.
+void launch_export_cells(msim_sim* s, int2* out) {
+    const dim3 g(agent_blocks(s->n));
+    if (s->sorted) {
+        hipLaunchKernelGGL(k_export_cells_rec, g, dim3(kBlock), 0, s->stream, s->buf[s->cur].pos, s->rec[s->cur],
+                           s->n, s->grid.inv_cell, out);
+    } else {
+        hipLaunchKernelGGL(k_export_cells, g, dim3(kBlock), 0, s->stream, s->agents(s->cur), s->grid.inv_cell, out);
+    }
+}
+
+}  // namespace
+
 extern "C" int msim_read_cells(msim_sim* s, int32_t* cell_xy) {
     if (!s || (s->n && !cell_xy)) return fail(s, MSIM_ERR_INVALID, "NULL argument");
     if (s->n == 0) return MSIM_OK;
     int2* d = nullptr;
     MSIM_TRY_HIP(s, hipMallocAsync(reinterpret_cast<void**>(&d), sizeof(int2) * s->n, s->stream));
-    hipLaunchKernelGGL(k_export_cells, dim3(agent_blocks(s->n)), dim3(kBlock), 0, s->stream,
-                       s->agents(s->cur), s->grid.inv_cell, d);
+    launch_export_cells(s, d);
     int rc = check_launch(s, "k_export_cells");
     if (rc == MSIM_OK) {
         hipError_t e = hipMemcpyAsync(cell_xy, d, sizeof(int2) * s->n, hipMemcpyDeviceToHost, s->stream);
@@ -1317,8 +1340,7 @@ extern "C" int msim_read_cells_noalloc(msim_sim* s, int32_t* cell_xy) {
     int2* d = nullptr;
     int rc = cells_scratch_ptr(s, &d);
     if (rc != MSIM_OK) return rc;
-    hipLaunchKernelGGL(k_export_cells, dim3(agent_blocks(s->n)), dim3(kBlock), 0, s->stream, s->agents(s->cur),
-                       s->grid.inv_cell, d);
+    launch_export_cells(s, d);
     rc = check_launch(s, "k_export_cells");
     if (rc == MSIM_OK) {
         const hipError_t e = hipMemcpyAsync(cell_xy, d, sizeof(int2) * s->n, hipMemcpyDeviceToHost, s->stream);

@@ -70,15 +70,22 @@ __device__ __forceinline__ float2 next_position(float2 p, float2 t, float s) {
     return t;
 }
 
-// Hot agent state in slot order (one copy per binning buffer).
+// Hot agent state in slot order (one copy per binning buffer).  Target,
+// speed and caller index share one 16-byte record, so that moving an agent
+// takes three gathers and three stores (position, record, next bucket):
+// each vector-memory instruction of the move's scattered gathers costs about
+// as much as its bytes.  Positions and next buckets stay separate arrays,
+// which the neighbour pass and the count passes stream alone.
 struct SortedBufs {
-    float2* pos;     // [n + pad]
-    float2* tgt;     // [n + pad]
-    float* spd;      // [n + pad]; nullptr when every agent moves at spd_const
-    uint32_t* perm;  // [n] caller index of the slot's agent
-    uint32_t* nxt;   // [n] bucket (in the agent's table) of its next position
-    float2* dir;     // [n + pad] heading, written on a batch's last tick only
+    float2* pos;    // [n + pad]
+    uint4* rec;     // [n] {target x, target y, speed (float bits; unused with spd_const), caller index}
+    uint32_t* nxt;  // [n] bucket (in the agent's table) of its next position
+    float2* dir;    // [n + pad] heading, written on a batch's last tick only
 };
+
+__device__ __forceinline__ uint4 hot_record(float2 tgt, float spd, uint32_t caller) {
+    return make_uint4(__float_as_uint(tgt.x), __float_as_uint(tgt.y), __float_as_uint(spd), caller);
+}
 
 // Caller-order route state of one agent (cold[2 h], cold[2 h + 1]): road,
 // connection index and count of the endpoint it heads to (route_endpoint),
@@ -185,9 +192,10 @@ __device__ __forceinline__ HotAgent load_agent(const SortedMove& mv, uint32_t k)
     HotAgent a;
     a.b = src.nxt[k];
     a.p = src.pos[k];
-    a.tg = src.tgt[k];
-    a.sp = UNIFORM ? mv.spd_const : src.spd[k];
-    a.h = src.perm[k];
+    const uint4 r = src.rec[k];
+    a.tg = make_float2(__uint_as_float(r.x), __uint_as_float(r.y));
+    a.sp = UNIFORM ? mv.spd_const : __uint_as_float(r.z);
+    a.h = r.w;
     return a;
 }
 
@@ -220,11 +228,9 @@ __device__ __forceinline__ bool step_agent(const SortedMove& mv, const TableView
     }
     n_ins += (cx != cell_coord(p.x, t.inv_cell) || cy != cell_coord(p.y, t.inv_cell)) ? 1u : 0u;
     dst.pos[q] = np;
-    if (!UNIFORM) dst.spd[q] = sp;
-    dst.perm[q] = h;
+    dst.rec[q] = hot_record(tg, sp, h);  // an arrival's target is replaced by k_route_sorted
     if (!arr) {
         next_bucket = bucket_of_pos(next_position(np, tg, sp), t);
-        dst.tgt[q] = tg;
         dst.nxt[q] = next_bucket;
         if (WRITE_DIR) dst.dir[q] = d;
     }
@@ -614,7 +620,7 @@ __device__ __forceinline__ void route_segment_list(const SortedMove& mv, const T
             mv.cold[2u * hh] = cold_word(r, c);
             mv.cold[2u * hh + 1u] = st;
             nb = bucket_of_pos(next_position(at, nt, s), t);
-            dst.tgt[q] = nt;
+            *reinterpret_cast<float2*>(&dst.rec[q]) = nt;  // the record's target
             dst.nxt[q] = nb;
             if (WRITE_DIR) dst.dir[q] = heading(nt, at, s);
         }
@@ -658,7 +664,9 @@ static __global__ __launch_bounds__(kBlock) void k_sorted_next(SortedBufs st, ui
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     uint32_t b = 0;
     if (i < n) {
-        const float2 np = next_position(st.pos[i], st.tgt[i], UNIFORM ? spd_const : st.spd[i]);
+        const uint4 r = st.rec[i];
+        const float2 np = next_position(st.pos[i], make_float2(__uint_as_float(r.x), __uint_as_float(r.y)),
+                                        UNIFORM ? spd_const : __uint_as_float(r.z));
         b = bucket_of(cell_coord(np.x, inv_cell), cell_coord(np.y, inv_cell), lw, wmask, hmask);
         st.nxt[i] = b;
     }
@@ -725,6 +733,55 @@ static __global__ __launch_bounds__(kBlock) void k_gather_perm(const uint32_t* _
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i < n) {
         dst[i] = src[perm[i]];
+    }
+}
+
+// dst[caller index of slot i] = src[i], the caller index from the slot's record.
+template <class V>
+static __global__ __launch_bounds__(kBlock) void k_scatter_rec(const uint4* __restrict__ rec,
+                                                               const V* __restrict__ src, V* __restrict__ dst,
+                                                               uint32_t n) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) {
+        dst[rec[i].w] = src[i];
+    }
+}
+
+// Cells of the sorted state's agents in caller order (the caller index from
+// each slot's record): k_export_cells for the sorted state.
+static __global__ __launch_bounds__(kBlock) void k_export_cells_rec(const float2* __restrict__ pos,
+                                                                    const uint4* __restrict__ rec, uint32_t n,
+                                                                    float inv_cell, int2* __restrict__ out) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) {
+        const float2 p = pos[i];
+        out[rec[i].w] = make_int2(cell_coord(p.x, inv_cell), cell_coord(p.y, inv_cell));
+    }
+}
+
+// Records from the classic per-slot arrays (entering the sorted state); spd
+// nullptr: every agent moves at the constant speed.
+static __global__ __launch_bounds__(kBlock) void k_pack_rec(const float2* __restrict__ tgt,
+                                                            const float* __restrict__ spd,
+                                                            const uint32_t* __restrict__ perm,
+                                                            uint4* __restrict__ rec, uint32_t n) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) {
+        rec[i] = hot_record(tgt[i], spd ? spd[i] : 0.0f, perm[i]);
+    }
+}
+
+// The classic per-slot arrays from the records (readbacks, leaving the
+// sorted state).
+static __global__ __launch_bounds__(kBlock) void k_unpack_rec(const uint4* __restrict__ rec,
+                                                              float2* __restrict__ tgt, float* __restrict__ spd,
+                                                              uint32_t* __restrict__ perm, uint32_t n) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) {
+        const uint4 r = rec[i];
+        tgt[i] = make_float2(__uint_as_float(r.x), __uint_as_float(r.y));
+        if (spd) spd[i] = __uint_as_float(r.z);
+        perm[i] = r.w;
     }
 }
 
