@@ -186,11 +186,13 @@ struct HotAgent {
     uint32_t h, b;
 };
 
+// The agent of old slot k, whose next bucket (counted for its placement) is
+// `bucket`.
 template <bool UNIFORM>
-__device__ __forceinline__ HotAgent load_agent(const SortedMove& mv, uint32_t k) {
+__device__ __forceinline__ HotAgent load_agent(const SortedMove& mv, uint32_t k, uint32_t bucket) {
     const SortedBufs& src = mv.src;
     HotAgent a;
-    a.b = src.nxt[k];
+    a.b = bucket;
     a.p = src.pos[k];
     const uint4 r = src.rec[k];
     a.tg = make_float2(__uint_as_float(r.x), __uint_as_float(r.y));
@@ -239,11 +241,14 @@ __device__ __forceinline__ bool step_agent(const SortedMove& mv, const TableView
     return arr;
 }
 
-// Moves the agent of old slot k to new slot q (load_agent, then step_agent).
+// Moves the agent of old slot k, counted in `bucket`, to new slot q
+// (load_agent, then step_agent).
 template <bool UNIFORM, bool WRITE_DIR>
-__device__ __forceinline__ bool move_agent(const SortedMove& mv, const TableView& t, uint32_t k, uint32_t q,
-                                           uint32_t& n_ins, uint4& rec, float& rec_spd, uint32_t& next_bucket) {
-    return step_agent<UNIFORM, WRITE_DIR>(mv, t, load_agent<UNIFORM>(mv, k), q, n_ins, rec, rec_spd, next_bucket);
+__device__ __forceinline__ bool move_agent(const SortedMove& mv, const TableView& t, uint32_t k, uint32_t bucket,
+                                           uint32_t q, uint32_t& n_ins, uint4& rec, float& rec_spd,
+                                           uint32_t& next_bucket) {
+    return step_agent<UNIFORM, WRITE_DIR>(mv, t, load_agent<UNIFORM>(mv, k, bucket), q, n_ins, rec, rec_spd,
+                                          next_bucket);
 }
 
 // Appends one arrival record at list index i.
@@ -263,7 +268,8 @@ struct Window {
 // Rank pass over window slots [lo, hi) (window-relative index rel0 for lo):
 // each agent whose next bucket lies in the segment takes cursor[bucket]++,
 // its new slot.  The segment's output position j = slot - base records the
-// agent's window index in map[j] when j < cap; past cap (a segment more
+// agent's window index in map[j] and its bucket in the segment in mapb[j]
+// when j < cap; past cap (a segment more
 // crowded than the map holds) the agent is moved and stored at once.  A
 // position at or past limit (the slots the segment was given) is a count
 // mismatch: the agent is not stored and kFaultBinCount is raised.  Reads
@@ -273,7 +279,8 @@ struct Window {
 template <uint32_t U, uint32_t T, bool UNIFORM, bool WRITE_DIR>
 __device__ __forceinline__ void sorted_rank(const SortedMove& mv, const TableView& t, const NextCount& nc,
                                             uint32_t s0, uint32_t lo, uint32_t hi, uint32_t rel0, uint32_t* cursor,
-                                            uint16_t* map, uint32_t base, uint32_t cap, uint32_t limit,
+                                            uint16_t* map, uint16_t* mapb, uint32_t base, uint32_t cap,
+                                            uint32_t limit,
                                             uint32_t* arr_count, uint32_t arr_base, uint32_t& n_ins) {
     if (lo >= hi) {
         return;
@@ -310,9 +317,10 @@ __device__ __forceinline__ void sorted_rank(const SortedMove& mv, const TableVie
                     atomicOr(mv.faults, kFaultBinCount);
                 } else if (j < cap) {
                     map[j] = static_cast<uint16_t>(rel0 + (k - lo));
+                    mapb[j] = static_cast<uint16_t>(b[u] - s0);
                 } else {
                     moved = true;
-                    arr = move_agent<UNIFORM, WRITE_DIR>(mv, t, k, base + j, n_ins, rec, rs, nb);
+                    arr = move_agent<UNIFORM, WRITE_DIR>(mv, t, k, b[u], base + j, n_ins, rec, rs, nb);
                     if (arr) put_arrival<UNIFORM>(mv, arr_base + atomicAdd(arr_count, 1u), rec, rs);
                 }
             }
@@ -323,14 +331,15 @@ __device__ __forceinline__ void sorted_rank(const SortedMove& mv, const TableVie
 
 // Store pass over the segment's first min(total, cap) new slots, in slot
 // order: each agent's state is gathered from its old slot (the window index
-// in map), moved and stored, so every wave writes consecutive slots and each
+// in map; its bucket, s0 + mapb, needs no gather), moved and stored, so every wave writes consecutive slots and each
 // output line is written whole.  A round's U gathers are all issued before
 // the first agent is stepped (its stores would otherwise hold back the
 // next agent's loads).  Arrivals are listed for k_route_sorted (one LDS
 // atomic per wave and round).
 template <uint32_t U, uint32_t T, bool UNIFORM, bool WRITE_DIR>
 __device__ __forceinline__ void sorted_store(const SortedMove& mv, const TableView& t, const NextCount& nc,
-                                             const Window& win, const uint16_t* map, uint32_t base, uint32_t count,
+                                             const Window& win, const uint16_t* map, const uint16_t* mapb,
+                                             uint32_t s0, uint32_t base, uint32_t count,
                                              uint32_t* arr_count, uint32_t arr_base, uint32_t& n_ins) {
     const unsigned long long lt = (1ull << __lane_id()) - 1ull;
     for (uint32_t j0 = 0; j0 < count; j0 += U * T) {
@@ -341,7 +350,7 @@ __device__ __forceinline__ void sorted_store(const SortedMove& mv, const TableVi
 #pragma unroll
         for (uint32_t u = 0; u < U; u++) {  // j0 < count: slot count - 1 is mapped
             const uint32_t j = min(j0 + u * T + threadIdx.x, count - 1u);
-            ag[u] = load_agent<UNIFORM>(mv, win.slot(map[j]));
+            ag[u] = load_agent<UNIFORM>(mv, win.slot(map[j]), s0 + mapb[j]);
         }
 #pragma unroll
         for (uint32_t u = 0; u < U; u++) {
@@ -375,9 +384,16 @@ __device__ __forceinline__ void sorted_store(const SortedMove& mv, const TableVi
     }
 }
 
-// LDS of the sorted move: the segment's counters (4 bytes per bucket) and the
-// new-slot -> window-index map (2 bytes per slot, 2 slots per bucket).
-__host__ __device__ constexpr size_t sorted_lds_bytes(uint32_t seg_log) { return size_t{8} << seg_log; }
+// LDS of the sorted move: the segment's counters (4 bytes per bucket) and,
+// for its first sorted_map_cap new slots, each agent's window index and its
+// bucket in the segment (2 + 2 bytes).  Segments of up to 2^12 buckets map 2
+// slots per bucket (12 bytes per bucket in all), larger ones 1 (8 bytes).
+__host__ __device__ constexpr uint32_t sorted_map_cap(uint32_t seg_log) {
+    return seg_log <= 12u ? (2u << seg_log) : (1u << seg_log);
+}
+__host__ __device__ constexpr size_t sorted_lds_bytes(uint32_t seg_log) {
+    return (size_t{4} << seg_log) + size_t{4} * sorted_map_cap(seg_log);
+}
 
 // Count and place segment g (global index; s0 its first bucket in table t).
 // Writes the segment's absolute bucket offsets to start_out[s0 + j];
@@ -458,9 +474,10 @@ __device__ __forceinline__ void sorted_segment(const SortedMove& mv, const Table
     __syncthreads();
     uint32_t n_ins = 0;
     uint16_t* map = reinterpret_cast<uint16_t*>(hist + S);
+    uint16_t* mapb = map + sorted_map_cap(t.seg_log);
     // window indices must fit the map's 16 bits; a window past that moves
     // every agent straight from the rank pass
-    const uint32_t cap = wsize <= 65536u ? 2u * S : 0u;
+    const uint32_t cap = wsize <= 65536u ? sorted_map_cap(t.seg_log) : 0u;
     if (one_round) {
 #pragma unroll
         for (uint32_t r = 0; r < R; r++) {
@@ -473,25 +490,27 @@ __device__ __forceinline__ void sorted_segment(const SortedMove& mv, const Table
                     atomicOr(mv.faults, kFaultBinCount);
                 } else if (j < cap) {
                     map[j] = static_cast<uint16_t>(idx);
+                    mapb[j] = static_cast<uint16_t>(pk[r] >> 16);
                 } else {
                     uint4 rec;
                     float rs;
                     moved = true;
-                    arr = move_agent<UNIFORM, WRITE_DIR>(mv, t, win.slot(idx), base + j, n_ins, rec, rs, nb);
+                    arr = move_agent<UNIFORM, WRITE_DIR>(mv, t, win.slot(idx), s0 + (pk[r] >> 16), base + j, n_ins,
+                                                         rec, rs, nb);
                     if (arr) put_arrival<UNIFORM>(mv, base + atomicAdd(&sh[0], 1u), rec, rs);
                 }
             }
             next_add(nc, nb, moved && !arr);
         }
     } else {
-        sorted_rank<U, T, UNIFORM, WRITE_DIR>(mv, t, nc, s0, a0, a1, 0u, hist, map, base, cap, limit, &sh[0],
-                                              base, n_ins);
-        sorted_rank<U, T, UNIFORM, WRITE_DIR>(mv, t, nc, s0, c0, c1, a1 - a0, hist, map, base, cap, limit,
+        sorted_rank<U, T, UNIFORM, WRITE_DIR>(mv, t, nc, s0, a0, a1, 0u, hist, map, mapb, base, cap, limit,
+                                              &sh[0], base, n_ins);
+        sorted_rank<U, T, UNIFORM, WRITE_DIR>(mv, t, nc, s0, c0, c1, a1 - a0, hist, map, mapb, base, cap, limit,
                                               &sh[0], base, n_ins);
     }
     __syncthreads();
-    sorted_store<U, T, UNIFORM, WRITE_DIR>(mv, t, nc, win, map, base, min(min(total, cap), limit), &sh[0], base,
-                                           n_ins);
+    sorted_store<U, T, UNIFORM, WRITE_DIR>(mv, t, nc, win, map, mapb, s0, base, min(min(total, cap), limit),
+                                           &sh[0], base, n_ins);
     __syncthreads();
     next_flush(nc);
     // block totals of insertions and arrivals, one slot update each
