@@ -468,10 +468,14 @@ __device__ __forceinline__ void sorted_segment(const SortedMove& mv, const Table
     for (uint32_t q = 0; q < per; q++) {  // counts -> absolute bucket offsets
         const uint32_t v = hist[tid * per + q];
         hist[tid * per + q] = base + excl;
-        start_out[s0 + tid * per + q] = base + min(excl, limit);
         excl += v;
     }
     __syncthreads();
+    // the offsets into the new bucket table, clamped to the segment's slots,
+    // stored in bucket order (whole lines per wave)
+    for (uint32_t j = tid; j < S; j += T) {
+        start_out[s0 + j] = min(hist[j], base + limit);
+    }
     uint32_t n_ins = 0;
     uint16_t* map = reinterpret_cast<uint16_t*>(hist + S);
     uint16_t* mapb = map + sorted_map_cap(t.seg_log);
@@ -503,6 +507,7 @@ __device__ __forceinline__ void sorted_segment(const SortedMove& mv, const Table
             next_add(nc, nb, moved && !arr);
         }
     } else {
+        __syncthreads();  // the rank pass advances the offsets the stores above read
         sorted_rank<U, T, UNIFORM, WRITE_DIR>(mv, t, nc, s0, a0, a1, 0u, hist, map, mapb, base, cap, limit,
                                               &sh[0], base, n_ins);
         sorted_rank<U, T, UNIFORM, WRITE_DIR>(mv, t, nc, s0, c0, c1, a1 - a0, hist, map, mapb, base, cap, limit,
