@@ -399,10 +399,14 @@ __device__ __forceinline__ uint32_t bin_count_segment(const float2* __restrict__
     }
     uint32_t total = 0;
     uint32_t excl = block_excl_scan<T>(run, lds, total);
-    for (uint32_t q = 0; q < per; q++) {
+    for (uint32_t q = 0; q < per; q++) {  // counts -> segment-local exclusive offsets
         const uint32_t v = hist[tid * per + q];
-        start_out[s0 + tid * per + q] = excl;
+        hist[tid * per + q] = excl;
         excl += v;
+    }
+    __syncthreads();
+    for (uint32_t j = tid; j < S; j += T) {  // stored in bucket order (whole lines per wave)
+        start_out[s0 + j] = hist[j];
     }
     return total;
 }
