@@ -1,8 +1,8 @@
 // msim_sorted.hpp — the move dispatch on bucket-sorted agent state.
 //
-// The hot agent state (position, target, speed, caller index) is kept in the
-// slot order of the current binning, so the binning needs no slot -> agent
-// map and positions need no gather.  One launch per move dispatch advances
+// The hot agent state (position; target, speed and caller index in one
+// 16-byte record) is kept in the slot order of the current binning, so the
+// binning needs no slot -> agent map and positions need no gather.  One launch per move dispatch advances
 // every agent (random_move.comp:830-852) and writes it, together with the
 // rest of its hot state, straight into the next binning's slot order:
 //
@@ -20,10 +20,13 @@
 //     move and route choice counted, per segment, the agents whose next
 //     bucket lies in it, and k_seg_scan turned those counts into first slots,
 //     so no segment waits for another;
-//   * place pass: the window slots whose `nxt` lies in the segment load
-//     their state (coalesced: the window is a contiguous slot range), move,
-//     and are stored at their new slots with the bucket of their following
-//     move as the new `nxt`;
+//   * rank: the window slots whose `nxt` lies in the segment take their new
+//     slots from LDS counters; an LDS map records, per new slot, the old
+//     slot's window index and the counted bucket;
+//   * store pass: the new slots in order (whole lines written per wave),
+//     each agent's position and record gathered from its old slot through
+//     the map, moved, checked against its counted bucket and stored with the
+//     bucket of its following move as the new `nxt`;
 //   * arrivals are stored at once but for what the route choice decides
 //     (target, next bucket, heading): each is listed (new slot, caller index,
 //     position, speed) in the segment's own slice [base, base + total) of
@@ -33,7 +36,7 @@
 //     of the endpoint the agent heads to, and are touched only there.
 //
 // Each agent's arithmetic is the plain tick's exactly; only where the state
-// is stored changes.  The agents' order inside a bucket follows the place
+// is stored changes.  The agents' order inside a bucket follows the rank
 // pass's LDS atomics and carries no meaning: no result depends on it.
 #pragma once
 
