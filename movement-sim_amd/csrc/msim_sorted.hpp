@@ -429,7 +429,9 @@ __device__ __forceinline__ void sorted_segment(const SortedMove& mv, const Table
     // atomics also give each agent its rank in its bucket (kept in
     // registers, so no rank pass is needed); larger windows are counted,
     // then ranked, in rounds.
-    constexpr uint32_t R = 2 * U;
+    // (U = 4: 12 slots per thread, 6,144 per round, so that more windows
+    // take one round; the registers stay within the same occupancy)
+    constexpr uint32_t R = U == 4 ? 12 : 2 * U;
     const bool one_round = wsize <= R * T;  // workgroup-uniform
     // (bucket offset << 16) | rank in the bucket, ~0 when not the segment's
     // agent (segments have at most 2^14 buckets, windows here R T <= 2^13 slots)
