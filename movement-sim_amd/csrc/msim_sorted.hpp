@@ -743,17 +743,6 @@ static __global__ __launch_bounds__(kBlock) void k_cold_publish(const uint32_t* 
     rng[i] = cold[2u * h + 1u];
 }
 
-// dst[perm[i]] = src[i]
-template <class V>
-static __global__ __launch_bounds__(kBlock) void k_scatter_perm(const uint32_t* __restrict__ perm,
-                                                                const V* __restrict__ src, V* __restrict__ dst,
-                                                                uint32_t n) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i < n) {
-        dst[perm[i]] = src[i];
-    }
-}
-
 // dst[i] = src[perm[i]]
 template <class V>
 static __global__ __launch_bounds__(kBlock) void k_gather_perm(const uint32_t* __restrict__ perm,
