@@ -24,3 +24,30 @@ def test_descent_per_layout_finds_a_layout_that_needs_its_own_segment_size():
 def test_batch_space_covers_the_sorted_state():
     assert 3 in autotune.BATCH_SPACE["binning"] and 2 in autotune.BATCH_SPACE["bin_unroll"]
     assert 3 in autotune.SPACE["binning"] and 2 in autotune.SPACE["bin_unroll"]
+
+
+class _Handle:
+    """Records run_ticks / sync / time_ticks calls in order."""
+
+    def __init__(self):
+        self.calls = []
+
+    def run_ticks(self, n):
+        self.calls.append(("run", n))
+
+    def sync(self):
+        self.calls.append(("sync",))
+
+    def time_ticks(self, n, kernels=False):
+        self.calls.append(("time", n))
+        return 1.0, {}
+
+
+def test_timed_run_replays_graphs_captured_before_it():
+    # the warm-up, then one untimed run of the timed length rounded up to an
+    # even number of ticks (it ends on the buffers it started from), so the
+    # graph segments the timed run replays are captured outside it
+    for ticks, primed in ((100, 100), (101, 102), (200, 200)):
+        h = _Handle()
+        autotune._prime(h, 20, ticks)
+        assert h.calls == [("run", 20), ("run", primed), ("sync",)]
