@@ -75,10 +75,11 @@ __device__ __forceinline__ float2 next_position(float2 p, float2 t, float s) {
 
 // Hot agent state in slot order (one copy per binning buffer).  Target,
 // speed and caller index share one 16-byte record, so that moving an agent
-// takes three gathers and three stores (position, record, next bucket):
-// each vector-memory instruction of the move's scattered gathers costs about
-// as much as its bytes.  Positions and next buckets stay separate arrays,
-// which the neighbour pass and the count passes stream alone.
+// takes two gathers (position, record) and three stores (position, record,
+// next bucket): a scattered gather costs about the same per instruction
+// whatever its width, so fewer, wider ones are cheaper.  Positions and next
+// buckets stay separate arrays, which the neighbour pass and the count
+// passes stream alone.
 struct SortedBufs {
     float2* pos;    // [n + pad]
     uint4* rec;     // [n] {target x, target y, speed (float bits; unused with spd_const), caller index}
